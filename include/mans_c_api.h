@@ -1,0 +1,176 @@
+/*
+ * mans_c_api.h -- C ABI of the MI355X tANS decoder library (libmultians_amd.so).
+ *
+ * Plain pointers and sizes only; no HIP or torch types.  Device pointers are
+ * passed as void* / typed pointers into device memory; `stream` is a hipStream_t
+ * passed as void* (NULL = the default stream).  Every entry point returns 0 on
+ * success or a negative MANS_E* code; mans_last_error() describes the last failure
+ * on the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   mans_decode            cuhd::CUHDGPUDecoder::decode
+ *                          (include/cuhd_gpu_decoder.h:24-35, src/multians_gpu_decoder.cu:575-687)
+ *   mans_workspace_bytes   sizing done by cuhd::CUHDGPUDecoderMemory's constructor
+ *                          (src/multians_gpu_decoder_memory.cc:13-19)
+ *   mans_malloc/mans_free  CUHDGPUMemoryBuffer<T>::allocate/free (zero-filled allocate,
+ *                          include/cuhd_gpu_memory_buffer.hxx:35-49)
+ *   mans_memcpy_h2d/d2h    CUHDGPUMemoryBuffer<T>::cpy_host_to_device / cpy_device_to_host
+ *                          (include/cuhd_gpu_memory_buffer.hxx:52-67)
+ *   mans_host_alloc_pinned cudaMallocHost in CUHDGPUDecoderMemory::allocate
+ *                          (src/multians_gpu_decoder_memory.cc:45-47)
+ *   mans_set_device        cudaSetDevice (src/main.cc:200)
+ *   mans_error_string      the message CUERR prints (include/cuhd_cuda_definitions.h:21-28)
+ *   mans_build_tables,     ANSTableGenerator::generate_table / generate_encoder_table /
+ *   mans_encode            get_decoder_table and ANSEncoder::encode
+ *                          (src/ans_table_generator.cc:164-288, src/ans_encoder.cc:13-97):
+ *                          host-side producers of the decoder's inputs
+ *   mans_decode_batch      no single counterpart: decodes many independent streams (each
+ *                          with its own table) in one launch sequence
+ */
+#ifndef MANS_C_API_H_
+#define MANS_C_API_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+#define MANS_OK 0
+#define MANS_EINVAL (-1)      /* bad argument (null pointer, size, L, ...)            */
+#define MANS_EHIP (-2)        /* HIP runtime error                                    */
+#define MANS_ENOMEM (-3)      /* allocation failed / workspace too small              */
+#define MANS_ETABLE (-4)      /* decoder table rejected (see mans_decode)             */
+#define MANS_ECORRUPT (-5)    /* stream inconsistent with table/sizes                 */
+#define MANS_EUNSUPPORTED (-6)
+
+const char* mans_error_string(int code);
+/* Message for the last failing call on this thread (never NULL). */
+const char* mans_last_error(void);
+
+/* ---------------------------------------------------- device / memory / copy */
+int mans_set_device(int device);
+int mans_get_device_count(int* count);
+/* Zero-filled device allocation (like CUHDGPUMemoryBuffer<T>::allocate). */
+int mans_malloc(void** ptr, size_t bytes);
+int mans_free(void* ptr);
+int mans_memset(void* dst, int value, size_t bytes, void* stream);
+int mans_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int mans_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int mans_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+int mans_host_alloc_pinned(void** ptr, size_t bytes);
+int mans_host_free_pinned(void* ptr);
+int mans_stream_create(void** stream);
+int mans_stream_destroy(void* stream);
+int mans_sync(void* stream);            /* hipStreamSynchronize (NULL: device) */
+int mans_event_create(void** ev);
+int mans_event_destroy(void* ev);
+int mans_event_record(void* ev, void* stream);
+int mans_event_elapsed_ms(void* start, void* stop, float* ms); /* synchronises stop */
+
+/* ------------------------------------------------------------- single stream */
+/*
+ * Stream format (the reference's CUHDInputBuffer, src/multians_input_buffer.cc:15-30):
+ *   units[0..n_units)  u32 in DECODER order (reversed encoder emission); the decoder
+ *                      reads zeros beyond n_units, so no padding is required (the
+ *                      reference's +4 zero units are accepted and ignored).
+ *   first_bit          valid bits in units[0] (left-aligned); decoding starts at bit
+ *                      32 - first_bit of units[0], bits are taken LSB-first.
+ *   first_state        initial state, in [L, 2L).
+ * Table: L u32 entries, reference CUHDCodetableItem layout
+ *   (include/cuhd_codetable.h:20-24): bits 0-15 next_state, 16-23 symbol,
+ *   24-31 min_num_bits.  Supported: 2 <= L <= 16384 and every entry satisfies
+ *   L <= next<<nbits and (next+1)<<nbits <= 2L (always true for power-of-two L and
+ *   tables built by ANSTableGenerator); other tables return MANS_ETABLE.
+ * Output: out[0..n_out) symbols in decode order (reverse of the encoder input), the
+ *   first n_out steps of the decoder started at (first_bit, first_state).
+ */
+typedef struct mans_decode_args {
+    const uint32_t* units;      /* device */
+    uint64_t n_units;
+    uint8_t* out;               /* device, n_out bytes */
+    uint64_t n_out;
+    const uint32_t* table;      /* device, L entries */
+    uint32_t L;
+    uint32_t first_state;
+    uint32_t first_bit;
+    uint32_t chunk_units;       /* sync-chunk length in units; 0 = tuned */
+    uint32_t checkpoint_units;  /* checkpoint stride in units; 0 = tuned */
+    uint32_t flags;             /* MANS_F_* */
+    void* workspace;            /* device, >= mans_workspace_bytes(...) */
+    uint64_t workspace_bytes;
+    void* stream;               /* hipStream_t or NULL */
+} mans_decode_args;
+
+#define MANS_F_SYNC_ERRORS 1u   /* synchronise and report corrupt-stream errors */
+
+/* Upper bound of the workspace any tuning choice needs for n_units. */
+int mans_workspace_bytes(uint64_t n_units, uint32_t L, uint64_t* bytes);
+/* Workspace for an explicit geometry (chunk_units / checkpoint_units as in the args). */
+int mans_workspace_bytes_geometry(uint64_t n_units, uint32_t chunk_units,
+                                  uint32_t checkpoint_units, uint64_t* bytes);
+/* Enqueue the decode on args->stream.  Returns after enqueueing (like the
+ * reference, which returns before its last kernel completes) unless
+ * MANS_F_SYNC_ERRORS is set.  Integrity faults found on the device are reported
+ * by mans_decode_status(). */
+int mans_decode(const mans_decode_args* args);
+/* Device-side status word of the last decode that used `workspace`:
+ * 0 = ok, else a MANS_E* code (synchronises `stream`). */
+int mans_decode_status(void* workspace, void* stream, int* status);
+
+/* --------------------------------------------------------------------- batch */
+typedef struct mans_stream_desc {
+    const uint32_t* units;      /* device */
+    uint64_t n_units;
+    uint8_t* out;               /* device */
+    uint64_t n_out;
+    const uint32_t* table;      /* device, L entries */
+    uint32_t L;
+    uint32_t first_state;
+    uint32_t first_bit;
+    uint32_t reserved;
+} mans_stream_desc;
+
+typedef struct mans_batch_opts {
+    void* workspace;            /* device, >= mans_batch_workspace_bytes(...) */
+    uint64_t workspace_bytes;
+    void* stream;
+    uint32_t flags;
+    uint32_t reserved;
+} mans_batch_opts;
+
+typedef struct mans_batch_stats {
+    uint64_t total_units;
+    uint64_t total_symbols;
+    uint32_t n_streams;
+    uint32_t n_chunks;
+    uint32_t n_items;
+    uint32_t status;
+} mans_batch_stats;
+
+int mans_batch_workspace_bytes(const mans_stream_desc* descs, size_t n, uint64_t* bytes);
+int mans_decode_batch(const mans_stream_desc* descs, size_t n, const mans_batch_opts* opts,
+                      mans_batch_stats* stats);
+
+/* --------------------------------------------------------- host-side codec */
+/* Symbols s = min(255, floor(X)), X ~ Exp(lambda); element i depends only on (seed, i). */
+int mans_gen_symbols(uint64_t seed, double lambda, uint8_t* out, uint64_t n, int threads);
+/* Histogram -> counts summing to L (absent -> 0, present -> >= 1). */
+int mans_normalize_counts(const uint64_t* hist, uint32_t nsym, uint32_t L, uint32_t* counts);
+/* Encoder table (3 u32 per (symbol, state): next_state, code, length; symbol-major)
+ * and decoder table (L u32, reference layout) from counts summing to L. */
+int mans_build_tables(const uint32_t* counts, uint32_t nsym, uint32_t L, uint32_t* enc,
+                      uint32_t* dec);
+uint64_t mans_max_encoded_units(uint64_t n);
+/* Encode in[0..n) into units (decoder order); returns the unit count in *n_units. */
+int mans_encode(const uint8_t* in, uint64_t n, const uint32_t* enc, uint32_t L,
+                uint32_t* units, uint64_t* n_units, uint32_t* first_bit,
+                uint32_t* first_state);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MANS_C_API_H_ */

@@ -1,0 +1,162 @@
+"""ctypes bindings of libmultians_amd.so (the C ABI declared in include/mans_c_api.h).
+
+The library is the only decode path: importing a decode function with the library
+missing raises, there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libmultians_amd.so")
+
+MANS_OK = 0
+MANS_EINVAL = -1
+MANS_EHIP = -2
+MANS_ENOMEM = -3
+MANS_ETABLE = -4
+MANS_ECORRUPT = -5
+MANS_F_SYNC_ERRORS = 1
+
+
+class MansError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("mans error %d: %s" % (code, msg))
+        self.code = code
+
+
+class DecodeArgs(C.Structure):
+    _fields_ = [("units", C.c_void_p), ("n_units", C.c_uint64), ("out", C.c_void_p),
+                ("n_out", C.c_uint64), ("table", C.c_void_p), ("L", C.c_uint32),
+                ("first_state", C.c_uint32), ("first_bit", C.c_uint32),
+                ("chunk_units", C.c_uint32), ("checkpoint_units", C.c_uint32),
+                ("flags", C.c_uint32), ("workspace", C.c_void_p),
+                ("workspace_bytes", C.c_uint64), ("stream", C.c_void_p)]
+
+
+class StreamDesc(C.Structure):
+    _fields_ = [("units", C.c_void_p), ("n_units", C.c_uint64), ("out", C.c_void_p),
+                ("n_out", C.c_uint64), ("table", C.c_void_p), ("L", C.c_uint32),
+                ("first_state", C.c_uint32), ("first_bit", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("workspace", C.c_void_p), ("workspace_bytes", C.c_uint64),
+                ("stream", C.c_void_p), ("flags", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("total_units", C.c_uint64), ("total_symbols", C.c_uint64),
+                ("n_streams", C.c_uint32), ("n_chunks", C.c_uint32),
+                ("n_items", C.c_uint32), ("status", C.c_uint32)]
+
+
+# every symbol include/mans_c_api.h declares
+EXPORTS = [
+    "mans_error_string", "mans_last_error", "mans_set_device", "mans_get_device_count",
+    "mans_malloc", "mans_free", "mans_memset", "mans_memcpy_h2d", "mans_memcpy_d2h",
+    "mans_memcpy_d2d", "mans_host_alloc_pinned", "mans_host_free_pinned",
+    "mans_stream_create", "mans_stream_destroy", "mans_sync", "mans_event_create",
+    "mans_event_destroy", "mans_event_record", "mans_event_elapsed_ms",
+    "mans_workspace_bytes", "mans_workspace_bytes_geometry", "mans_decode", "mans_decode_status",
+    "mans_batch_workspace_bytes", "mans_decode_batch", "mans_gen_symbols",
+    "mans_normalize_counts", "mans_build_tables", "mans_max_encoded_units", "mans_encode",
+]
+
+_lib = None
+
+
+def lib():
+    """Load the library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libmultians_amd.so not built: run `python -m multians_amd.build`")
+    l = C.CDLL(LIB_PATH)
+    vp, u64, u32, i32 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int
+    sig = {
+        "mans_error_string": ([i32], C.c_char_p),
+        "mans_last_error": ([], C.c_char_p),
+        "mans_set_device": ([i32], i32),
+        "mans_get_device_count": ([C.POINTER(i32)], i32),
+        "mans_malloc": ([C.POINTER(vp), C.c_size_t], i32),
+        "mans_free": ([vp], i32),
+        "mans_memset": ([vp, i32, C.c_size_t, vp], i32),
+        "mans_memcpy_h2d": ([vp, vp, C.c_size_t, vp], i32),
+        "mans_memcpy_d2h": ([vp, vp, C.c_size_t, vp], i32),
+        "mans_memcpy_d2d": ([vp, vp, C.c_size_t, vp], i32),
+        "mans_host_alloc_pinned": ([C.POINTER(vp), C.c_size_t], i32),
+        "mans_host_free_pinned": ([vp], i32),
+        "mans_stream_create": ([C.POINTER(vp)], i32),
+        "mans_stream_destroy": ([vp], i32),
+        "mans_sync": ([vp], i32),
+        "mans_event_create": ([C.POINTER(vp)], i32),
+        "mans_event_destroy": ([vp], i32),
+        "mans_event_record": ([vp, vp], i32),
+        "mans_event_elapsed_ms": ([vp, vp, C.POINTER(C.c_float)], i32),
+        "mans_workspace_bytes": ([u64, u32, C.POINTER(u64)], i32),
+        "mans_workspace_bytes_geometry": ([u64, u32, u32, C.POINTER(u64)], i32),
+        "mans_decode": ([C.POINTER(DecodeArgs)], i32),
+        "mans_decode_status": ([vp, vp, C.POINTER(i32)], i32),
+        "mans_batch_workspace_bytes": ([C.POINTER(StreamDesc), C.c_size_t, C.POINTER(u64)], i32),
+        "mans_decode_batch": ([C.POINTER(StreamDesc), C.c_size_t, C.POINTER(BatchOpts),
+                               C.POINTER(BatchStats)], i32),
+        "mans_gen_symbols": ([u64, C.c_double, vp, u64, i32], i32),
+        "mans_normalize_counts": ([vp, u32, u32, vp], i32),
+        "mans_build_tables": ([vp, u32, u32, vp, vp], i32),
+        "mans_max_encoded_units": ([u64], u64),
+        "mans_encode": ([vp, u64, vp, u32, vp, C.POINTER(u64), C.POINTER(u32), C.POINTER(u32)], i32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(l, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = l
+    return l
+
+
+def check(rc):
+    if rc != 0:
+        raise MansError(rc, lib().mans_last_error().decode())
+    return rc
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ------------------------------------------------------------------ host codec
+
+def gen_symbols(seed, lam, n, threads=8):
+    out = np.empty(n, np.uint8)
+    check(lib().mans_gen_symbols(seed, lam, _ptr(out), n, threads))
+    return out
+
+
+def normalize_counts(hist, L):
+    hist = np.ascontiguousarray(hist, np.uint64)
+    counts = np.zeros(len(hist), np.uint32)
+    check(lib().mans_normalize_counts(_ptr(hist), len(hist), L, _ptr(counts)))
+    return counts
+
+
+def build_tables(counts, L, want_enc=True):
+    counts = np.ascontiguousarray(counts, np.uint32)
+    nsym = len(counts)
+    enc = np.zeros(3 * nsym * L, np.uint32) if want_enc else None
+    dec = np.zeros(L, np.uint32)
+    check(lib().mans_build_tables(_ptr(counts), nsym, L, _ptr(enc) if want_enc else None, _ptr(dec)))
+    return enc, dec
+
+
+def encode(symbols, enc, L):
+    symbols = np.ascontiguousarray(symbols, np.uint8)
+    n = len(symbols)
+    units = np.empty(lib().mans_max_encoded_units(n), np.uint32)
+    nu, fb, fs = C.c_uint64(), C.c_uint32(), C.c_uint32()
+    check(lib().mans_encode(_ptr(symbols), n, _ptr(enc), L, _ptr(units), C.byref(nu), C.byref(fb),
+                            C.byref(fs)))
+    return units[:nu.value].copy(), fb.value, fs.value

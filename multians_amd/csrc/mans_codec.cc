@@ -1,0 +1,177 @@
+// mans_codec.cc -- host-side producers of the decoder's inputs (C ABI).
+//
+// These are the reference's codec-construction layer, rebuilt for throughput
+// (thousands of tables and gigabytes of symbols for the batched workloads):
+//   mans_build_tables   ANSTableGenerator::generate_table / generate_encoder_table /
+//                       get_decoder_table (src/ans_table_generator.cc:164-288)
+//   mans_encode         ANSEncoder::encode (src/ans_encoder.cc:13-97) + the unit
+//                       reversal of CUHDInputBuffer (src/multians_input_buffer.cc:29)
+//   mans_normalize_counts  counts from a histogram (absent symbols get 0, present >= 1)
+//   mans_gen_symbols    s = min(255, floor(X)), X ~ Exp(lambda), counter-based
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <queue>
+#include <thread>
+#include <vector>
+
+#include "mans_internal.h"
+
+namespace {
+
+inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+void gen_range(uint64_t seed, const uint64_t* thr, uint8_t* out, uint64_t b, uint64_t e) {
+    for (uint64_t i = b; i < e; ++i) {
+        const uint64_t u = mix64(seed + (i + 1) * 0x9E3779B97F4A7C15ull) >> 11;
+        // thr[] is decreasing; count thresholds above u (thr[0] = 2^53 > u always)
+        uint32_t s = 0;
+        for (uint32_t step = 128; step; step >>= 1)
+            if (s + step <= 255u && u < thr[s + step]) s += step;
+        out[i] = (uint8_t)s;
+    }
+}
+
+struct Cand {
+    double p;
+    uint32_t s;
+    bool operator>(const Cand& o) const { return p > o.p || (p == o.p && s > o.s); }
+};
+
+}  // namespace
+
+extern "C" {
+
+int mans_gen_symbols(uint64_t seed, double lambda, uint8_t* out, uint64_t n, int threads) {
+    if (!out && n) return set_error(MANS_EINVAL, "out is NULL");
+    if (!(lambda > 0.0)) return set_error(MANS_EINVAL, "lambda must be > 0");
+    uint64_t thr[256];
+    for (int k = 0; k < 256; ++k) thr[k] = (uint64_t)std::floor(std::ldexp(std::exp(-lambda * k), 53));
+    if (threads <= 1 || n < (1u << 20)) {
+        gen_range(seed, thr, out, 0, n);
+        return MANS_OK;
+    }
+    std::vector<std::thread> th;
+    uint64_t per = (n + threads - 1) / threads;
+    for (int t = 0; t < threads; ++t) {
+        uint64_t b = std::min<uint64_t>(n, per * t), e = std::min<uint64_t>(n, b + per);
+        th.emplace_back(gen_range, seed, thr, out, b, e);
+    }
+    for (auto& x : th) x.join();
+    return MANS_OK;
+}
+
+int mans_normalize_counts(const uint64_t* hist, uint32_t nsym, uint32_t L, uint32_t* counts) {
+    if (!hist || !counts || !nsym) return set_error(MANS_EINVAL, "bad histogram");
+    uint64_t total = 0;
+    uint32_t present = 0;
+    for (uint32_t s = 0; s < nsym; ++s) { total += hist[s]; present += hist[s] != 0; }
+    if (!total || present > L) return set_error(MANS_EINVAL, "histogram cannot be normalised to L=%u", L);
+    int64_t sum = 0;
+    for (uint32_t s = 0; s < nsym; ++s) {
+        uint64_t c = hist[s] ? (uint64_t)(((unsigned __int128)hist[s] * L) / total) : 0;
+        counts[s] = hist[s] ? (c ? (uint32_t)c : 1u) : 0u;
+        sum += counts[s];
+    }
+    while (sum != (int64_t)L) {
+        uint32_t best = nsym;
+        for (uint32_t s = 0; s < nsym; ++s) {
+            if (!counts[s] || (sum > (int64_t)L && counts[s] <= 1)) continue;
+            if (best == nsym || counts[s] > counts[best]) best = s;
+        }
+        if (best == nsym) return set_error(MANS_EINVAL, "normalisation failed");
+        if (sum < (int64_t)L) { counts[best] += (uint32_t)((int64_t)L - sum); sum = L; }
+        else { --counts[best]; --sum; }
+    }
+    return MANS_OK;
+}
+
+int mans_build_tables(const uint32_t* counts, uint32_t nsym, uint32_t L, uint32_t* enc,
+                      uint32_t* dec) {
+    if (!counts || !nsym || nsym > 256 || L < 2 || L > 32768)
+        return set_error(MANS_EINVAL, "bad table parameters");
+    uint64_t sum = 0;
+    for (uint32_t s = 0; s < nsym; ++s) sum += counts[s];
+    if (sum != L) return set_error(MANS_EINVAL, "counts sum to %llu, not L=%u", (unsigned long long)sum, L);
+    // Spread: state L+i goes to the symbol with the smallest key p_s (lowest symbol
+    // on ties); p_s starts at 0.5/P_s and grows by 1/P_s, P_s = count/L in double.
+    std::priority_queue<Cand, std::vector<Cand>, std::greater<Cand>> pq;
+    std::vector<double> P(nsym);
+    std::vector<uint32_t> X(nsym);
+    for (uint32_t s = 0; s < nsym; ++s) {
+        X[s] = counts[s];
+        if (!counts[s]) continue;
+        P[s] = (double)counts[s] / (double)L;
+        pq.push({0.5 / P[s], s});
+    }
+    std::vector<uint32_t> st_sym(L), st_x(L);
+    for (uint32_t i = 0; i < L; ++i) {
+        Cand c = pq.top();
+        pq.pop();
+        st_sym[i] = c.s;
+        st_x[i] = X[c.s]++;
+        pq.push({c.p + 1.0 / P[c.s], c.s});
+    }
+    if (enc) std::memset(enc, 0, sizeof(uint32_t) * 3ull * nsym * L);
+    for (uint32_t i = 0; i < L; ++i) {
+        const uint32_t s = st_sym[i], xv = st_x[i];
+        uint32_t first_len = 0;
+        bool have_len = false;
+        for (uint32_t k = 0; ((uint64_t)xv << k) < 2ull * L; ++k) {
+            uint64_t lo = (uint64_t)xv << k, hi = lo + (1ull << k);
+            lo = std::max<uint64_t>(lo, L);
+            hi = std::min<uint64_t>(hi, 2ull * L);
+            if (lo >= hi) continue;
+            if (!have_len && k > 0) { first_len = k; have_len = true; }
+            if (enc) {
+                for (uint64_t x = lo; x < hi; ++x) {
+                    uint32_t* e = enc + 3ull * ((uint64_t)s * L + (x - L));
+                    e[0] = L + i;
+                    e[1] = (uint32_t)(x & ((1ull << k) - 1));
+                    e[2] = k;
+                }
+            }
+        }
+        // reference decoder entry: next = X, nbits = first nonzero length met in
+        // ascending source-state order (src/ans_table_generator.cc:274-281)
+        if (dec) dec[i] = (xv & 0xFFFFu) | (s << 16) | (first_len << 24);
+    }
+    return MANS_OK;
+}
+
+uint64_t mans_max_encoded_units(uint64_t n) { return n / 2 + 2; }
+
+int mans_encode(const uint8_t* in, uint64_t n, const uint32_t* enc, uint32_t L, uint32_t* units,
+                uint64_t* n_units, uint32_t* first_bit, uint32_t* first_state) {
+    if ((!in && n) || !enc || !units || !n_units || !first_bit || !first_state)
+        return set_error(MANS_EINVAL, "NULL argument");
+    // codewords appended MSB-first to 32-bit units, state starting at L; the last
+    // unit keeps `first_bit` valid bits left-aligned (zeros below)
+    uint64_t acc = 0;
+    uint32_t fill = 0, x = L;
+    uint64_t nu = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint32_t* e = enc + 3ull * ((uint64_t)in[i] * L + (x - L));
+        if (!e[0]) return set_error(MANS_EINVAL, "symbol %u has no code", in[i]);
+        acc = (acc << e[2]) | e[1];
+        fill += e[2];
+        x = e[0];
+        if (fill >= 32) {
+            fill -= 32;
+            units[nu++] = (uint32_t)(acc >> fill);
+            acc &= (1ull << fill) - 1;
+        }
+    }
+    units[nu++] = fill ? (uint32_t)(acc << (32 - fill)) : 0u;
+    std::reverse(units, units + nu);
+    *n_units = nu;
+    *first_bit = fill;
+    *first_state = x;
+    return MANS_OK;
+}
+
+}  // extern "C"

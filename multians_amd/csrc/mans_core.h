@@ -1,0 +1,311 @@
+// mans_core.h -- per-lane decode logic shared by the gfx950 kernels and the CPU
+// emulation build (tests only).
+//
+// Method (self-synchronising subsequence decoding, after the reference's
+// src/multians_gpu_decoder.cu:16-573 and src/multicore_decoder.cc:92-273, re-planned
+// for large sync chunks):
+//
+//   chunk c     = units [c*S, (c+1)*S) of one stream (S = chunk_units)
+//   checkpoint q= unit boundary q*K (K = checkpoint_units, K | S); its value is
+//                 the (state, bit offset) of the first decoding step whose start
+//                 position p satisfies p >= 32*q*K ("entry" of the boundary).
+//   spec pass   : lane c decodes chunk c from a guessed state at bit 0 of the chunk
+//                 (chunk 0: the true start), storing every checkpoint of its path
+//                 and the number of steps taken before it.
+//   walk pass   : lane c walks chunk c again from the exit of chunk c-1's path and
+//                 compares each checkpoint with the stored one; equality means the
+//                 two paths have merged (decoding is deterministic in (p, state)),
+//                 so everything stored from there on is on the walker's path.
+//                 Checkpoints before the merge are stored separately (wb*).
+//   fixup       : a walk that crosses its whole chunk without merging leaves the
+//                 next chunk's walk start unproven; that chunk is walked again
+//                 from the proven exit (rounds until no walk fails).
+//   write pass  : every checkpoint is now known on the true path together with its
+//                 output offset (exclusive scan of per-chunk step counts), so item
+//                 q decodes exactly off(q+1)-off(q) steps from checkpoint q and
+//                 writes them.  The last item decodes up to n_out, so zero-bit
+//                 trailing symbols are produced (the reference loses them when
+//                 n_units % S == 0).
+//
+// All arithmetic is unsigned integer.  A decoding step (multians_gpu_decoder.cu:131-166)
+// for a table entry e = {next, symbol, nbits}:
+//     state' = (next << nbits) | bits(p, nbits);   p += nbits
+// Entries are staged into a derived word:
+//     bits 0-4 nbits | bits 8-15 symbol | bits 16-31 (next << nbits) - L
+// so that state_index' = (w >> 16) + bits(p, w & 31), state_index = state - L.
+#ifndef MANS_CORE_H_
+#define MANS_CORE_H_
+
+#include <stdint.h>
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define MANS_HD __host__ __device__ __forceinline__
+#else
+#define MANS_HD inline
+#endif
+
+namespace mans {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// Packed boundary entry: state index (x - L) in bits 0-15, bit offset 0..32 in 16-21.
+MANS_HD uint32_t pack_entry(uint32_t xi, uint32_t off) { return xi | (off << 16); }
+MANS_HD uint32_t entry_state(uint32_t v) { return v & 0xFFFFu; }
+MANS_HD uint32_t entry_off(uint32_t v) { return (v >> 16) & 63u; }
+
+// Derived LDS table word from a reference CUHDCodetableItem word.  Returns 0 and
+// sets *bad when the entry cannot be decoded by the fast step: the next state must
+// stay inside [L, 2L) for every value of the nbits read, and a zero-bit entry must
+// strictly decrease the state (so zero-bit runs are finite).
+MANS_HD uint32_t derive_entry(uint32_t e, uint32_t L, uint32_t index, int* bad) {
+    uint32_t next = e & 0xFFFFu, sym = (e >> 16) & 0xFFu, nb = e >> 24;
+    if (nb > 15u || next == 0u) { *bad = 1; return 0u; }
+    uint32_t lo = next << nb, hi = (next + 1u) << nb;
+    if (lo < L || hi > 2u * L) { *bad = 1; return 0u; }
+    if (nb == 0u && lo - L >= index) { *bad = 1; return 0u; }
+    return nb | (sym << 8) | ((lo - L) << 16);
+}
+
+MANS_HD uint32_t funnel_shr(uint32_t hi, uint32_t lo, uint32_t s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_alignbit(hi, lo, s);
+#else
+    s &= 31u;
+    return s ? (lo >> s) | (hi << (32u - s)) : lo;
+#endif
+}
+
+MANS_HD uint32_t low_bits(uint32_t w, uint32_t word) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ubfe(w, 0u, word);   // width = word[4:0]
+#else
+    uint32_t n = word & 31u;
+    return n ? (w & (0xFFFFFFFFu >> (32u - n))) : 0u;
+#endif
+}
+
+// Bit cursor over one stream: lo/hi are units u and u+1, nx is unit u+2 (prefetched),
+// o in [0, 32) is the bit offset inside lo.  Units at or beyond n read as zero.
+struct Cursor {
+    const uint32_t* units;
+    uint32_t n;
+    uint32_t u;
+    uint32_t o;
+    uint32_t lo, hi, nx;
+};
+
+MANS_HD uint32_t load_unit(const uint32_t* units, uint32_t n, uint32_t i) {
+    return i < n ? units[i] : 0u;
+}
+
+// Position the cursor at absolute bit (32*unit + off), off in [0, 32].
+MANS_HD void cursor_init(Cursor& c, const uint32_t* units, uint32_t n, uint32_t unit,
+                         uint32_t off) {
+    c.units = units;
+    c.n = n;
+    c.u = unit + (off >> 5);
+    c.o = off & 31u;
+    c.lo = load_unit(units, n, c.u);
+    c.hi = load_unit(units, n, c.u + 1u);
+    c.nx = load_unit(units, n, c.u + 2u);
+}
+
+// One decoding step on state index xi with derived table tab.  Returns the derived
+// word (symbol in bits 8-15).  Advances the cursor; returns crossed=1 when the next
+// step starts in the following unit.
+template <typename Tab>
+MANS_HD uint32_t step(const Tab& tab, Cursor& c, uint32_t& xi, uint32_t& crossed) {
+    uint32_t w = tab[xi];
+    uint32_t win = funnel_shr(c.hi, c.lo, c.o);
+    xi = (w >> 16) + low_bits(win, w);
+    c.o += w & 31u;
+    crossed = 0u;
+    if (c.o >= 32u) {
+        c.o -= 32u;
+        c.u += 1u;
+        c.lo = c.hi;
+        c.hi = c.nx;
+        c.nx = load_unit(c.units, c.n, c.u + 2u);
+        crossed = 1u;
+    }
+    return w;
+}
+
+// ------------------------------------------------------------------ layouts
+// Per-stream parameters (device copy of one mans_stream_desc plus the plan).
+struct StreamPlan {
+    const uint32_t* units;
+    uint8_t* out;
+    const uint32_t* table;
+    uint64_t n_out;
+    uint32_t n_units;
+    uint32_t L;
+    uint32_t first_state;
+    uint32_t first_bit;
+    uint32_t S;          // chunk units (multiple of K, power of two)
+    uint32_t K;          // checkpoint units (power of two)
+    uint32_t log2K;
+    uint32_t n_chunks;   // ceil(n_units / S)
+    uint32_t n_ck;       // ceil(n_units / K)
+    uint32_t chunk_base; // global index of chunk 0
+    uint64_t ck_base;    // global index of checkpoint 0
+    uint32_t task_chunk; // first spec/walk task
+    uint32_t task_item;  // first write task
+};
+
+// Workspace arrays (global indices).
+struct Work {
+    // per chunk
+    uint32_t* exitv;     // spec path entry of boundary (c+1)S
+    uint32_t* own_total; // spec path steps in chunk c
+    uint32_t* wentry;    // walk start (entry of boundary cS on the proven path)
+    uint32_t* wexit;     // walk exit when it did not merge
+    uint32_t* wtotal;    // walk steps in chunk when it did not merge
+    uint32_t* mq;        // merge checkpoint (stream-local index) or kNone
+    uint32_t* wcm;       // walk steps before the merge checkpoint
+    uint32_t* fflag;     // 1 = walk into this chunk failed (pending fixup)
+    uint64_t* coff;      // exclusive offset of chunk c's first output symbol
+    // per checkpoint
+    uint32_t* ck;        // spec path entry
+    uint32_t* ckc;       // spec path steps since chunk start
+    uint32_t* wb;        // walk entry (before merge)
+    uint32_t* wbc;       // walk steps since chunk start
+    // misc
+    uint32_t* status;    // [0] error bits, [1] fail count, [2] fail count (next round), [3..] counters
+    uint32_t* flist;     // fail list (global chunk ids)
+    uint32_t* flist2;
+    uint32_t* ctick;     // task tickets: [0] spec, [1] walk, [2] write
+};
+
+// ---------------------------------------------------------------- lane bodies
+
+// Spec pass for chunk c of stream sp (lane body).
+template <typename Tab>
+MANS_HD void lane_spec(const StreamPlan& sp, const Work& w, const Tab& tab, uint32_t c) {
+    const uint32_t L = sp.L;
+    const uint32_t u0 = c * sp.S;
+    uint32_t u_stop = u0 + sp.S;
+    if (u_stop > sp.n_units) u_stop = sp.n_units;
+    Cursor cur;
+    uint32_t xi, off0;
+    if (c == 0u) {
+        off0 = 32u - sp.first_bit;
+        xi = sp.first_state - L;
+    } else {
+        off0 = 0u;
+        xi = sp.first_state - L;   // guessed state (the reference guesses the same)
+    }
+    cursor_init(cur, sp.units, sp.n_units, u0, off0);
+    const uint64_t qb = sp.ck_base;
+    const uint32_t gc = sp.chunk_base + c;
+    const uint32_t kmask = sp.K - 1u;
+    w.ck[qb + (u0 >> sp.log2K)] = pack_entry(xi, off0);
+    w.ckc[qb + (u0 >> sp.log2K)] = 0u;
+    if (c == 0u) {
+        w.mq[gc] = 0u;
+        w.wcm[gc] = 0u;
+        w.wentry[gc] = pack_entry(xi, off0);
+    }
+    uint32_t cnt = 0u;
+    while (cur.u < u_stop) {
+        uint32_t crossed;
+        step(tab, cur, xi, crossed);
+        ++cnt;
+        if (crossed && (cur.u & kmask) == 0u && cur.u < u_stop) {
+            uint64_t q = qb + (cur.u >> sp.log2K);
+            w.ck[q] = pack_entry(xi, cur.o);
+            w.ckc[q] = cnt;
+        }
+    }
+    w.exitv[gc] = pack_entry(xi, cur.o);
+    w.own_total[gc] = cnt;
+}
+
+// Walk into chunk c (c >= 1) from `entry` (packed, offset relative to unit c*S).
+// Returns 1 when it merged with the stored path, 0 when it crossed the whole chunk.
+template <typename Tab>
+MANS_HD uint32_t lane_walk(const StreamPlan& sp, const Work& w, const Tab& tab, uint32_t c,
+                           uint32_t entry) {
+    const uint32_t u0 = c * sp.S;
+    uint32_t u_stop = u0 + sp.S;
+    if (u_stop > sp.n_units) u_stop = sp.n_units;
+    const uint64_t qb = sp.ck_base;
+    const uint32_t gc = sp.chunk_base + c;
+    const uint32_t kmask = sp.K - 1u;
+    w.wentry[gc] = entry;
+    uint32_t q0 = u0 >> sp.log2K;
+    if (w.ck[qb + q0] == entry) {
+        w.mq[gc] = q0;
+        w.wcm[gc] = 0u;
+        return 1u;
+    }
+    w.wb[qb + q0] = entry;
+    w.wbc[qb + q0] = 0u;
+    Cursor cur;
+    uint32_t xi = entry_state(entry);
+    cursor_init(cur, sp.units, sp.n_units, u0, entry_off(entry));
+    uint32_t cnt = 0u;
+    while (cur.u < u_stop) {
+        uint32_t crossed;
+        step(tab, cur, xi, crossed);
+        ++cnt;
+        if (crossed && (cur.u & kmask) == 0u && cur.u < u_stop) {
+            uint32_t q = cur.u >> sp.log2K;
+            uint32_t v = pack_entry(xi, cur.o);
+            if (w.ck[qb + q] == v) {
+                w.mq[gc] = q;
+                w.wcm[gc] = cnt;
+                return 1u;
+            }
+            w.wb[qb + q] = v;
+            w.wbc[qb + q] = cnt;
+        }
+    }
+    w.mq[gc] = kNone;
+    w.wexit[gc] = pack_entry(xi, cur.o);
+    w.wtotal[gc] = cnt;
+    return 0u;
+}
+
+// Exit of chunk c (entry of boundary (c+1)S) along the chunk's current walk/path.
+MANS_HD uint32_t chunk_exit(const StreamPlan& sp, const Work& w, uint32_t c) {
+    const uint32_t gc = sp.chunk_base + c;
+    return w.mq[gc] == kNone ? w.wexit[gc] : w.exitv[gc];
+}
+
+// True number of steps owned by chunk c (valid after walks and fixups).
+MANS_HD uint32_t chunk_count(const StreamPlan& sp, const Work& w, uint32_t c) {
+    const uint32_t gc = sp.chunk_base + c;
+    uint32_t m = w.mq[gc];
+    if (m == kNone) return w.wtotal[gc];
+    return w.wcm[gc] + (w.own_total[gc] - w.ckc[sp.ck_base + m]);
+}
+
+// True entry and chunk-relative step count at checkpoint q.
+MANS_HD void checkpoint_truth(const StreamPlan& sp, const Work& w, uint32_t q, uint32_t* entry,
+                              uint32_t* cum) {
+    const uint32_t c = (q << sp.log2K) / sp.S;
+    const uint32_t gc = sp.chunk_base + c;
+    const uint64_t qq = sp.ck_base + q;
+    uint32_t m = w.mq[gc];
+    if (m == kNone || q < m) {
+        *entry = w.wb[qq];
+        *cum = w.wbc[qq];
+    } else {
+        *entry = w.ck[qq];
+        *cum = w.wcm[gc] + (w.ckc[qq] - w.ckc[sp.ck_base + m]);
+    }
+}
+
+MANS_HD uint64_t checkpoint_offset(const StreamPlan& sp, const Work& w, uint32_t q,
+                                   uint32_t* entry) {
+    uint32_t cum;
+    checkpoint_truth(sp, w, q, entry, &cum);
+    const uint32_t c = (q << sp.log2K) / sp.S;
+    return w.coff[sp.chunk_base + c] + cum;
+}
+
+}  // namespace mans
+
+#endif  // MANS_CORE_H_

@@ -1,0 +1,137 @@
+// mans_emu.cc -- CPU emulation of the decode pipeline (TESTS ONLY, libmans_emu.so).
+//
+// Runs the per-lane bodies of mans_core.h sequentially, phase by phase, in the same
+// order the kernels of mans_kernels.hip run them, so the algorithm (spec, walk,
+// fixup rounds, scan, count-driven write) can be checked against the oracle without
+// a GPU.  Lanes of one kernel touch disjoint state, so a sequential loop over lanes
+// reproduces what the kernel computes.  The product never loads this library.
+#include <cstring>
+#include <vector>
+
+#include "mans_core.h"
+
+using namespace mans;
+
+namespace {
+
+struct HostTab {
+    const uint32_t* p;
+    uint32_t operator[](uint32_t i) const { return p[i]; }
+};
+
+uint32_t ilog2(uint32_t v) { uint32_t r = 0; while ((1u << (r + 1)) <= v) ++r; return r; }
+
+}  // namespace
+
+extern "C" {
+
+// Returns 0, -4 for a rejected table.  stats (optional, 8 u64): n_chunks, n_ck,
+// walk failures (round 1), fixup rounds, total re-walks, merges, 0, 0.
+int mans_emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint64_t n_out,
+                    const uint32_t* table, uint32_t L, uint32_t first_state, uint32_t first_bit,
+                    uint32_t S, uint32_t K, uint64_t* stats, uint32_t* ck_entries_out,
+                    uint64_t* ck_offsets_out) {
+    if (!n_out) return 0;
+    std::vector<uint32_t> tab(L);
+    int bad = 0;
+    for (uint32_t i = 0; i < L; ++i) tab[i] = derive_entry(table[i], L, i, &bad);
+    if (bad) return -4;
+    StreamPlan sp;
+    std::memset(&sp, 0, sizeof(sp));
+    sp.units = units;
+    sp.out = out;
+    sp.table = table;
+    sp.n_out = n_out;
+    sp.n_units = (uint32_t)n_units;
+    sp.L = L;
+    sp.first_state = first_state;
+    sp.first_bit = first_bit;
+    sp.S = S;
+    sp.K = K;
+    sp.log2K = ilog2(K);
+    sp.n_chunks = (uint32_t)((n_units + S - 1) / S);
+    sp.n_ck = (uint32_t)((n_units + K - 1) / K);
+    const uint32_t nc = sp.n_chunks, nq = sp.n_ck;
+    std::vector<uint32_t> exitv(nc), own(nc), went(nc), wexit(nc), wtot(nc), mq(nc), wcm(nc),
+        ff(nc), fl(nc + 1), fl2(nc + 1), status(64);
+    std::vector<uint64_t> coff(nc);
+    std::vector<uint32_t> ck(nq), ckc(nq), wb(nq), wbc(nq);
+    Work w;
+    w.exitv = exitv.data(); w.own_total = own.data(); w.wentry = went.data();
+    w.wexit = wexit.data(); w.wtotal = wtot.data(); w.mq = mq.data(); w.wcm = wcm.data();
+    w.fflag = ff.data(); w.coff = coff.data(); w.ck = ck.data(); w.ckc = ckc.data();
+    w.wb = wb.data(); w.wbc = wbc.data(); w.status = status.data(); w.flist = fl.data();
+    w.flist2 = fl2.data(); w.ctick = nullptr;
+    HostTab ht{tab.data()};
+    uint64_t st_fail = 0, st_rounds = 0, st_rewalks = 0, st_merges = 0;
+    // k_spec
+    for (uint32_t c = 0; c < nc; ++c) lane_spec(sp, w, ht, c);
+    // k_walk
+    for (uint32_t c = 1; c < nc; ++c) {
+        uint32_t ok = lane_walk(sp, w, ht, c, exitv[c - 1]);
+        st_merges += ok;
+        ff[c] = 0;
+    }
+    // initial fixup targets: successors of failed walks whose exit changed
+    for (uint32_t c = 1; c + 1 < nc; ++c)
+        if (mq[c] == kNone && wexit[c] != exitv[c]) fl[status[1]++] = c + 1;
+    st_fail = status[1];
+    // k_fixup (same phases as the kernel)
+    uint32_t* list = fl.data();
+    uint32_t* next = fl2.data();
+    for (;;) {
+        uint32_t n = status[1];
+        if (!n) break;
+        ++st_rounds;
+        for (uint32_t i = 0; i < n; ++i) {          // A: capture entries, mark
+            uint32_t t = list[i];
+            next[i] = chunk_exit(sp, w, t - 1);
+            ff[t] = 1;
+        }
+        for (uint32_t i = 0; i < n; ++i) {          // B: walk again
+            ++st_rewalks;
+            lane_walk(sp, w, ht, list[i], next[i]);
+        }
+        status[2] = 0;
+        for (uint32_t i = 0; i < n; ++i) {          // C: consistency of t and t+1
+            uint32_t t = list[i];
+            if (w.wentry[t] != chunk_exit(sp, w, t - 1)) next[status[2]++] = t;
+            if (t + 1 < nc && !ff[t + 1] && w.wentry[t + 1] != chunk_exit(sp, w, t))
+                next[status[2]++] = t + 1;
+        }
+        for (uint32_t i = 0; i < n; ++i) ff[list[i]] = 0;
+        status[1] = status[2];
+        std::swap(list, next);
+    }
+    // k_scan
+    uint64_t run = 0;
+    for (uint32_t c = 0; c < nc; ++c) {
+        coff[c] = run;
+        run += chunk_count(sp, w, c);
+    }
+    // k_write
+    for (uint32_t q = 0; q < nq; ++q) {
+        uint32_t entry, e2;
+        uint64_t off = checkpoint_offset(sp, w, q, &entry);
+        uint64_t end = (q + 1 < nq) ? checkpoint_offset(sp, w, q + 1, &e2) : n_out;
+        if (ck_entries_out) ck_entries_out[q] = entry;
+        if (ck_offsets_out) ck_offsets_out[q] = off;
+        if (end > n_out) end = n_out;
+        if (off >= end) continue;
+        Cursor cur;
+        cursor_init(cur, units, (uint32_t)n_units, q << sp.log2K, entry_off(entry));
+        uint32_t xi = entry_state(entry);
+        for (uint64_t i = off; i < end; ++i) {
+            uint32_t crossed;
+            uint32_t wd = step(ht, cur, xi, crossed);
+            out[i] = (uint8_t)(wd >> 8);
+        }
+    }
+    if (stats) {
+        stats[0] = nc; stats[1] = nq; stats[2] = st_fail; stats[3] = st_rounds;
+        stats[4] = st_rewalks; stats[5] = st_merges; stats[6] = 0; stats[7] = 0;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+"""Python host layer over the C ABI, used by tests and bench.py.
+
+Mirrors the reference's host objects (src/multians_input_buffer.cc, src/multians_codetable.cc,
+src/multians_output_buffer.cc) and decode entry point (CUHDGPUDecoder::decode,
+include/cuhd_gpu_decoder.h:24-35).  Device memory and streams come from torch;
+all decoding runs in libmultians_amd.so.
+"""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+
+@dataclass
+class EncodedStream:
+    """A compressed stream in the reference's CUHDInputBuffer format."""
+    units: np.ndarray          # u32, decoder order
+    first_bit: int
+    first_state: int
+    table: np.ndarray          # u32[L], CUHDCodetableItem layout
+    L: int
+    n_symbols: int
+    counts: np.ndarray = None
+
+    @property
+    def n_units(self):
+        return len(self.units)
+
+
+def encode_symbols(symbols, L, counts=None):
+    """Histogram -> counts -> tables -> encoded stream (host, product codec)."""
+    symbols = np.ascontiguousarray(symbols, np.uint8)
+    if counts is None:
+        hist = np.bincount(symbols, minlength=256).astype(np.uint64)
+        counts = _lib.normalize_counts(hist, L)
+    enc, dec = _lib.build_tables(counts, L)
+    units, fb, fs = _lib.encode(symbols, enc, L)
+    return EncodedStream(units, fb, fs, dec, L, len(symbols), counts)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def workspace_bytes(n_units, L, chunk_units=0, checkpoint_units=0):
+    b = C.c_uint64()
+    if chunk_units and checkpoint_units:
+        _lib.check(_lib.lib().mans_workspace_bytes_geometry(n_units, chunk_units,
+                                                            checkpoint_units, C.byref(b)))
+    else:
+        _lib.check(_lib.lib().mans_workspace_bytes(n_units, L, C.byref(b)))
+    return b.value
+
+
+class DeviceStream:
+    """A stream resident in device memory (torch tensors), ready to decode."""
+
+    def __init__(self, es: EncodedStream, device="cuda", chunk_units=0, checkpoint_units=0):
+        torch = _torch()
+        self.es = es
+        self.units = torch.from_numpy(es.units.view(np.int32)).to(device)
+        self.table = torch.from_numpy(es.table.view(np.int32)).to(device)
+        self.out = torch.zeros(max(es.n_symbols, 1), dtype=torch.uint8, device=device)
+        self.ws = torch.zeros(workspace_bytes(es.n_units, es.L, chunk_units, checkpoint_units),
+                              dtype=torch.uint8, device=device)
+
+    def args(self, stream=None, chunk_units=0, checkpoint_units=0, flags=0, n_out=None):
+        torch = _torch()
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        a = _lib.DecodeArgs()
+        a.units = self.units.data_ptr()
+        a.n_units = self.es.n_units
+        a.out = self.out.data_ptr()
+        a.n_out = self.es.n_symbols if n_out is None else n_out
+        a.table = self.table.data_ptr()
+        a.L = self.es.L
+        a.first_state = self.es.first_state
+        a.first_bit = self.es.first_bit
+        a.chunk_units = chunk_units
+        a.checkpoint_units = checkpoint_units
+        a.flags = flags
+        a.workspace = self.ws.data_ptr()
+        a.workspace_bytes = self.ws.numel()
+        a.stream = st
+        return a
+
+    def decode(self, **kw):
+        a = self.args(**kw)
+        _lib.check(_lib.lib().mans_decode(C.byref(a)))
+        return self.out
+
+    def status(self, stream=None):
+        torch = _torch()
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        s = C.c_int()
+        _lib.check(_lib.lib().mans_decode_status(C.c_void_p(self.ws.data_ptr()), C.c_void_p(st),
+                                                 C.byref(s)))
+        return s.value

@@ -1,0 +1,94 @@
+/*
+ * mans_oracle.h -- CPU oracle for tANS subsequence decoding (TEST INFRASTRUCTURE ONLY).
+ *
+ * This library is the checker.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it; the product (multians_amd/) never does.
+ *
+ * It restates, in plain C, the reference's algorithm for the decode path:
+ *   - tANS spread / encoder table / decoder table    src/ans_table_generator.cc:164-288
+ *   - single-state encoder + stream format            src/ans_encoder.cc:13-97,
+ *                                                     src/multians_input_buffer.cc:15-30
+ *   - one decoding step (table lookup, bit refill)    src/multians_gpu_decoder.cu:127-169
+ *   - MulticoreDecoder (the CPU decoder)              src/multicore_decoder.cc:17-333
+ * plus a canonical sequential decoder that also reports, per subsequence of S
+ * units, the true entry point (bit, state) and the number of owned steps.
+ *
+ * How each function is pinned (see tests/test_oracle.py):
+ *   mo_build_tables      vs the compiled reference ANSTableGenerator (oracle/_ref)
+ *                        on the demo distributions; invariants (every state filled,
+ *                        decode(encode(x)) == x).
+ *   mo_encode            vs the compiled reference ANSEncoder (units, first_bit,
+ *                        first_state; don't-care tail bits masked); integrity
+ *                        invariant (decoder ends at state L, bit 32*n_units).
+ *   mo_decode_seq        round trip against the original symbols; brute-force
+ *                        re-derivation of entries/counts on tiny streams.
+ *   mo_decode_multicore  vs the compiled reference MulticoreDecoder (same S, T),
+ *                        including the streams where the reference drops trailing
+ *                        zero-bit symbols.
+ *   mo_gen_symbols       closed form: empirical frequencies vs exp(-lambda k).
+ */
+#ifndef MANS_ORACLE_H_
+#define MANS_ORACLE_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Encoder table row entry: enc[s * L + (x - L)] for symbol s, state x in [L, 2L). */
+typedef struct {
+    uint32_t next_state; /* state after encoding s from x, in [L, 2L); 0 = unset */
+    uint32_t code;       /* low `len` bits of x, emitted MSB-first              */
+    uint32_t len;        /* number of emitted bits                               */
+} mo_enc_entry;
+
+/* symbols s = min(255, floor(X)), X ~ Exp(lambda), from a counter-based PRNG.
+ * Element i depends only on (seed, i). */
+void mo_gen_symbols(uint64_t seed, double lambda, uint8_t* out, size_t n);
+
+/* Histogram -> counts summing to L (absent -> 0, present -> >= 1). */
+int mo_normalize(const uint64_t* hist, int nsym, uint32_t L, uint32_t* counts);
+
+/* Reference distribution of the demo (ans_table_generator.cc:18-59) needs libstdc++
+ * <random>; it is taken from oracle/_ref, not restated. */
+
+/* Spread + encoder table + decoder table (reference layout u32: next | sym<<16 | nbits<<24).
+ * counts[nsym] sum to L; zero counts allowed (those rows stay unset). */
+int mo_build_tables(const uint32_t* counts, int nsym, uint32_t L,
+                    mo_enc_entry* enc /* nsym*L */, uint32_t* dec /* L */);
+
+/* Encode in[0..n) -> units in DECODER order (reversed), returns n_units.
+ * out must hold mo_max_units(n, L) units.  Don't-care tail bits are zero. */
+size_t mo_max_units(size_t n, uint32_t L);
+size_t mo_encode(const uint8_t* in, size_t n, const mo_enc_entry* enc, uint32_t L,
+                 uint32_t* out_units, uint32_t* first_bit, uint32_t* first_state);
+
+/* Canonical sequential decoder.
+ *   units[n_units] (+ reads beyond n_units treated as zero), table[L]
+ *   writes out[n_out] symbols (decode order).
+ *   If S > 0: entries[n_sub], counts[n_sub] with n_sub = ceil(n_units / S):
+ *     entries[j] = (x - L) | (p - 32*S*j) << 16 : first step with bit position p >= 32*S*j
+ *     counts[j]  = number of the first n_out steps owned by subsequence j
+ *                  (the last subsequence also owns every later step).
+ *   end_state/end_bit: state and absolute bit position after n_out steps.
+ *   Returns 0, or -1 if the walk leaves [L, 2L) (invalid table). */
+int mo_decode_seq(const uint32_t* units, size_t n_units, const uint32_t* table, uint32_t L,
+                  uint32_t first_state, uint32_t first_bit, uint8_t* out, size_t n_out,
+                  uint32_t S, uint32_t* entries, uint32_t* counts,
+                  uint32_t* end_state, uint64_t* end_bit);
+
+/* Restatement of MulticoreDecoder::decode (multicore_decoder.cc:17-90) with pthreads.
+ * units must be padded by >= S+2 zero units after n_units (the reference pads 4 and
+ * reads up to ceil(n/S)*S+1).  out is NOT cleared: bytes the method never writes
+ * keep their previous contents, as in the reference.  Returns 0 or -1. */
+int mo_decode_multicore(size_t S, size_t T, const uint32_t* units, size_t n_units,
+                        const uint32_t* table, uint32_t L, uint32_t first_state,
+                        uint32_t first_bit, uint8_t* out, size_t n_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

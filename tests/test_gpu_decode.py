@@ -1,0 +1,53 @@
+"""GPU parity: the HIP decoder (through the C ABI) against the oracle.
+
+Bar: bit-exact.  Each case decodes a seeded stream and compares with the oracle's
+sequential decoder and with the original symbols (reversed: the decoder emits in
+decode order, src/main.cc:123).
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(lam, L, n, seed=11, **kw):
+    import torch
+    from multians_amd.decoder import DeviceStream, encode_symbols
+    sym = ol.gen_symbols(seed, lam, n)
+    es = encode_symbols(sym, L)
+    ds = DeviceStream(es, chunk_units=kw.get("chunk_units", 0),
+                      checkpoint_units=kw.get("checkpoint_units", 0))
+    out = ds.decode(**kw)
+    torch.cuda.synchronize()
+    got = out[:n].cpu().numpy()
+    st = ol.Stream(es.units, es.first_bit, es.first_state, es.table, L, n)
+    ref, _, _, _, _ = ol.decode_seq(st)
+    assert ds.status() == 0
+    return got, ref, sym
+
+
+@pytest.mark.parametrize("lam", [0.002, 0.01, 0.1, 1.0, 3.0])
+@pytest.mark.parametrize("L", [1024, 4096])
+def test_decode_matches_oracle(lam, L):
+    print(f"case lam={lam} L={L}", flush=True)
+    got, ref, sym = _case(lam, L, 3_000_000)
+    assert np.array_equal(got, ref)
+    assert np.array_equal(got[::-1], sym)
+
+
+@pytest.mark.parametrize("S,K", [(64, 8), (256, 16), (1024, 16), (4096, 32)])
+def test_decode_geometries(S, K):
+    """Small chunks force many walk failures and fixup rounds."""
+    print(f"case S={S} K={K}", flush=True)
+    got, ref, sym = _case(0.1, 4096, 2_000_000, chunk_units=S, checkpoint_units=K)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 100, 1000, 4095, 65537])
+def test_small_streams(n):
+    print(f"case n={n}", flush=True)
+    got, ref, sym = _case(0.5, 1024, n)
+    assert np.array_equal(got, ref)
+    assert np.array_equal(got[::-1], sym)

@@ -1,0 +1,202 @@
+"""Pin the oracle (oracle/libmans_oracle.so) before trusting it.
+
+Checks, each chosen so a plausible mistake fails one of them:
+  * against the reference's own CPU code compiled from its sources (oracle/_ref):
+    tables of the demo distributions (src/main.cc:55-73), encoder output
+    (units, first_bit, first_state), MulticoreDecoder output;
+  * against committed golden vectors (tests/golden/) made by that same build,
+    so the pinning holds where the reference is absent;
+  * against invariants: decode(encode(x)) == x, the decoder ends at state L at bit
+    32*n_units, brute-force re-derivation of subsequence entries and counts;
+  * closed forms: symbol frequencies of mo_gen_symbols vs exp(-lambda k).
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+need_ref = pytest.mark.skipif(not ol.ref_available(), reason="oracle/_ref not built")
+
+
+def _ref_demo(lam, L=1024, nsym=256, seed=5):
+    R = ol.ref()
+    counts = np.zeros(nsym, np.uint32)
+    dec = np.zeros(L, np.uint32)
+    en, ec, el = (np.zeros(nsym * L, np.uint32) for _ in range(3))
+    R.ref_demo_tables(lam, nsym, L, seed, counts, dec, en, ec, el)
+    return counts, dec, en, ec, el
+
+
+@need_ref
+@pytest.mark.parametrize("lam", [0.1, 0.26, 0.42, 1.06, 2.34])
+@pytest.mark.parametrize("L", [1024, 4096])
+def test_tables_match_reference(lam, L):
+    counts, dec, en, ec, el = _ref_demo(lam, L)
+    _, enc, dec_o = ol.build_tables(counts, L)
+    assert np.array_equal(dec, dec_o)
+    assert np.array_equal(en, enc[:, :, 0].ravel())
+    assert np.array_equal(ec, enc[:, :, 1].ravel())
+    assert np.array_equal(el, enc[:, :, 2].ravel())
+
+
+def _ref_encode(counts, L, data):
+    R = ol.ref()
+    n = len(data)
+    buf = np.zeros(n + 8, np.uint8)
+    buf[:n] = data
+    units = np.zeros(n // 2 + 16, np.uint32)
+    fb, fs = C.c_uint32(), C.c_uint32()
+    nu = R.ref_encode(counts, len(counts), L, buf, n, units, len(units), C.byref(fb), C.byref(fs))
+    assert nu > 0
+    return units[:nu], fb.value, fs.value
+
+
+def _units_equal(a, b, first_bit):
+    """Equal except the don't-care low bits of unit 0 (left-aligned first_bit valid bits)."""
+    if len(a) != len(b):
+        return False
+    mask = np.uint32((0xFFFFFFFF << (32 - first_bit)) & 0xFFFFFFFF) if first_bit else np.uint32(0)
+    return bool(np.array_equal(a[1:], b[1:]) and (a[0] & mask) == (b[0] & mask))
+
+
+@need_ref
+@pytest.mark.parametrize("lam", [0.1, 0.58, 2.34])
+@pytest.mark.parametrize("n", [1, 2, 31, 1000, 150000])
+def test_encoder_matches_reference(lam, n):
+    L = 1024
+    counts, *_ = _ref_demo(lam, L)
+    data = np.zeros(n + 8, np.uint8)
+    ol.ref().ref_gen_test_data(counts, 256, n, L, 5, data)
+    data = data[:n]
+    ru, rfb, rfs = _ref_encode(counts, L, data)
+    st, _ = ol.encode(data, counts, L)
+    assert (rfb, rfs) == (st.first_bit, st.first_state)
+    assert _units_equal(ru, st.units, rfb)
+
+
+@need_ref
+@pytest.mark.parametrize("lam", [0.1, 1.0, 2.34])
+@pytest.mark.parametrize("S,T", [(4, 1), (4, 8), (8, 3), (32, 16)])
+@pytest.mark.parametrize("L", [1024, 4096])
+def test_multicore_matches_reference(lam, S, T, L):
+    counts, *_ = _ref_demo(lam, L)
+    n = 120000
+    data = np.zeros(n + 8, np.uint8)
+    ol.ref().ref_gen_test_data(counts, 256, n, L, 5, data)
+    st, _ = ol.encode(data[:n], counts, L)
+    ours = ol.decode_multicore(st, S=S, T=T)
+    ref = np.zeros(n, np.uint8)
+    t = ol.ref().ref_multicore(S, T, np.concatenate([st.units, np.zeros(S + 8, np.uint32)]),
+                               len(st.units), st.table, L, st.first_state, st.first_bit, ref, n)
+    assert t >= 0
+    assert np.array_equal(ours, ref)
+
+
+def _q1_stream(S):
+    """A stream whose zero-bit trailing symbols fall at unit n_units with n_units % S == 0:
+    symbol 7 takes > L/2 of the states, so the first encoded symbols cost 0 bits."""
+    L = 1024
+    counts = np.ones(256, np.uint32)
+    counts[7] = L - 255
+    for seed in range(200):
+        rng = np.random.default_rng(seed)
+        n = 5000 + seed
+        data = np.where(rng.random(n) < 0.9, 7, rng.integers(0, 256, n)).astype(np.uint8)
+        data[:3] = 7
+        st, _ = ol.encode(data, counts, L)
+        if len(st.units) % S == 0:
+            return st, data, counts
+    raise AssertionError("no Q1 stream found")
+
+
+@need_ref
+def test_multicore_drops_zero_bit_tail_like_reference():
+    """Where the reference loses trailing zero-bit symbols (n_units % S == 0), the
+    restatement loses exactly the same ones, and the sequential oracle does not."""
+    S, T = 4, 4
+    st, data, counts = _q1_stream(S)
+    n = len(data)
+    seq, *_ = ol.decode_seq(st)
+    assert np.array_equal(seq[::-1], data)
+    ours = ol.decode_multicore(st, S=S, T=T)
+    ref = np.zeros(n, np.uint8)
+    ol.ref().ref_multicore(S, T, np.concatenate([st.units, np.zeros(S + 8, np.uint32)]),
+                           len(st.units), st.table, st.L, st.first_state, st.first_bit, ref, n)
+    assert np.array_equal(ours, ref)
+    assert not np.array_equal(ref, seq)          # the reference really drops symbols
+    diff = np.nonzero(ref != seq)[0]
+    assert diff.min() >= n - 8 and np.all(ref[diff] == 0)
+
+
+@pytest.mark.parametrize("lam,L", [(0.002, 4096), (0.1, 1024), (1.0, 4096), (3.0, 32),
+                                   (3.0, 4096), (0.05, 512), (1.5, 16384)])
+def test_roundtrip_and_end_invariant(lam, L):
+    sym = ol.gen_symbols(9, lam, 200000)
+    st, _ = ol.encode(sym, L=L)
+    out, _, _, es, eb = ol.decode_seq(st)
+    assert np.array_equal(out[::-1], sym)
+    assert es == L                       # encoder starts at state index 0
+    assert eb == 32 * len(st.units)      # every bit consumed
+
+
+def test_entries_counts_bruteforce():
+    """Entries/counts per subsequence re-derived by an independent pure-Python walk."""
+    L = 64
+    sym = ol.gen_symbols(3, 0.4, 700)
+    st, _ = ol.encode(sym, L=L)
+    bits = []
+    for u in st.units:
+        bits += [(int(u) >> i) & 1 for i in range(32)]
+    bits += [0] * 64
+    for S in (1, 2, 3, 5):
+        out, ent, cnt, _, _ = ol.decode_seq(st, S=S)
+        nsub = -(-len(st.units) // S)
+        p, x = 32 - st.first_bit, st.first_state
+        e_bf, c_bf = [None] * nsub, [0] * nsub
+        for _ in range(len(sym)):
+            for j in range(nsub):
+                if e_bf[j] is None and p >= 32 * S * j:
+                    e_bf[j] = (x - L) | ((p - 32 * S * j) << 16)
+            c_bf[min(p // (32 * S), nsub - 1)] += 1
+            e = int(st.table[x - L])
+            nb = e >> 24
+            v = sum(bits[p + i] << i for i in range(nb))
+            x = ((e & 0xFFFF) << nb) | v
+            p += nb
+        assert list(ent) == e_bf and list(cnt) == c_bf
+
+
+@pytest.mark.parametrize("lam", [0.05, 0.5, 2.0])
+def test_symbol_generator_closed_form(lam):
+    n = 2_000_000
+    s = ol.gen_symbols(123, lam, n)
+    k = np.arange(256)
+    p = np.exp(-lam * k) - np.exp(-lam * (k + 1))
+    p[255] = np.exp(-lam * 255)
+    f = np.bincount(s, minlength=256) / n
+    sd = np.sqrt(p * (1 - p) / n)
+    assert np.all(np.abs(f - p) <= 6 * sd + 1e-9)
+    # element i depends only on (seed, i)
+    assert np.array_equal(ol.gen_symbols(123, lam, 1000), s[:1000])
+
+
+def test_golden_vectors():
+    """Vectors written by tests/golden/make_golden.py from the compiled reference."""
+    path = os.path.join(GOLDEN, "reference_vectors.json")
+    g = json.load(open(path))
+    for case in g["cases"]:
+        L = case["L"]
+        counts = np.array(case["counts"], np.uint32)
+        _, _, dec = ol.build_tables(counts, L)
+        assert np.array_equal(dec, np.array(case["dec_table"], np.uint32))
+        data = np.frombuffer(bytes.fromhex(case["symbols_hex"]), np.uint8)
+        st, _ = ol.encode(data, counts, L)
+        assert st.first_bit == case["first_bit"] and st.first_state == case["first_state"]
+        assert _units_equal(np.array(case["units"], np.uint32), st.units, st.first_bit)
+        mc = ol.decode_multicore(st, S=case["S"], T=case["T"])
+        assert mc.tobytes().hex() == case["multicore_hex"]
