@@ -102,7 +102,13 @@ typedef struct mans_decode_args {
     void* workspace;            /* device, >= mans_workspace_bytes(...) */
     uint64_t workspace_bytes;
     void* stream;               /* hipStream_t or NULL */
+    void** kernel_events;       /* optional: MANS_N_KERNEL_EVENTS hipEvent_t recorded on
+                                   `stream` around the kernels (timing), or NULL */
 } mans_decode_args;
+
+/* kernel_events[i] is recorded before kernel i (0 init, 1 spec, 2 walk, 3 fixup,
+ * 4 scan, 5 write); kernel_events[6] after the last one. */
+#define MANS_N_KERNEL_EVENTS 7
 
 #define MANS_F_SYNC_ERRORS 1u   /* synchronise and report corrupt-stream errors */
 

@@ -1,0 +1,21 @@
+// multians.h -- umbrella header (reference include/multians.h:11-30).  The GPU classes
+// are always available; the reference's CPU MulticoreDecoder is not part of this
+// library (it is the checker/CPU baseline under oracle/).
+#ifndef MULTIANS_H_
+#define MULTIANS_H_
+
+#include "ans_encoder.h"
+#include "ans_encoder_table.h"
+#include "ans_table_generator.h"
+#include "cuhd_codetable.h"
+#include "cuhd_constants.h"
+#include "cuhd_gpu_codetable.h"
+#include "cuhd_gpu_decoder.h"
+#include "cuhd_gpu_decoder_memory.h"
+#include "cuhd_gpu_input_buffer.h"
+#include "cuhd_gpu_output_buffer.h"
+#include "cuhd_input_buffer.h"
+#include "cuhd_output_buffer.h"
+#include "cuhd_util.h"
+
+#endif

@@ -32,7 +32,12 @@ class DecodeArgs(C.Structure):
                 ("first_state", C.c_uint32), ("first_bit", C.c_uint32),
                 ("chunk_units", C.c_uint32), ("checkpoint_units", C.c_uint32),
                 ("flags", C.c_uint32), ("workspace", C.c_void_p),
-                ("workspace_bytes", C.c_uint64), ("stream", C.c_void_p)]
+                ("workspace_bytes", C.c_uint64), ("stream", C.c_void_p),
+                ("kernel_events", C.c_void_p)]
+
+
+N_KERNEL_EVENTS = 7
+KERNEL_NAMES = ["init", "spec", "walk", "fixup", "scan", "write"]
 
 
 class StreamDesc(C.Structure):

@@ -23,7 +23,7 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 LIB_SOURCES = ["mans_kernels.hip", "mans_runtime.hip", "mans_codec.cc"]
-CUHD_SOURCES = ["cuhd_gpu_decoder.cc", "cuhd_buffers.cc"]
+CUHD_SOURCES = ["cuhd_gpu_decoder.cc", "cuhd_buffers.cc", "ans_compat.cc"]
 
 
 def _run(cmd):

@@ -348,14 +348,23 @@ Work make_work(void* base, const Layout& l) {
 }
 
 static int launch_pipeline(const KArgs& a, uint32_t n_chunk_tasks, uint32_t n_item_tasks,
-                           uint32_t n_scan_blocks, uint32_t maxL, hipStream_t st) {
+                           uint32_t n_scan_blocks, uint32_t maxL, hipStream_t st,
+                           void** ev = nullptr) {
     size_t lds = (size_t)maxL * 4u;
+    auto mark = [&](int i) { if (ev) hipEventRecord((hipEvent_t)ev[i], st); };
+    mark(0);
     hipLaunchKernelGGL(k_init, dim3(1), dim3(64), 0, st, a.w, 64u);
+    mark(1);
     hipLaunchKernelGGL(k_spec, dim3(n_chunk_tasks), dim3(kWG), lds, st, a);
+    mark(2);
     hipLaunchKernelGGL(k_walk, dim3(n_chunk_tasks), dim3(kWG), lds, st, a);
+    mark(3);
     hipLaunchKernelGGL(k_fixup, dim3(1), dim3(kWG), lds, st, a);
+    mark(4);
     hipLaunchKernelGGL(k_scan, dim3(n_scan_blocks), dim3(kScanWG), 0, st, a);
+    mark(5);
     hipLaunchKernelGGL(k_write, dim3(n_item_tasks), dim3(kWG), lds, st, a);
+    mark(6);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(MANS_EHIP, "kernel launch: %s", hipGetErrorString(e));
     return MANS_OK;
@@ -435,7 +444,7 @@ extern "C" int mans_decode(const mans_decode_args* d) {
     uint32_t nct = (sp.n_chunks + kWG - 1) / kWG;
     uint32_t nit = (sp.n_ck + kWG - 1) / kWG;
     a.n_tasks = nct;
-    rc = launch_pipeline(a, nct, nit, 1, sp.L, (hipStream_t)d->stream);
+    rc = launch_pipeline(a, nct, nit, 1, sp.L, (hipStream_t)d->stream, d->kernel_events);
     if (rc) return rc;
     if (d->flags & MANS_F_SYNC_ERRORS) {
         int status = 0;

@@ -67,7 +67,8 @@ class DeviceStream:
         self.ws = torch.zeros(workspace_bytes(es.n_units, es.L, chunk_units, checkpoint_units),
                               dtype=torch.uint8, device=device)
 
-    def args(self, stream=None, chunk_units=0, checkpoint_units=0, flags=0, n_out=None):
+    def args(self, stream=None, chunk_units=0, checkpoint_units=0, flags=0, n_out=None,
+             kernel_events=None):
         torch = _torch()
         st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         a = _lib.DecodeArgs()
@@ -85,6 +86,7 @@ class DeviceStream:
         a.workspace = self.ws.data_ptr()
         a.workspace_bytes = self.ws.numel()
         a.stream = st
+        a.kernel_events = kernel_events
         return a
 
     def decode(self, **kw):

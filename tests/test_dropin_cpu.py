@@ -1,0 +1,99 @@
+"""CPU checks of the drop-in C++ layer and the C ABI (no GPU calls).
+
+* libmultians_amd.so loads and exports every function include/mans_c_api.h declares.
+* The drop-in ANSTableGenerator / ANSEncoder (libcuhd_amd.so, driven by mans_demo, a
+  program shaped like the reference's src/main.cc) produce the reference's counts,
+  decoder tables, test data and encoded units for every demo lambda (pinned against
+  the reference's own code compiled into oracle/_ref).
+* The product host codec (mans_* C functions) equals the oracle's.
+"""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "multians_amd", "lib")
+
+
+def _declared(header):
+    src = open(os.path.join(ROOT, "include", header)).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(mans_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_c_abi_exports_everything_declared():
+    lib = C.CDLL(os.path.join(LIB, "libmultians_amd.so"))
+    names = _declared("mans_c_api.h")
+    assert len(names) > 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    from multians_amd import _lib
+    assert sorted(_lib.EXPORTS) == names
+
+
+def test_c_abi_errors_without_gpu():
+    from multians_amd import _lib
+    L = _lib.lib()
+    assert L.mans_decode(None) == _lib.MANS_EINVAL
+    assert b"NULL" in L.mans_last_error()
+    assert L.mans_error_string(_lib.MANS_ETABLE) == b"decoder table rejected"
+
+
+def _fnv(b):
+    h = 1469598103934665603
+    for x in b:
+        h ^= x
+        h = (h * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+@pytest.mark.skipif(not ol.ref_available(), reason="oracle/_ref not built")
+def test_dropin_codec_matches_reference():
+    out = subprocess.run([os.path.join(LIB, "mans_demo"), "0", "0", "--no-gpu", "--checksums"],
+                         check=True, stdout=subprocess.PIPE, text=True).stdout
+    rows = [l.split() for l in out.splitlines() if l.startswith("CHK")]
+    assert len(rows) == 15
+    R = ol.ref()
+    n, L = 100000, 1024
+    for r in rows:
+        kv = dict(x.split("=") for x in r[2:])
+        lam = float(np.float32(r[1]))
+        counts = np.zeros(256, np.uint32)
+        dec = np.zeros(L, np.uint32)
+        en, ec, el = (np.zeros(256 * L, np.uint32) for _ in range(3))
+        R.ref_demo_tables(lam, 256, L, 5, counts, dec, en, ec, el)
+        assert int(kv["counts"]) == _fnv(counts.tobytes())
+        assert int(kv["dec"]) == _fnv(dec.tobytes())
+        data = np.zeros(n + 8, np.uint8)
+        R.ref_gen_test_data(counts, 256, n, L, 5, data)
+        assert int(kv["data"]) == _fnv(data[:n].tobytes())
+        units = np.zeros(n, np.uint32)
+        fb, fs = C.c_uint32(), C.c_uint32()
+        nu = R.ref_encode(counts, 256, L, data, n, units, n, C.byref(fb), C.byref(fs))
+        units = units[:nu].copy()
+        units[0] &= np.uint32((0xFFFFFFFF << (32 - fb.value)) & 0xFFFFFFFF) if fb.value else 0
+        assert (int(kv["nunits"]), int(kv["first_bit"]), int(kv["first_state"])) == (nu, fb.value, fs.value)
+        assert int(kv["units"]) == _fnv(units.tobytes())
+
+
+@pytest.mark.parametrize("lam,L", [(0.002, 4096), (0.1, 1024), (1.0, 4096), (3.0, 256)])
+def test_product_codec_matches_oracle(lam, L):
+    from multians_amd import _lib
+    from multians_amd.decoder import encode_symbols
+    sym = ol.gen_symbols(77, lam, 400000)
+    assert np.array_equal(_lib.gen_symbols(77, lam, 400000), sym)
+    es = encode_symbols(sym, L)
+    st, counts = ol.encode(sym, L=L)
+    assert np.array_equal(es.counts, counts)
+    assert np.array_equal(es.table, st.table)
+    assert np.array_equal(es.units, st.units)
+    assert (es.first_bit, es.first_state) == (st.first_bit, st.first_state)
+    enc, _ = _lib.build_tables(counts, L)
+    _, encarr, _ = ol.build_tables(counts, L)
+    assert np.array_equal(enc.reshape(256, L, 3), encarr)

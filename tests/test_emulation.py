@@ -1,0 +1,98 @@
+"""The decode pipeline's per-lane logic (multians_amd/csrc/mans_core.h), run on the CPU
+by the emulation build (libmans_emu.so, phases in kernel order), against the oracle.
+
+This checks the algorithm -- speculative chunks, merge walks, fixup rounds, scan,
+count-driven writes -- for many geometries, including tiny chunks that force long
+chains of failed walks, and the checkpoint table (entries + output offsets) against the
+oracle's true path.  The GPU tests run the same logic on the device.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+
+
+@pytest.fixture(scope="module")
+def emu():
+    lib = C.CDLL(os.path.join(ROOT, "multians_amd", "lib", "libmans_emu.so"))
+    lib.mans_emu_decode.argtypes = [_u32p, C.c_uint64, _u8p, C.c_uint64, _u32p, C.c_uint32,
+                                    C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _u64p,
+                                    C.c_void_p, C.c_void_p]
+    return lib
+
+
+def _run(emu, st, S, K, n_out=None, want_ck=False):
+    n_out = st.n if n_out is None else n_out
+    out = np.zeros(max(n_out, 1), np.uint8)
+    stats = np.zeros(8, np.uint64)
+    nq = -(-len(st.units) // K)
+    ent = np.zeros(nq, np.uint32)
+    off = np.zeros(nq, np.uint64)
+    rc = emu.mans_emu_decode(st.units, len(st.units), out, n_out, st.table, st.L, st.first_state,
+                             st.first_bit, S, K, stats, ent.ctypes.data, off.ctypes.data)
+    assert rc == 0
+    return out[:n_out], stats, ent, off
+
+
+@pytest.mark.parametrize("lam", [0.002, 0.01, 0.1, 1.0, 3.0])
+@pytest.mark.parametrize("S,K", [(16, 4), (64, 8), (256, 16), (1024, 16), (2048, 64)])
+def test_pipeline_matches_oracle(emu, lam, S, K):
+    sym = ol.gen_symbols(31, lam, 1_000_000)
+    st, _ = ol.encode(sym, L=4096)
+    out, stats, ent, off = _run(emu, st, S, K)
+    assert np.array_equal(out[::-1], sym)
+    ref, entries, counts, _, _ = ol.decode_seq(st, S=K)
+    assert np.array_equal(out, ref)
+    # checkpoint entries are the true path's; offsets are the exclusive scan of
+    # per-K-unit step counts
+    assert np.array_equal(ent, entries)
+    assert np.array_equal(off, np.concatenate([[0], np.cumsum(counts.astype(np.uint64))[:-1]]))
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_small_streams(emu, seed):
+    rng = np.random.default_rng(seed)
+    L = int(rng.choice([64, 256, 1024, 4096]))
+    n = int(rng.integers(1, 20000))
+    lam = float(rng.choice([0.01, 0.3, 1.0, 4.0]))
+    sym = ol.gen_symbols(seed, lam, n)
+    if len(np.unique(sym)) > L:
+        L = 4096
+    st, _ = ol.encode(sym, L=L)
+    K = int(rng.choice([1, 2, 4, 8]))
+    S = K * int(rng.choice([1, 2, 4, 16]))
+    out, *_ = _run(emu, st, S, K)
+    assert np.array_equal(out[::-1], sym)
+
+
+def test_zero_bit_tail_not_lost(emu):
+    """Trailing zero-bit symbols at bit 32*n_units are produced for any chunk size
+    (the reference loses them when n_units % S == 0)."""
+    L = 1024
+    counts = np.ones(256, np.uint32)
+    counts[7] = L - 255
+    rng = np.random.default_rng(1)
+    for n in range(3000, 3040):
+        data = np.where(rng.random(n) < 0.9, 7, rng.integers(0, 256, n)).astype(np.uint8)
+        data[:5] = 7
+        st, _ = ol.encode(data, counts, L)
+        for S, K in [(4, 4), (4, 1), (8, 2)]:
+            out, *_ = _run(emu, st, S, K)
+            assert np.array_equal(out[::-1], data)
+
+
+def test_partial_output(emu):
+    sym = ol.gen_symbols(5, 0.2, 50000)
+    st, _ = ol.encode(sym, L=1024)
+    full, *_ = _run(emu, st, 64, 8)
+    for n_out in [1, 17, 4096, 49999]:
+        part, *_ = _run(emu, st, 64, 8, n_out=n_out)
+        assert np.array_equal(part, full[:n_out])
