@@ -57,6 +57,11 @@ def make_streams(lams, n, L, seed0=1000):
     return res
 
 
+def _lib_names():
+    from multians_amd import _lib
+    return _lib.KERNEL_NAMES
+
+
 def kernel_times(ds, torch, reps=3):
     """Per-kernel durations (ms) of one decode, from HIP events on the decode stream."""
     from multians_amd import _lib
@@ -155,7 +160,7 @@ def bench_plain(args, torch):
                     "ratio": round(n / (4.0 * ds.es.n_units), 3),
                     "hbm_frac": round(algo / ms / 1e6 / HBM_PEAK_GBS, 4),
                     "kernels_ms": {k: round(float(v), 4) for k, v in
-                                   zip(["init", "spec", "walk", "fixup", "scan", "write"], kt)}})
+                                   zip(_lib_names(), kt)}})
         log(json.dumps(per[-1]))
     for _ in range(args.warmup):
         for ds in devs:
@@ -178,11 +183,10 @@ def bench_plain(args, torch):
         for k, v in p["kernels_ms"].items():
             ktot[k] = ktot.get(k, 0.0) + v
     dom = max(ktot, key=ktot.get)
-    # algorithmic bytes per launch: spec/walk read the compressed units (4 B/unit); the
-    # write kernel reads them and writes the decoded bytes; scan/fixup/init are small.
+    # algorithmic bytes per launch: k_sync reads the compressed units (4 B/unit); the
+    # write kernel reads them and writes the decoded bytes; the others are small.
     units_all = sum(ds.es.n_units for ds in devs)
-    algo = {"spec": 4.0 * units_all, "walk": 4.0 * units_all,
-            "write": 4.0 * units_all + total}.get(dom, 4.0 * units_all)
+    algo = {"sync": 4.0 * units_all, "write": 4.0 * units_all + total}.get(dom, 4.0 * units_all)
     ach = algo / (ktot[dom] / 1e3) / 1e9
     roof = {"bound": "hbm", "kernel": "k_" + dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,

@@ -106,7 +106,7 @@ typedef struct mans_decode_args {
                                    `stream` around the kernels (timing), or NULL */
 } mans_decode_args;
 
-/* kernel_events[i] is recorded before kernel i (0 init, 1 spec, 2 walk, 3 fixup,
+/* kernel_events[i] is recorded before kernel i (0 init, 1 sync, 2 seam, 3 fixup,
  * 4 scan, 5 write); kernel_events[6] after the last one. */
 #define MANS_N_KERNEL_EVENTS 7
 

@@ -37,7 +37,7 @@ class DecodeArgs(C.Structure):
 
 
 N_KERNEL_EVENTS = 7
-KERNEL_NAMES = ["init", "spec", "walk", "fixup", "scan", "write"]
+KERNEL_NAMES = ["init", "sync", "seam", "fixup", "scan", "write"]
 
 
 class StreamDesc(C.Structure):

@@ -58,12 +58,16 @@ MANS_HD uint32_t entry_off(uint32_t v) { return (v >> 16) & 63u; }
 // sets *bad when the entry cannot be decoded by the fast step: the next state must
 // stay inside [L, 2L) for every value of the nbits read, and a zero-bit entry must
 // strictly decrease the state (so zero-bit runs are finite).
-MANS_HD uint32_t derive_entry(uint32_t e, uint32_t L, uint32_t index, int* bad) {
+// A zero-bit entry that does not decrease the state (a table with a symbol owning all
+// L states has only such entries) sets *cyc: position-driven decoding could then loop
+// forever, so those tables are only decoded when the stream has no coded bits beyond
+// its first checkpoint interval (count-driven write pass only).
+MANS_HD uint32_t derive_entry(uint32_t e, uint32_t L, uint32_t index, int* bad, int* cyc) {
     uint32_t next = e & 0xFFFFu, sym = (e >> 16) & 0xFFu, nb = e >> 24;
     if (nb > 15u || next == 0u) { *bad = 1; return 0u; }
     uint32_t lo = next << nb, hi = (next + 1u) << nb;
     if (lo < L || hi > 2u * L) { *bad = 1; return 0u; }
-    if (nb == 0u && lo - L >= index) { *bad = 1; return 0u; }
+    if (nb == 0u && lo - L >= index) *cyc = 1;
     return nb | (sym << 8) | ((lo - L) << 16);
 }
 
@@ -95,8 +99,21 @@ struct Cursor {
     uint32_t lo, hi, nx;
 };
 
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const uint4 __attribute__((address_space(1)))* gptr_u4;
+typedef const uint32_t __attribute__((address_space(1)))* gptr_u32;
+#endif
+
+// Unit i of a stream, zero at or beyond n.  Branch-free: the address is clamped to a
+// valid unit and the value masked.
 MANS_HD uint32_t load_unit(const uint32_t* units, uint32_t n, uint32_t i) {
-    return i < n ? units[i] : 0u;
+    const uint32_t ia = i < n ? i : 0u;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t v = ((gptr_u32)units)[ia];
+#else
+    const uint32_t v = units[ia];
+#endif
+    return i < n ? v : 0u;
 }
 
 // Position the cursor at absolute bit (32*unit + off), off in [0, 32].
@@ -127,6 +144,90 @@ MANS_HD uint32_t step(const Tab& tab, Cursor& c, uint32_t& xi, uint32_t& crossed
         c.lo = c.hi;
         c.hi = c.nx;
         c.nx = load_unit(c.units, c.n, c.u + 2u);
+        crossed = 1u;
+    }
+    return w;
+}
+
+// Cursor for long sequential walks (spec and walk passes): units arrive in aligned
+// blocks of four (one 16-byte load).  Two block registers A and B alternate: the
+// cursor reads units from one while the other's load is in flight, and a block is
+// reloaded in place once consumed (no register copies of data still in flight, which
+// would make the prefetch wait).
+struct U4 { uint32_t x, y, z, w; };
+
+// Aligned block of four units starting at b (b % 4 == 0), zeros at or beyond n.
+// Branch-free (clamped address, masked values).  Reads whole 16-byte blocks, so the
+// buffer must be readable up to n rounded up to a multiple of four units.
+MANS_HD U4 load_block(const uint32_t* units, uint32_t n, uint32_t b) {
+    const uint32_t ba = b < n ? b : 0u;
+    U4 r;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint4 v = ((gptr_u4)units)[ba >> 2];
+    r.x = v.x; r.y = v.y; r.z = v.z; r.w = v.w;
+#else
+    r.x = units[ba]; r.y = ba + 1u < n ? units[ba + 1u] : 0u;
+    r.z = ba + 2u < n ? units[ba + 2u] : 0u; r.w = ba + 3u < n ? units[ba + 3u] : 0u;
+#endif
+    r.x = b < n ? r.x : 0u;
+    r.y = b + 1u < n ? r.y : 0u;
+    r.z = b + 2u < n ? r.z : 0u;
+    r.w = b + 3u < n ? r.w : 0u;
+    return r;
+}
+
+MANS_HD uint32_t pick4(const U4& v, uint32_t k) {
+    return k == 0u ? v.x : (k == 1u ? v.y : (k == 2u ? v.z : v.w));
+}
+
+struct BCursor {
+    const uint32_t* units;
+    uint32_t n;
+    uint32_t u;       // unit of lo
+    uint32_t o;       // bit offset in lo
+    uint32_t lo, hi;
+    uint32_t k;       // position of unit u+2 in the 8-unit ring A|B (0..7)
+    uint32_t nb;      // unit index of the next block to load
+    U4 A, B;
+};
+
+// Position at absolute bit 32*unit + off (off in [0, 32]).
+MANS_HD void bcursor_init(BCursor& c, const uint32_t* units, uint32_t n, uint32_t unit, uint32_t off) {
+    c.units = units;
+    c.n = n;
+    c.u = unit + (off >> 5);
+    c.o = off & 31u;
+    const uint32_t bb = c.u & ~3u;            // A = block of u, B = the next block
+    c.A = load_block(units, n, bb);
+    c.B = load_block(units, n, bb + 4u);
+    const uint32_t k0 = c.u - bb;             // 0..3
+    c.lo = pick4(c.A, k0);
+    c.hi = k0 == 3u ? c.B.x : pick4(c.A, k0 + 1u);
+    c.k = k0 + 2u;
+    c.nb = bb + 8u;
+}
+
+MANS_HD void bcursor_advance(BCursor& c) {
+    c.u += 1u;
+    c.lo = c.hi;
+    const uint32_t k = c.k;
+    c.hi = k < 4u ? pick4(c.A, k) : pick4(c.B, k - 4u);
+    c.k = (k + 1u) & 7u;
+    // a block is reloaded once its last unit has moved into hi
+    if (k == 3u) { c.A = load_block(c.units, c.n, c.nb); c.nb += 4u; }
+    if (k == 7u) { c.B = load_block(c.units, c.n, c.nb); c.nb += 4u; }
+}
+
+template <typename Tab>
+MANS_HD uint32_t bstep(const Tab& tab, BCursor& c, uint32_t& xi, uint32_t& crossed) {
+    uint32_t w = tab[xi];
+    uint32_t win = funnel_shr(c.hi, c.lo, c.o);
+    xi = (w >> 16) + low_bits(win, w);
+    c.o += w & 31u;
+    crossed = 0u;
+    if (c.o >= 32u) {
+        c.o -= 32u;
+        bcursor_advance(c);
         crossed = 1u;
     }
     return w;
@@ -176,6 +277,7 @@ struct Work {
     uint32_t* flist;     // fail list (global chunk ids)
     uint32_t* flist2;
     uint32_t* ctick;     // task tickets: [0] spec, [1] walk, [2] write
+    uint64_t* bsum;      // scan block sums
 };
 
 // ---------------------------------------------------------------- lane bodies
@@ -187,7 +289,7 @@ MANS_HD void lane_spec(const StreamPlan& sp, const Work& w, const Tab& tab, uint
     const uint32_t u0 = c * sp.S;
     uint32_t u_stop = u0 + sp.S;
     if (u_stop > sp.n_units) u_stop = sp.n_units;
-    Cursor cur;
+    BCursor cur;
     uint32_t xi, off0;
     if (c == 0u) {
         off0 = 32u - sp.first_bit;
@@ -196,7 +298,7 @@ MANS_HD void lane_spec(const StreamPlan& sp, const Work& w, const Tab& tab, uint
         off0 = 0u;
         xi = sp.first_state - L;   // guessed state (the reference guesses the same)
     }
-    cursor_init(cur, sp.units, sp.n_units, u0, off0);
+    bcursor_init(cur, sp.units, sp.n_units, u0, off0);
     const uint64_t qb = sp.ck_base;
     const uint32_t gc = sp.chunk_base + c;
     const uint32_t kmask = sp.K - 1u;
@@ -210,7 +312,7 @@ MANS_HD void lane_spec(const StreamPlan& sp, const Work& w, const Tab& tab, uint
     uint32_t cnt = 0u;
     while (cur.u < u_stop) {
         uint32_t crossed;
-        step(tab, cur, xi, crossed);
+        bstep(tab, cur, xi, crossed);
         ++cnt;
         if (crossed && (cur.u & kmask) == 0u && cur.u < u_stop) {
             uint64_t q = qb + (cur.u >> sp.log2K);
@@ -242,13 +344,13 @@ MANS_HD uint32_t lane_walk(const StreamPlan& sp, const Work& w, const Tab& tab, 
     }
     w.wb[qb + q0] = entry;
     w.wbc[qb + q0] = 0u;
-    Cursor cur;
+    BCursor cur;
     uint32_t xi = entry_state(entry);
-    cursor_init(cur, sp.units, sp.n_units, u0, entry_off(entry));
+    bcursor_init(cur, sp.units, sp.n_units, u0, entry_off(entry));
     uint32_t cnt = 0u;
     while (cur.u < u_stop) {
         uint32_t crossed;
-        step(tab, cur, xi, crossed);
+        bstep(tab, cur, xi, crossed);
         ++cnt;
         if (crossed && (cur.u & kmask) == 0u && cur.u < u_stop) {
             uint32_t q = cur.u >> sp.log2K;
@@ -298,12 +400,15 @@ MANS_HD void checkpoint_truth(const StreamPlan& sp, const Work& w, uint32_t q, u
     }
 }
 
+// Output offset of checkpoint q within its stream.  coff holds the exclusive scan of
+// chunk counts over all chunks of all streams; the stream's own offsets start at
+// coff[chunk_base].
 MANS_HD uint64_t checkpoint_offset(const StreamPlan& sp, const Work& w, uint32_t q,
                                    uint32_t* entry) {
     uint32_t cum;
     checkpoint_truth(sp, w, q, entry, &cum);
     const uint32_t c = (q << sp.log2K) / sp.S;
-    return w.coff[sp.chunk_base + c] + cum;
+    return w.coff[sp.chunk_base + c] - w.coff[sp.chunk_base] + cum;
 }
 
 }  // namespace mans

@@ -33,9 +33,9 @@ int mans_emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint6
                     uint64_t* ck_offsets_out) {
     if (!n_out) return 0;
     std::vector<uint32_t> tab(L);
-    int bad = 0;
-    for (uint32_t i = 0; i < L; ++i) tab[i] = derive_entry(table[i], L, i, &bad);
-    if (bad) return -4;
+    int bad = 0, cyc = 0;
+    for (uint32_t i = 0; i < L; ++i) tab[i] = derive_entry(table[i], L, i, &bad, &cyc);
+    if (bad || (cyc && (n_units + K - 1) / K > 1)) return -4;
     StreamPlan sp;
     std::memset(&sp, 0, sizeof(sp));
     sp.units = units;
@@ -64,8 +64,14 @@ int mans_emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint6
     w.flist2 = fl2.data(); w.ctick = nullptr;
     HostTab ht{tab.data()};
     uint64_t st_fail = 0, st_rounds = 0, st_rewalks = 0, st_merges = 0;
-    // k_spec
-    for (uint32_t c = 0; c < nc; ++c) lane_spec(sp, w, ht, c);
+    // k_spec (a table with zero-bit cycles has a single checkpoint interval here:
+    // only the chunk-0 start is recorded)
+    if (cyc) {
+        ck[0] = pack_entry(first_state - L, 32u - first_bit);
+        ckc[0] = 0; mq[0] = 0; wcm[0] = 0;
+    } else {
+        for (uint32_t c = 0; c < nc; ++c) lane_spec(sp, w, ht, c);
+    }
     // k_walk
     for (uint32_t c = 1; c < nc; ++c) {
         uint32_t ok = lane_walk(sp, w, ht, c, exitv[c - 1]);

@@ -3,10 +3,11 @@
 //
 // Replaces cuhd::CUHDGPUDecoder::decode and its kernels
 // (src/multians_gpu_decoder.cu:195-687 of the reference):
-//   phase1_decode_subseq           -> k_spec   (speculative decode, checkpoints)
-//   phase2_synchronise_blocks+loop -> k_walk + k_fixup (merge detection on device,
-//                                     no host round trip)
-//   phase3 copies + thrust scan    -> k_scan   (per-stream exclusive scan, u64)
+//   phase1_decode_subseq           -> k_sync   (speculative decode + checkpoints, then
+//                                     merge walks inside the workgroup's range)
+//   phase2_synchronise_blocks+loop -> k_seam + k_fixup (walks across workgroup ranges,
+//                                     fixup rounds; on device, no host round trip)
+//   phase3 copies + thrust scan    -> k_scan_reduce/top/down (u64 exclusive scan)
 //   phase4_decode_write_output     -> k_write  (count-driven decode + store)
 // The per-lane logic lives in mans_core.h.
 #include <hip/hip_runtime.h>
@@ -25,7 +26,6 @@ using namespace mans;
 namespace {
 
 constexpr int kWG = 256;           // lanes per workgroup (4 waves)
-constexpr int kScanWG = 1024;
 
 struct LdsTab {
     const uint32_t* p;
@@ -67,15 +67,23 @@ __device__ uint32_t stream_of_chunk(const KArgs& a, uint32_t gc) {
     return lo;
 }
 
-// Stage the derived table of stream sp into LDS; flags invalid entries.
-__device__ void stage_table(const StreamPlan& sp, uint32_t* lds, uint32_t* status) {
-    int bad = 0;
+// Stage the derived table of stream sp into LDS.  Returns (uniform) 0, or 1 when the
+// table has zero-bit cycles (see derive_entry); invalid tables set status[0] bit 0.
+__device__ uint32_t stage_table(const StreamPlan& sp, uint32_t* lds, uint32_t* status) {
+    __shared__ uint32_t s_flags;
+    if (threadIdx.x == 0) s_flags = 0u;
+    __syncthreads();
+    int bad = 0, cyc = 0;
     for (uint32_t i = threadIdx.x; i < sp.L; i += blockDim.x) {
-        uint32_t e = sp.table[i];
-        lds[i] = derive_entry(e, sp.L, i, &bad);
+        uint32_t e = load_unit(sp.table, sp.L, i);
+        lds[i] = derive_entry(e, sp.L, i, &bad, &cyc);
     }
     if (bad) atomicOr(&status[0], 1u);
+    if (cyc) atomicOr(&s_flags, 1u);
     __syncthreads();
+    const uint32_t f = s_flags;
+    if (f && sp.n_ck > 1u && threadIdx.x == 0) atomicOr(&status[0], 1u);
+    return __builtin_amdgcn_readfirstlane(f);
 }
 
 __global__ void k_init(Work w, uint32_t n_status) {
@@ -83,61 +91,293 @@ __global__ void k_init(Work w, uint32_t n_status) {
     if (threadIdx.x < 4) w.ctick[threadIdx.x] = 0u;
 }
 
-__global__ __launch_bounds__(kWG) void k_spec(KArgs a) {
-    extern __shared__ uint32_t lds[];
-    __shared__ uint32_t s_stream;
-    const uint32_t t = blockIdx.x;
-    if (threadIdx.x == 0) s_stream = stream_of_task<false>(a, t);
-    __syncthreads();
-    const StreamPlan& sp = *plan_of(a, s_stream);
-    stage_table(sp, lds, a.w.status);
-    const uint32_t c = (t - sp.task_chunk) * kWG + threadIdx.x;
-    if (c >= sp.n_chunks) return;
-    LdsTab tab{lds};
-    lane_spec(sp, a.w, tab, c);
-}
+// ------------------------------------------------------------ round engine
+// Spec and walk decodes run in lockstep rounds per workgroup: in round r, lane l
+// decodes the steps starting in units [rs_l, rs_l + kR) of its region, rs_l =
+// rbase_l + r*kR, reading its units from an LDS buffer.  The units of round r+1
+// (kRL per lane, one 16-byte load per 4 units) are loaded into registers before
+// round r is decoded and written to LDS after it, so global-memory latency overlaps
+// a whole round of decoding.  A round is exactly one checkpoint interval (kR = K),
+// so checkpoints and merge tests happen between rounds, outside the step loop.
+constexpr uint32_t kR = 32;    // units decoded per round (= checkpoint stride)
+constexpr uint32_t kRL = 36;   // units staged per lane per round (kR + lookahead)
+constexpr uint32_t kLS = 36;   // LDS stride per lane in units (16-byte multiple)
+constexpr uint32_t kPieces = kRL / 4;   // 16-byte loads per lane segment (9)
 
-__global__ __launch_bounds__(kWG) void k_walk(KArgs a) {
-    extern __shared__ uint32_t lds[];
-    __shared__ uint32_t s_stream;
-    const uint32_t t = blockIdx.x;
-    if (threadIdx.x == 0) s_stream = stream_of_task<false>(a, t);
-    __syncthreads();
-    const StreamPlan& sp = *plan_of(a, s_stream);
-    stage_table(sp, lds, a.w.status);
-    const uint32_t c = (t - sp.task_chunk) * kWG + threadIdx.x;
-    if (c == 0u || c >= sp.n_chunks) return;
-    LdsTab tab{lds};
-    const uint32_t gc = sp.chunk_base + c;
-    uint32_t ok = lane_walk(sp, a.w, tab, c, a.w.exitv[gc - 1u]);
-    a.w.fflag[gc] = 0u;
-    if (!ok && c + 1u < sp.n_chunks && a.w.wexit[gc] != a.w.exitv[gc]) {
-        // the walk into c+1 started from exitv[c], which is not chunk c's exit
-        uint32_t slot = atomicAdd(&a.w.status[1], 1u);
-        a.w.flist[slot] = gc + 1u;
-    }
-}
-
-// Fixup rounds: one workgroup; tables are read from global memory (rare path).
-struct GlobalTabView {
-    const uint32_t* raw;
-    uint32_t L;
-    __device__ __forceinline__ uint32_t operator[](uint32_t i) const {
-        int bad = 0;
-        return derive_entry(raw[i], L, i, &bad);
-    }
+struct RoundIO {
+    uint32_t* buf;        // LDS, kWG * kLS units
+    uint32_t* rbase;      // LDS, kWG: first unit of round 0 per lane (kNone = no load)
+    const uint32_t* units;
+    uint32_t n;
 };
 
-__global__ __launch_bounds__(kWG) void k_fixup(KArgs a) {
-    // The list holds chunks whose recorded walk start (wentry) may differ from the
-    // current exit of their predecessor.  Each round walks them again from that exit
-    // and lists any chunk (itself, or its successor) still inconsistent afterwards.
-    // The leftmost listed chunk always starts from a proven exit, so the rounds end.
+struct Prefetch { uint4 v[kPieces]; };
+
+// issue the loads of round r for all lanes: pieces 0-7 of lane t/8 + 32j by thread t
+// (eight threads per 128-byte lane segment), piece 8 of lane t by thread t
+__device__ __forceinline__ void round_load(const RoundIO& io, uint32_t r, Prefetch& pf) {
+    const uint32_t t = threadIdx.x;
+#pragma unroll
+    for (uint32_t j = 0; j < kPieces; ++j) {
+        const uint32_t lane = j < 8u ? (t >> 3) + 32u * j : t;
+        const uint32_t piece = j < 8u ? (t & 7u) : 8u;
+        const uint32_t rb = io.rbase[lane];
+        const uint32_t b = rb + r * kR + 4u * piece;
+        const U4 u = load_block(io.units, rb == kNone ? 0u : io.n, b);
+        pf.v[j] = make_uint4(u.x, u.y, u.z, u.w);
+    }
+}
+
+__device__ __forceinline__ void round_store(const RoundIO& io, const Prefetch& pf) {
+    const uint32_t t = threadIdx.x;
+#pragma unroll
+    for (uint32_t j = 0; j < kPieces; ++j) {
+        const uint32_t lane = j < 8u ? (t >> 3) + 32u * j : t;
+        const uint32_t piece = j < 8u ? (t & 7u) : 8u;
+        *reinterpret_cast<uint4*>(io.buf + lane * kLS + 4u * piece) = pf.v[j];
+    }
+}
+
+// Per-lane decode state.
+struct RLane {
+    uint32_t u, o, xi, cnt, u_stop, rbase;
+    uint32_t active;
+};
+
+// Decode the lane's steps of round r (those starting in units [rs, min(rs+kR, u_stop))).
+// Branch-free step: the unit window (lo, hi) and the next unit n1 shift on a crossing
+// by selects; n1 is re-read from LDS every step (off the dependency chain).
+__device__ __forceinline__ void round_decode(const LdsTab& tab, const uint32_t* lbuf, RLane& L,
+                                             uint32_t r) {
+    if (!L.active) return;
+    const uint32_t rs = L.rbase + r * kR;
+    const uint32_t re = min(rs + kR, L.u_stop);
+    if (L.u >= re) return;
+    uint32_t ui = L.u - rs;                  // unit index within the staged segment
+    const uint32_t ue = re - rs;
+    uint32_t o = L.o, xi = L.xi, cnt = L.cnt;
+    uint32_t lo = lbuf[ui], hi = lbuf[ui + 1u], n1 = lbuf[ui + 2u];
+    do {
+        const uint32_t w = tab[xi];
+        const uint32_t win = funnel_shr(hi, lo, o);
+        xi = (w >> 16) + low_bits(win, w);
+        o += w & 31u;
+        ++cnt;
+        const bool c = o >= 32u;
+        o &= 31u;
+        lo = c ? hi : lo;
+        hi = c ? n1 : hi;
+        ui += c ? 1u : 0u;
+        n1 = lbuf[ui + 2u];
+    } while (ui < ue);
+    L.u = rs + ui; L.o = o; L.xi = xi; L.cnt = cnt;
+}
+
+// Run rounds (at most nr_max, uniform) until no lane is active.  After each round,
+// at_end(L, re) is called for lanes that reached a round boundary re < u_stop (their
+// state is the entry of unit re); it may deactivate the lane.
+template <typename F>
+__device__ void run_rounds(const LdsTab& tab, RoundIO& io, RLane& L, uint32_t nr_max, F&& at_end) {
+    __syncthreads();                       // rbase[] visible
+    Prefetch pf;
+    round_load(io, 0u, pf);
+    for (uint32_t r = 0; r < nr_max; ++r) {
+        __syncthreads();                   // previous round's reads of buf done
+        round_store(io, pf);
+        __syncthreads();
+        if (r + 1u < nr_max) round_load(io, r + 1u, pf);
+        round_decode(tab, io.buf + threadIdx.x * kLS, L, r);
+        if (L.active) {
+            const uint32_t re = L.rbase + (r + 1u) * kR;
+            if (re < L.u_stop) {
+                if (L.u == re) at_end(L, re);
+            } else {
+                L.active = 0u;             // region done
+            }
+        }
+        if (!__syncthreads_or(L.active)) break;
+    }
+}
+
+__device__ __forceinline__ uint32_t ld_coherent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Walk into chunk c (c >= 1) of sp from `entry` for the lanes with want != 0, in rounds.
+// Same semantics as mans::lane_walk with K == kR.  Returns 1 if merged.
+__device__ uint32_t walk_rounds(const StreamPlan& sp, const Work& w, const LdsTab& tab,
+                                RoundIO& io, uint32_t want, uint32_t c, uint32_t entry) {
+    const uint64_t qb = sp.ck_base;
+    const uint32_t gc = sp.chunk_base + c;
+    RLane L;
+    L.active = 0u;
+    uint32_t merged = 0u, late = 0u, mq = 0u, wcm = 0u;
+    if (want) {
+        const uint32_t u0 = c * sp.S;
+        L.u_stop = min(u0 + sp.S, sp.n_units);
+        L.rbase = u0;
+        w.wentry[gc] = entry;
+        const uint32_t q0 = u0 >> sp.log2K;
+        if (ld_coherent(&w.ck[qb + q0]) == entry) {
+            w.mq[gc] = q0;
+            w.wcm[gc] = 0u;
+            merged = 1u;
+        } else {
+            w.wb[qb + q0] = entry;
+            w.wbc[qb + q0] = 0u;
+            L.active = 1u;
+            L.xi = entry_state(entry);
+            L.u = u0 + (entry_off(entry) >> 5);
+            L.o = entry_off(entry) & 31u;
+            L.cnt = 0u;
+        }
+    }
+    io.rbase[threadIdx.x] = L.active ? L.rbase : kNone;
+    run_rounds(tab, io, L, (sp.S + kR - 1u) / kR, [&](RLane& l, uint32_t re) {
+        const uint32_t q = re >> sp.log2K;
+        const uint32_t v = pack_entry(l.xi, l.o);
+        if (ld_coherent(&w.ck[qb + q]) == v) {
+            mq = q;
+            wcm = l.cnt;
+            merged = 1u;
+            late = 1u;
+            l.active = 0u;
+            return;
+        }
+        w.wb[qb + q] = v;
+        w.wbc[qb + q] = l.cnt;
+    });
+    if (want && late) {                 // merged during the rounds
+        w.mq[gc] = mq;
+        w.wcm[gc] = wcm;
+    } else if (want && !merged) {
+        w.mq[gc] = kNone;
+        w.wexit[gc] = pack_entry(L.xi, L.o);
+        w.wtotal[gc] = L.cnt;
+    }
+    return merged;
+}
+
+// Spec pass for 256 consecutive chunks of one stream, then the walks into chunks
+// c0+1 .. c0+255 (their predecessors' exits are this workgroup's own results).
+__global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
     extern __shared__ uint32_t lds[];
-    __shared__ uint32_t s_n, s_uni, s_first;
+    __shared__ uint32_t s_stream;
+    __shared__ uint32_t s_rbase[kWG];
+    const uint32_t t = blockIdx.x;
+    if (threadIdx.x == 0) s_stream = stream_of_task<false>(a, t);
+    __syncthreads();
+    const StreamPlan sp = *plan_of(a, __builtin_amdgcn_readfirstlane(s_stream));
+    const uint32_t cyc = stage_table(sp, lds, a.w.status);
+    LdsTab tab{lds};
+    RoundIO io{lds + sp.L, s_rbase, sp.units, sp.n_units};
+    const Work& w = a.w;
+    const uint32_t c0 = (t - sp.task_chunk) * kWG;
+    const uint32_t c = c0 + threadIdx.x;
+    if (cyc) {
+        // zero-bit cycles: only a single checkpoint interval is decodable (count-driven)
+        if (c == 0u && sp.n_chunks) {
+            const uint32_t off0 = 32u - sp.first_bit;
+            w.ck[sp.ck_base] = pack_entry(sp.first_state - sp.L, off0);
+            w.ckc[sp.ck_base] = 0u;
+            w.mq[sp.chunk_base] = 0u;
+            w.wcm[sp.chunk_base] = 0u;
+        }
+        return;
+    }
+    const bool valid = c < sp.n_chunks;
+    const uint64_t qb = sp.ck_base;
+    // ---- spec
+    RLane L;
+    L.active = valid ? 1u : 0u;
+    if (valid) {
+        const uint32_t u0 = c * sp.S;
+        const uint32_t off0 = c == 0u ? 32u - sp.first_bit : 0u;
+        L.rbase = u0;
+        L.u_stop = min(u0 + sp.S, sp.n_units);
+        L.xi = sp.first_state - sp.L;
+        L.u = u0 + (off0 >> 5);
+        L.o = off0 & 31u;
+        L.cnt = 0u;
+        w.ck[qb + (u0 >> sp.log2K)] = pack_entry(L.xi, off0);
+        w.ckc[qb + (u0 >> sp.log2K)] = 0u;
+        if (c == 0u) {
+            w.mq[sp.chunk_base] = 0u;
+            w.wcm[sp.chunk_base] = 0u;
+            w.wentry[sp.chunk_base] = pack_entry(L.xi, off0);
+        }
+    }
+    s_rbase[threadIdx.x] = valid ? L.rbase : kNone;
+    run_rounds(tab, io, L, (sp.S + kR - 1u) / kR, [&](RLane& l, uint32_t re) {
+        w.ck[qb + (re >> sp.log2K)] = pack_entry(l.xi, l.o);
+        w.ckc[qb + (re >> sp.log2K)] = l.cnt;
+    });
+    const uint32_t my_exit = pack_entry(L.xi, L.o);
+    if (valid) {
+        w.exitv[sp.chunk_base + c] = my_exit;
+        w.own_total[sp.chunk_base + c] = L.cnt;
+    }
+    __threadfence();
+    __syncthreads();
+    // ---- walks into c+1 from this lane's exit
+    const uint32_t tgt = c + 1u;
+    const uint32_t want = (threadIdx.x + 1u < kWG && tgt < sp.n_chunks) ? 1u : 0u;
+    const uint32_t ok = walk_rounds(sp, w, tab, io, want, tgt, my_exit);
+    if (want) {
+        const uint32_t gt = sp.chunk_base + tgt;
+        w.fflag[gt] = 0u;
+        if (!ok && tgt + 1u < sp.n_chunks && w.wexit[gt] != ld_coherent(&w.exitv[gt])) {
+            uint32_t slot = atomicAdd(&w.status[1], 1u);
+            w.flist[slot] = gt + 1u;
+        }
+    }
+}
+
+// Walks into the first chunk of each workgroup range (chunks c0 = 256k, k >= 1), from
+// the exit of the previous workgroup's last chunk.
+__global__ __launch_bounds__(kWG) void k_seam(KArgs a) {
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t s_stream;
+    __shared__ uint32_t s_rbase[kWG];
+    const uint32_t t = blockIdx.x;
+    if (threadIdx.x == 0) s_stream = stream_of_task<false>(a, t);
+    __syncthreads();
+    const StreamPlan sp = *plan_of(a, __builtin_amdgcn_readfirstlane(s_stream));
+    // task t of this stream covers seams (t - task_chunk)*256 + lane, seam k -> chunk 256k
+    const uint32_t k = (t - sp.task_chunk) * kWG + threadIdx.x;
+    const uint32_t tgt = k * kWG;
+    const uint32_t want = (k >= 1u && tgt < sp.n_chunks) ? 1u : 0u;
+    if (!__syncthreads_or(want)) return;
+    if (stage_table(sp, lds, a.w.status)) return;
+    LdsTab tab{lds};
+    RoundIO io{lds + sp.L, s_rbase, sp.units, sp.n_units};
+    const Work& w = a.w;
+    const uint32_t gt = sp.chunk_base + tgt;
+    const uint32_t entry = want ? w.exitv[gt - 1u] : 0u;
+    const uint32_t ok = walk_rounds(sp, w, tab, io, want, tgt, entry);
+    if (want) {
+        w.fflag[gt] = 0u;
+        if (!ok && tgt + 1u < sp.n_chunks && w.wexit[gt] != w.exitv[gt]) {
+            uint32_t slot = atomicAdd(&w.status[1], 1u);
+            w.flist[slot] = gt + 1u;
+        }
+    }
+}
+
+// Fixup rounds (one workgroup).  The list holds chunks whose recorded walk start
+// (wentry) may differ from the current exit of their predecessor.  Each round walks
+// them again from that exit and lists any chunk (itself, or its successor) still
+// inconsistent afterwards.  The leftmost listed chunk always starts from a proven
+// exit, so the rounds end.  Walks of one stream at a time share the LDS table.
+__global__ __launch_bounds__(kWG) void k_fixup(KArgs a) {
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t s_n, s_sid, s_have, s_left;
+    __shared__ uint32_t s_rbase[kWG];
     uint32_t* list = a.w.flist;
     uint32_t* next = a.w.flist2;
     uint32_t* status = a.w.status;
+    const Work& w = a.w;
+    if (threadIdx.x == 0) s_have = kNone;
     for (uint32_t round = 0;; ++round) {
         __syncthreads();
         if (threadIdx.x == 0)
@@ -145,115 +385,266 @@ __global__ __launch_bounds__(kWG) void k_fixup(KArgs a) {
         __syncthreads();
         const uint32_t n = s_n;
         if (n == 0u) break;
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {      // capture + mark
-            uint32_t g = list[i];
+        for (uint32_t i = threadIdx.x; i < n; i += kWG) {        // capture + mark
+            const uint32_t g = list[i];
             const StreamPlan& sp = *plan_of(a, stream_of_chunk(a, g));
-            next[i] = chunk_exit(sp, a.w, g - sp.chunk_base - 1u);
-            a.w.fflag[g] = 1u;
+            next[i] = chunk_exit(sp, w, g - sp.chunk_base - 1u);
+            w.fflag[g] = 1u;
         }
+        __threadfence();
         __syncthreads();
-        // one stream listed (always so for a single stream): stage its table in LDS
-        if (threadIdx.x == 0) { s_uni = 1u; s_first = stream_of_chunk(a, list[0]); }
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
-            if (stream_of_chunk(a, list[i]) != s_first) s_uni = 0u;
-        __syncthreads();
-        if (s_uni) {
-            const StreamPlan& sp = *plan_of(a, s_first);
-            stage_table(sp, lds, status);
-            LdsTab tab{lds};
-            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
-                lane_walk(sp, a.w, tab, list[i] - sp.chunk_base, next[i]);
-        } else {
-            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-                uint32_t g = list[i];
-                const StreamPlan& sp = *plan_of(a, stream_of_chunk(a, g));
-                GlobalTabView tab{sp.table, sp.L};
-                lane_walk(sp, a.w, tab, g - sp.chunk_base, next[i]);
+        for (uint32_t b0 = 0; b0 < n; b0 += kWG) {               // walk again
+            const uint32_t i = b0 + threadIdx.x;
+            const uint32_t g = i < n ? list[i] : 0u;
+            const uint32_t sid = i < n ? stream_of_chunk(a, g) : kNone;
+            uint32_t pending = i < n ? 1u : 0u;
+            for (;;) {
+                if (threadIdx.x == 0) s_left = 0u;
+                __syncthreads();
+                if (pending) s_sid = sid;                      // any pending lane's stream
+                __syncthreads();
+                const uint32_t cur = __builtin_amdgcn_readfirstlane(s_sid);
+                if (!__syncthreads_or(pending)) break;
+                const StreamPlan sp = *plan_of(a, cur);
+                if (s_have != cur) {
+                    (void)stage_table(sp, lds, status);
+                    if (threadIdx.x == 0) s_have = cur;
+                }
+                LdsTab tab{lds};
+                RoundIO io{lds + sp.L, s_rbase, sp.units, sp.n_units};
+                const uint32_t mine = (pending && sid == cur) ? 1u : 0u;
+                walk_rounds(sp, w, tab, io, mine, mine ? g - sp.chunk_base : 1u, mine ? next[i] : 0u);
+                if (mine) pending = 0u;
+                __syncthreads();
             }
         }
         __threadfence();
         __syncthreads();
         if (threadIdx.x == 0) status[2] = 0u;
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {      // consistency
-            uint32_t g = list[i];
+        for (uint32_t i = threadIdx.x; i < n; i += kWG) {        // consistency
+            const uint32_t g = list[i];
             const StreamPlan& sp = *plan_of(a, stream_of_chunk(a, g));
-            uint32_t c = g - sp.chunk_base;
-            if (a.w.wentry[g] != chunk_exit(sp, a.w, c - 1u)) {
+            const uint32_t c = g - sp.chunk_base;
+            if (w.wentry[g] != chunk_exit(sp, w, c - 1u)) {
                 uint32_t slot = atomicAdd(&status[2], 1u);
                 next[slot] = g;
             }
-            if (c + 1u < sp.n_chunks && !a.w.fflag[g + 1u] &&
-                a.w.wentry[g + 1u] != chunk_exit(sp, a.w, c)) {
+            if (c + 1u < sp.n_chunks && !w.fflag[g + 1u] &&
+                w.wentry[g + 1u] != chunk_exit(sp, w, c)) {
                 uint32_t slot = atomicAdd(&status[2], 1u);
                 next[slot] = g + 1u;
             }
         }
         __threadfence();
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) a.w.fflag[list[i]] = 0u;
+        for (uint32_t i = threadIdx.x; i < n; i += kWG) w.fflag[list[i]] = 0u;
         if (threadIdx.x == 0) {
             status[1] = __hip_atomic_load(&status[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            status[3] += 1u;                                              // rounds
+            status[3] += 1u;
         }
         __threadfence();
         uint32_t* tmp = list; list = next; next = tmp;
     }
 }
 
-// Per-stream exclusive scan of chunk step counts -> coff (u64).  One workgroup per
-// stream.
-__global__ __launch_bounds__(kScanWG) void k_scan(KArgs a) {
-    __shared__ uint64_t s_part[kScanWG];
-    const StreamPlan& sp = *plan_of(a, blockIdx.x);
-    const uint32_t n = sp.n_chunks;
-    const uint32_t per = (n + kScanWG - 1u) / kScanWG;
-    const uint32_t b = threadIdx.x * per;
-    const uint32_t e = min(n, b + per);
-    uint64_t sum = 0;
-    for (uint32_t c = b; c < e; ++c) sum += chunk_count(sp, a.w, c);
-    s_part[threadIdx.x] = sum;
-    __syncthreads();
-    // Hillis-Steele inclusive scan over kScanWG partials
-    for (uint32_t d = 1; d < kScanWG; d <<= 1) {
-        uint64_t v = threadIdx.x >= d ? s_part[threadIdx.x - d] : 0ull;
-        __syncthreads();
-        s_part[threadIdx.x] += v;
-        __syncthreads();
+// Exclusive scan of chunk step counts over all chunks of all streams (three passes:
+// block reduce, scan of block sums, block scan + carry).  A stream's offsets are the
+// global scan minus the value at its first chunk (see checkpoint_offset).
+constexpr uint32_t kScanBlock = 2048;   // chunks per scan block (256 lanes x 8)
+
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v) {
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t d = 1; d < 64u; d <<= 1) {
+        uint64_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
     }
-    uint64_t run = threadIdx.x ? s_part[threadIdx.x - 1] : 0ull;
-    for (uint32_t c = b; c < e; ++c) {
-        a.w.coff[sp.chunk_base + c] = run;
-        run += chunk_count(sp, a.w, c);
+    return v;
+}
+
+// inclusive scan over the workgroup (kWG lanes); returns the total in *tot
+__device__ uint64_t wg_incl_scan(uint64_t v, uint64_t* s_w, uint64_t* tot) {
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    v = wave_incl_scan(v);
+    if (lane == 63u) s_w[wv] = v;
+    __syncthreads();
+    uint64_t add = 0;
+    for (uint32_t i = 0; i < wv; ++i) add += s_w[i];
+    *tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    __syncthreads();
+    return v + add;
+}
+
+__device__ __forceinline__ uint64_t count_of_global(const KArgs& a, uint32_t gc, uint32_t total) {
+    if (gc >= total) return 0ull;
+    const StreamPlan& sp = *plan_of(a, stream_of_chunk(a, gc));
+    if (gc - sp.chunk_base >= sp.n_chunks) return 0ull;
+    return chunk_count(sp, a.w, gc - sp.chunk_base);
+}
+
+__global__ __launch_bounds__(kWG) void k_scan_reduce(KArgs a, uint32_t total, uint64_t* bsum) {
+    __shared__ uint64_t s_w[4];
+    const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * (kScanBlock / kWG);
+    uint64_t v = 0;
+    for (uint32_t i = 0; i < kScanBlock / kWG; ++i) {
+        uint64_t c = count_of_global(a, base + i, total);
+        if (base + i < total) a.w.coff[base + i] = c;
+        v += c;
+    }
+    uint64_t tot;
+    wg_incl_scan(v, s_w, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kWG) void k_scan_top(uint64_t* bsum, uint32_t nb) {
+    __shared__ uint64_t s_w[4];
+    __shared__ uint64_t s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (uint32_t b0 = 0; b0 < nb; b0 += kWG) {
+        uint32_t i = b0 + threadIdx.x;
+        uint64_t v = i < nb ? bsum[i] : 0ull, tot;
+        uint64_t incl = wg_incl_scan(v, s_w, &tot);
+        uint64_t carry = s_carry;
+        if (i < nb) bsum[i] = carry + incl - v;   // exclusive
+        __syncthreads();
+        if (threadIdx.x == 0) s_carry = carry + tot;
+        __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(kWG) void k_write(KArgs a) {
-    extern __shared__ uint32_t lds[];
-    __shared__ uint32_t s_stream;
-    const uint32_t t = blockIdx.x;
-    if (threadIdx.x == 0) s_stream = stream_of_task<true>(a, t);
-    __syncthreads();
-    const StreamPlan& sp = *plan_of(a, s_stream);
-    stage_table(sp, lds, a.w.status);
-    const uint32_t q = (t - sp.task_item) * kWG + threadIdx.x;
-    if (q >= sp.n_ck) return;
-    uint32_t entry, entry2;
-    uint64_t off = checkpoint_offset(sp, a.w, q, &entry);
-    uint64_t end = (q + 1u < sp.n_ck) ? checkpoint_offset(sp, a.w, q + 1u, &entry2) : sp.n_out;
-    if (end > sp.n_out) end = sp.n_out;
-    if (off >= end) return;
-    uint64_t cnt = end - off;
-    LdsTab tab{lds};
-    Cursor cur;
-    cursor_init(cur, sp.units, sp.n_units, q << sp.log2K, entry_off(entry));
+__global__ __launch_bounds__(kWG) void k_scan_down(KArgs a, uint32_t total, const uint64_t* bsum) {
+    __shared__ uint64_t s_w[4];
+    const uint32_t per = kScanBlock / kWG;
+    const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * per;
+    uint64_t c[kScanBlock / kWG];
+    uint64_t v = 0;
+    for (uint32_t i = 0; i < per; ++i) {
+        c[i] = base + i < total ? a.w.coff[base + i] : 0ull;
+        v += c[i];
+    }
+    uint64_t tot;
+    uint64_t run = bsum[blockIdx.x] + wg_incl_scan(v, s_w, &tot) - v;
+    for (uint32_t i = 0; i < per; ++i) {
+        if (base + i < total) a.w.coff[base + i] = run;
+        run += c[i];
+    }
+}
+
+// ---------------------------------------------------------------- write pass
+// Persistent workgroups; each takes a contiguous range of passes, a pass being 256
+// consecutive checkpoint intervals ("items", kR units each) of one stream.  A pass's
+// units are staged in LDS like a round of the sync kernels (lane segment = item), the
+// next pass's units load while the current pass decodes, every lane decodes its
+// item's step count from the item's true entry, and symbols are packed four to a
+// dword store (byte stores for the unaligned head and tail).
+constexpr uint32_t kMaxItemUnits = kR;
+
+__device__ __forceinline__ void pass_load(const uint32_t* units, uint32_t n, uint32_t u_first,
+                                          Prefetch& pf) {
+    const uint32_t t = threadIdx.x;
+#pragma unroll
+    for (uint32_t j = 0; j < kPieces; ++j) {
+        const uint32_t lane = j < 8u ? (t >> 3) + 32u * j : t;
+        const uint32_t piece = j < 8u ? (t & 7u) : 8u;
+        const U4 u = load_block(units, n, u_first + lane * kR + 4u * piece);
+        pf.v[j] = make_uint4(u.x, u.y, u.z, u.w);
+    }
+}
+
+__device__ __forceinline__ void item_write(const LdsTab& tab, const uint32_t* lbuf, uint32_t entry,
+                                           uint8_t* dst, uint32_t cnt) {
+    uint32_t ui = entry_off(entry) >> 5;
+    uint32_t o = entry_off(entry) & 31u;
+    uint32_t lo = lbuf[ui], hi = lbuf[ui + 1u], n1 = lbuf[ui + 2u];
     uint32_t xi = entry_state(entry);
-    uint8_t* o = sp.out + off;
-    for (uint64_t i = 0; i < cnt; ++i) {
-        uint32_t crossed;
-        uint32_t wd = step(tab, cur, xi, crossed);
-        o[i] = (uint8_t)(wd >> 8);
+    auto next_sym = [&]() -> uint32_t {
+        const uint32_t w = tab[xi];
+        const uint32_t win = funnel_shr(hi, lo, o);
+        xi = (w >> 16) + low_bits(win, w);
+        o += w & 31u;
+        const bool c = o >= 32u;
+        o &= 31u;
+        lo = c ? hi : lo;
+        hi = c ? n1 : hi;
+        ui += c ? 1u : 0u;
+        n1 = lbuf[min(ui + 2u, kRL - 1u)];
+        return w;
+    };
+    uint32_t i = 0;
+    const uint32_t head = (uint32_t)((4u - ((uintptr_t)dst & 3u)) & 3u);
+    const uint32_t h = head < cnt ? head : cnt;
+    for (; i < h; ++i) dst[i] = (uint8_t)(next_sym() >> 8);
+    uint32_t* dw = (uint32_t*)(dst + i);
+    const uint32_t nd = (cnt - i) >> 2;
+    for (uint32_t d = 0; d < nd; ++d) {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t w = next_sym();
+            // acc = (acc >> 8) | (symbol << 24): byte 1 of w goes on top
+            acc = __builtin_amdgcn_perm(w, acc, 0x05030201u);
+        }
+        dw[d] = acc;
+    }
+    i += nd << 2;
+    for (; i < cnt; ++i) dst[i] = (uint8_t)(next_sym() >> 8);
+}
+
+__global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint32_t per_wg) {
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t s_stream, s_have;
+    __shared__ uint64_t s_off[kWG + 1];
+    const uint32_t p0 = blockIdx.x * per_wg;
+    const uint32_t p1 = min(n_passes, p0 + per_wg);
+    if (p0 >= p1) return;
+    if (threadIdx.x == 0) s_have = kNone;
+    // prefetch the first pass
+    if (threadIdx.x == 0) s_stream = stream_of_task<true>(a, p0);
+    __syncthreads();
+    Prefetch pf;
+    {
+        const StreamPlan sp = *plan_of(a, __builtin_amdgcn_readfirstlane(s_stream));
+        pass_load(sp.units, sp.n_units, ((p0 - sp.task_item) * kWG) << sp.log2K, pf);
+    }
+    for (uint32_t t = p0; t < p1; ++t) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_stream = stream_of_task<true>(a, t);
+        __syncthreads();
+        const uint32_t sid = __builtin_amdgcn_readfirstlane(s_stream);
+        const StreamPlan sp = *plan_of(a, sid);
+        uint32_t* buf = lds + sp.L;
+        if (s_have != sid) {
+            (void)stage_table(sp, lds, a.w.status);    // ends with a barrier
+            if (threadIdx.x == 0) s_have = sid;
+        }
+        const uint32_t q0 = (t - sp.task_item) * kWG;
+        const uint32_t nitems = min((uint32_t)kWG, sp.n_ck - q0);
+        RoundIO io{buf, nullptr, sp.units, sp.n_units};
+        round_store(io, pf);
+        const uint32_t q = q0 + threadIdx.x;
+        uint32_t entry = 0;
+        if (threadIdx.x < nitems) s_off[threadIdx.x] = checkpoint_offset(sp, a.w, q, &entry);
+        if (threadIdx.x == 0) {
+            uint32_t e2;
+            s_off[nitems] = (q0 + nitems < sp.n_ck) ? checkpoint_offset(sp, a.w, q0 + nitems, &e2)
+                                                    : sp.n_out;
+        }
+        __syncthreads();
+        if (t + 1u < p1) {
+            // the next pass may belong to the next stream
+            const uint32_t ns = stream_of_task<true>(a, t + 1u);
+            const StreamPlan np = *plan_of(a, __builtin_amdgcn_readfirstlane(ns));
+            pass_load(np.units, np.n_units, ((t + 1u - np.task_item) * kWG) << np.log2K, pf);
+        }
+        if (threadIdx.x < nitems) {
+            uint64_t off = s_off[threadIdx.x], end = s_off[threadIdx.x + 1];
+            if (end > sp.n_out) end = sp.n_out;
+            if (off < end) {
+                LdsTab tab{lds};
+                item_write(tab, buf + threadIdx.x * kLS, entry, sp.out + off, (uint32_t)(end - off));
+            }
+        }
     }
 }
 
@@ -270,17 +661,21 @@ static uint32_t ilog2(uint32_t v) { uint32_t r = 0; while ((1u << (r + 1)) <= v)
 // close to one decode of the stream.
 void choose_geometry(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t req_S,
                      uint32_t req_K, uint32_t* S, uint32_t* K) {
-    uint32_t k = req_K ? req_K : 16u;
-    if (k < 1u) k = 1u;
-    k = 1u << ilog2(k);
-    uint32_t s = req_S ? req_S : 0u;
+    // checkpoints sit at round boundaries of the sync kernels: K is the round length
+    (void)req_K;
+    uint32_t k = kR;
+    uint32_t s = req_S;
     if (!s) {
-        // symbols per unit drives the per-unit decode cost and the merge distance
+        // Sync chunks must be long against the distance over which a guessed-state
+        // decode merges with the true path, or walks fail and need fixup rounds.
+        // Measured at L = 4096 (units, 99.9th percentile): 851 at 4.3 symbols/unit,
+        // 1832 at 5.5, ~4000-4600 at 8.5-21, ~600 at 100+.
         double spu = n_units ? (double)n_out / (double)n_units : 4.0;
-        s = spu > 12.0 ? 4096u : (spu > 6.0 ? 2048u : 1024u);
-        (void)L;
-        // keep enough chunks to fill the device for mid-size streams
-        while (s > 256u && n_units / s < 2048u) s >>= 1;
+        if (spu <= 4.6) s = 2048u;
+        else if (spu <= 6.5) s = 4096u;
+        else if (spu <= 40.0) s = 8192u;
+        else s = 2048u;
+        if (L <= 1024u) s >>= 1;
     }
     s = 1u << ilog2(s < k ? k : s);
     *S = s;
@@ -291,7 +686,7 @@ void choose_geometry(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t req_
 struct Layout {
     uint64_t chunks = 0, cks = 0, bytes = 0;
     uint64_t o_exitv, o_own, o_went, o_wexit, o_wtot, o_mq, o_wcm, o_ff, o_coff;
-    uint64_t o_ck, o_ckc, o_wb, o_wbc, o_status, o_fl, o_fl2, o_tick, o_plans;
+    uint64_t o_ck, o_ckc, o_wb, o_wbc, o_status, o_fl, o_fl2, o_tick, o_plans, o_bsum;
 };
 
 static uint64_t align256(uint64_t v) { return (v + 255u) & ~255ull; }
@@ -319,6 +714,7 @@ Layout make_layout(uint64_t chunks, uint64_t cks, uint64_t n_plans) {
     l.o_fl = take(4 * chunks + 4);
     l.o_fl2 = take(4 * chunks + 4);
     l.o_tick = take(4 * 8);
+    l.o_bsum = take(8 * (chunks / 2048 + 2));
     l.o_plans = take(sizeof(StreamPlan) * n_plans);
     l.bytes = o;
     return l;
@@ -344,26 +740,41 @@ Work make_work(void* base, const Layout& l) {
     w.flist = (uint32_t*)(b + l.o_fl);
     w.flist2 = (uint32_t*)(b + l.o_fl2);
     w.ctick = (uint32_t*)(b + l.o_tick);
+    w.bsum = (uint64_t*)(b + l.o_bsum);
     return w;
 }
 
 static int launch_pipeline(const KArgs& a, uint32_t n_chunk_tasks, uint32_t n_item_tasks,
-                           uint32_t n_scan_blocks, uint32_t maxL, hipStream_t st,
+                           uint32_t total_chunks, uint32_t maxL, uint32_t maxK, hipStream_t st,
                            void** ev = nullptr) {
     size_t lds = (size_t)maxL * 4u;
-    auto mark = [&](int i) { if (ev) hipEventRecord((hipEvent_t)ev[i], st); };
+    size_t lds_s = lds + (size_t)kWG * kLS * 4u;   // table + round buffer
+    auto mark = [&](int i) { if (ev) (void)hipEventRecord((hipEvent_t)ev[i], st); };
     mark(0);
     hipLaunchKernelGGL(k_init, dim3(1), dim3(64), 0, st, a.w, 64u);
     mark(1);
-    hipLaunchKernelGGL(k_spec, dim3(n_chunk_tasks), dim3(kWG), lds, st, a);
+    hipLaunchKernelGGL(k_sync, dim3(n_chunk_tasks), dim3(kWG), lds_s, st, a);
     mark(2);
-    hipLaunchKernelGGL(k_walk, dim3(n_chunk_tasks), dim3(kWG), lds, st, a);
+    hipLaunchKernelGGL(k_seam, dim3(n_chunk_tasks), dim3(kWG), lds_s, st, a);
     mark(3);
-    hipLaunchKernelGGL(k_fixup, dim3(1), dim3(kWG), lds, st, a);
+    hipLaunchKernelGGL(k_fixup, dim3(1), dim3(kWG), lds_s, st, a);
     mark(4);
-    hipLaunchKernelGGL(k_scan, dim3(n_scan_blocks), dim3(kScanWG), 0, st, a);
+    {
+        uint32_t nb = (total_chunks + kScanBlock - 1) / kScanBlock;
+        uint64_t* bsum = a.w.bsum;
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kWG), 0, st, a, total_chunks, bsum);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kWG), 0, st, bsum, nb);
+        hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kWG), 0, st, a, total_chunks, bsum);
+    }
     mark(5);
-    hipLaunchKernelGGL(k_write, dim3(n_item_tasks), dim3(kWG), lds, st, a);
+    {
+        // persistent write pass: about 8 workgroups per CU, contiguous pass ranges
+        uint32_t nwg = std::min<uint32_t>(n_item_tasks, 2048u);
+        uint32_t per = (n_item_tasks + nwg - 1) / nwg;
+        nwg = (n_item_tasks + per - 1) / per;
+        (void)maxK;
+        hipLaunchKernelGGL(k_write, dim3(nwg), dim3(kWG), lds_s, st, a, n_item_tasks, per);
+    }
     mark(6);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(MANS_EHIP, "kernel launch: %s", hipGetErrorString(e));
@@ -377,6 +788,9 @@ static int check_stream_args(const uint32_t* units, uint64_t n_units, const uint
     if (L < 2u || L > 16384u) return set_error(MANS_EINVAL, "L=%u outside [2, 16384]", L);
     if (n_out && (!units || !out)) return set_error(MANS_EINVAL, "units/out is NULL");
     if (n_out && n_units == 0) return set_error(MANS_EINVAL, "n_units is 0 but n_out > 0");
+    if (((uintptr_t)units & 15u) != 0u)
+        return set_error(MANS_EINVAL, "units must be 16-byte aligned (device allocations are)");
+    // units must also be readable up to n_units rounded up to 4 (whole 16-byte loads)
     if (n_units >= 0xFFFFFF00ull) return set_error(MANS_EINVAL, "n_units too large");
     if (first_bit > 32u) return set_error(MANS_EINVAL, "first_bit=%u > 32", first_bit);
     if (n_out && (first_state < L || first_state >= 2u * L))
@@ -444,7 +858,7 @@ extern "C" int mans_decode(const mans_decode_args* d) {
     uint32_t nct = (sp.n_chunks + kWG - 1) / kWG;
     uint32_t nit = (sp.n_ck + kWG - 1) / kWG;
     a.n_tasks = nct;
-    rc = launch_pipeline(a, nct, nit, 1, sp.L, (hipStream_t)d->stream, d->kernel_events);
+    rc = launch_pipeline(a, nct, nit, sp.n_chunks, sp.L, sp.K, (hipStream_t)d->stream, d->kernel_events);
     if (rc) return rc;
     if (d->flags & MANS_F_SYNC_ERRORS) {
         int status = 0;
@@ -473,10 +887,11 @@ namespace mans {
 // Plans for a batch: per-stream geometry, global chunk/checkpoint bases, task bases.
 static int build_plans(const mans_stream_desc* d, size_t n, std::vector<StreamPlan>& plans,
                        uint64_t* chunks, uint64_t* cks, uint32_t* chunk_tasks,
-                       uint32_t* item_tasks, uint32_t* maxL, uint64_t* units, uint64_t* syms) {
+                       uint32_t* item_tasks, uint32_t* maxL, uint32_t* maxK, uint64_t* units,
+                       uint64_t* syms) {
     plans.resize(n);
     uint64_t cb = 0, qb = 0, tc = 0, ti = 0, tu = 0, ts = 0;
-    uint32_t ml = 2;
+    uint32_t ml = 2, mk = 1;
     for (size_t i = 0; i < n; ++i) {
         int rc = check_stream_args(d[i].units, d[i].n_units, d[i].out, d[i].n_out, d[i].table,
                                    d[i].L, d[i].first_state, d[i].first_bit);
@@ -513,6 +928,7 @@ static int build_plans(const mans_stream_desc* d, size_t n, std::vector<StreamPl
         tu += d[i].n_units;
         ts += d[i].n_out;
         ml = std::max(ml, sp.L);
+        mk = std::max(mk, sp.K);
     }
     if (cb >= 0xFFFFFFF0ull || tc >= 0x7FFFFFFFull || ti >= 0x7FFFFFFFull)
         return set_error(MANS_EINVAL, "batch too large");
@@ -521,6 +937,7 @@ static int build_plans(const mans_stream_desc* d, size_t n, std::vector<StreamPl
     *chunk_tasks = (uint32_t)tc;
     *item_tasks = (uint32_t)ti;
     *maxL = ml;
+    *maxK = mk;
     *units = tu;
     *syms = ts;
     return MANS_OK;
@@ -532,8 +949,8 @@ extern "C" int mans_batch_workspace_bytes(const mans_stream_desc* descs, size_t 
     if (!descs || !n || !bytes) return set_error(MANS_EINVAL, "bad batch");
     std::vector<StreamPlan> plans;
     uint64_t chunks, cks, u, s;
-    uint32_t ct, it, ml;
-    int rc = build_plans(descs, n, plans, &chunks, &cks, &ct, &it, &ml, &u, &s);
+    uint32_t ct, it, ml, mk;
+    int rc = build_plans(descs, n, plans, &chunks, &cks, &ct, &it, &ml, &mk, &u, &s);
     if (rc) return rc;
     *bytes = make_layout(chunks, cks, n).bytes;
     return MANS_OK;
@@ -544,8 +961,8 @@ extern "C" int mans_decode_batch(const mans_stream_desc* descs, size_t n,
     if (!descs || !n || !opts) return set_error(MANS_EINVAL, "bad batch");
     std::vector<StreamPlan> plans;
     uint64_t chunks, cks, units, syms;
-    uint32_t ct, it, ml;
-    int rc = build_plans(descs, n, plans, &chunks, &cks, &ct, &it, &ml, &units, &syms);
+    uint32_t ct, it, ml, mk;
+    int rc = build_plans(descs, n, plans, &chunks, &cks, &ct, &it, &ml, &mk, &units, &syms);
     if (rc) return rc;
     Layout l = make_layout(chunks, cks, n);
     if (!opts->workspace || opts->workspace_bytes < l.bytes)
@@ -563,7 +980,7 @@ extern "C" int mans_decode_batch(const mans_stream_desc* descs, size_t n,
     a.n_streams = (uint32_t)n;
     a.n_tasks = ct;
     a.w = make_work(opts->workspace, l);
-    rc = launch_pipeline(a, ct, it, (uint32_t)n, ml, st);
+    rc = launch_pipeline(a, ct, it, (uint32_t)chunks, ml, mk, st);
     if (rc) return rc;
     if (stats) {
         stats->total_units = units;

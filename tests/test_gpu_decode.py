@@ -47,7 +47,25 @@ def test_decode_geometries(S, K):
 
 @pytest.mark.parametrize("n", [1, 2, 3, 7, 100, 1000, 4095, 65537])
 def test_small_streams(n):
+    """n=1 and other single-symbol streams have no coded bits and a table whose only
+    symbol owns every state (zero-bit self-loops; the reference decoder never
+    terminates on them)."""
     print(f"case n={n}", flush=True)
     got, ref, sym = _case(0.5, 1024, n)
     assert np.array_equal(got, ref)
     assert np.array_equal(got[::-1], sym)
+
+
+@pytest.mark.parametrize("n", [1, 50, 100000])
+def test_constant_stream(n):
+    print(f"case constant n={n}", flush=True)
+    import torch
+    from multians_amd.decoder import DeviceStream, encode_symbols
+    sym = np.full(n, 77, np.uint8)
+    es = encode_symbols(sym, 1024)
+    assert es.n_units == 1 and es.first_bit == 0
+    ds = DeviceStream(es)
+    out = ds.decode()
+    torch.cuda.synchronize()
+    assert ds.status() == 0
+    assert np.array_equal(out[:n].cpu().numpy(), sym)
