@@ -273,11 +273,12 @@ struct Work {
     uint32_t* wb;        // walk entry (before merge)
     uint32_t* wbc;       // walk steps since chunk start
     // misc
-    uint32_t* status;    // [0] error bits, [1] fail count, [2] fail count (next round), [3..] counters
-    uint32_t* flist;     // fail list (global chunk ids)
+    uint32_t* status;    // [0] error bits, [3] fixup rounds
+    uint32_t* flist;     // fixup lists (global chunk ids), stream s in its chunk range
     uint32_t* flist2;
     uint32_t* ctick;     // task tickets: [0] spec, [1] walk, [2] write
     uint64_t* bsum;      // scan block sums
+    uint32_t* fcnt;      // per stream: entries in its fixup list
 };
 
 // ---------------------------------------------------------------- lane bodies

@@ -53,7 +53,7 @@ int mans_emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint6
     sp.n_ck = (uint32_t)((n_units + K - 1) / K);
     const uint32_t nc = sp.n_chunks, nq = sp.n_ck;
     std::vector<uint32_t> exitv(nc), own(nc), went(nc), wexit(nc), wtot(nc), mq(nc), wcm(nc),
-        ff(nc), fl(nc + 1), fl2(nc + 1), status(64);
+        ff(nc), fl(nc + 1), fl2(nc + 1), status(64), fcnt(2);
     std::vector<uint64_t> coff(nc);
     std::vector<uint32_t> ck(nq), ckc(nq), wb(nq), wbc(nq);
     Work w;
@@ -61,7 +61,7 @@ int mans_emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint6
     w.wexit = wexit.data(); w.wtotal = wtot.data(); w.mq = mq.data(); w.wcm = wcm.data();
     w.fflag = ff.data(); w.coff = coff.data(); w.ck = ck.data(); w.ckc = ckc.data();
     w.wb = wb.data(); w.wbc = wbc.data(); w.status = status.data(); w.flist = fl.data();
-    w.flist2 = fl2.data(); w.ctick = nullptr;
+    w.flist2 = fl2.data(); w.ctick = nullptr; w.fcnt = fcnt.data(); w.bsum = nullptr;
     HostTab ht{tab.data()};
     uint64_t st_fail = 0, st_rounds = 0, st_rewalks = 0, st_merges = 0;
     // k_spec (a table with zero-bit cycles has a single checkpoint interval here:

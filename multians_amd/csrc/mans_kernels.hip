@@ -86,9 +86,11 @@ __device__ uint32_t stage_table(const StreamPlan& sp, uint32_t* lds, uint32_t* s
     return __builtin_amdgcn_readfirstlane(f);
 }
 
-__global__ void k_init(Work w, uint32_t n_status) {
-    for (uint32_t i = threadIdx.x; i < n_status; i += blockDim.x) w.status[i] = 0u;
-    if (threadIdx.x < 4) w.ctick[threadIdx.x] = 0u;
+__global__ void k_init(Work w, uint32_t n_status, uint32_t n_streams) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_status) w.status[i] = 0u;
+    if (i < 4u) w.ctick[i] = 0u;
+    if (i < n_streams) w.fcnt[i] = 0u;
 }
 
 // ------------------------------------------------------------ round engine
@@ -327,8 +329,8 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
         const uint32_t gt = sp.chunk_base + tgt;
         w.fflag[gt] = 0u;
         if (!ok && tgt + 1u < sp.n_chunks && w.wexit[gt] != ld_coherent(&w.exitv[gt])) {
-            uint32_t slot = atomicAdd(&w.status[1], 1u);
-            w.flist[slot] = gt + 1u;
+            uint32_t slot = atomicAdd(&w.fcnt[s_stream], 1u);
+            w.flist[sp.chunk_base + slot] = gt + 1u;
         }
     }
 }
@@ -358,93 +360,72 @@ __global__ __launch_bounds__(kWG) void k_seam(KArgs a) {
     if (want) {
         w.fflag[gt] = 0u;
         if (!ok && tgt + 1u < sp.n_chunks && w.wexit[gt] != w.exitv[gt]) {
-            uint32_t slot = atomicAdd(&w.status[1], 1u);
-            w.flist[slot] = gt + 1u;
+            uint32_t slot = atomicAdd(&w.fcnt[s_stream], 1u);
+            w.flist[sp.chunk_base + slot] = gt + 1u;
         }
     }
 }
 
-// Fixup rounds (one workgroup).  The list holds chunks whose recorded walk start
-// (wentry) may differ from the current exit of their predecessor.  Each round walks
-// them again from that exit and lists any chunk (itself, or its successor) still
-// inconsistent afterwards.  The leftmost listed chunk always starts from a proven
-// exit, so the rounds end.  Walks of one stream at a time share the LDS table.
+// Fixup rounds.  Workgroup g handles streams g, g+G, ...; a stream's list (in its
+// own chunk range of flist/flist2, count in fcnt) holds chunks whose recorded walk
+// start (wentry) may differ from the current exit of their predecessor.  Each round
+// walks them again from that exit and lists any chunk (itself, or its successor)
+// still inconsistent afterwards.  The leftmost listed chunk always starts from a
+// proven exit, so the rounds end.  Streams are independent, so no grid-wide sync.
 __global__ __launch_bounds__(kWG) void k_fixup(KArgs a) {
     extern __shared__ uint32_t lds[];
-    __shared__ uint32_t s_n, s_sid, s_have, s_left;
+    __shared__ uint32_t s_n, s_n2;
     __shared__ uint32_t s_rbase[kWG];
-    uint32_t* list = a.w.flist;
-    uint32_t* next = a.w.flist2;
-    uint32_t* status = a.w.status;
     const Work& w = a.w;
-    if (threadIdx.x == 0) s_have = kNone;
-    for (uint32_t round = 0;; ++round) {
+    for (uint32_t sid = blockIdx.x; sid < a.n_streams; sid += gridDim.x) {
         __syncthreads();
-        if (threadIdx.x == 0)
-            s_n = __hip_atomic_load(&status[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0) s_n = ld_coherent(&w.fcnt[sid]);
         __syncthreads();
-        const uint32_t n = s_n;
-        if (n == 0u) break;
-        for (uint32_t i = threadIdx.x; i < n; i += kWG) {        // capture + mark
-            const uint32_t g = list[i];
-            const StreamPlan& sp = *plan_of(a, stream_of_chunk(a, g));
-            next[i] = chunk_exit(sp, w, g - sp.chunk_base - 1u);
-            w.fflag[g] = 1u;
-        }
-        __threadfence();
-        __syncthreads();
-        for (uint32_t b0 = 0; b0 < n; b0 += kWG) {               // walk again
-            const uint32_t i = b0 + threadIdx.x;
-            const uint32_t g = i < n ? list[i] : 0u;
-            const uint32_t sid = i < n ? stream_of_chunk(a, g) : kNone;
-            uint32_t pending = i < n ? 1u : 0u;
-            for (;;) {
-                if (threadIdx.x == 0) s_left = 0u;
-                __syncthreads();
-                if (pending) s_sid = sid;                      // any pending lane's stream
-                __syncthreads();
-                const uint32_t cur = __builtin_amdgcn_readfirstlane(s_sid);
-                if (!__syncthreads_or(pending)) break;
-                const StreamPlan sp = *plan_of(a, cur);
-                if (s_have != cur) {
-                    (void)stage_table(sp, lds, status);
-                    if (threadIdx.x == 0) s_have = cur;
-                }
-                LdsTab tab{lds};
-                RoundIO io{lds + sp.L, s_rbase, sp.units, sp.n_units};
-                const uint32_t mine = (pending && sid == cur) ? 1u : 0u;
-                walk_rounds(sp, w, tab, io, mine, mine ? g - sp.chunk_base : 1u, mine ? next[i] : 0u);
-                if (mine) pending = 0u;
-                __syncthreads();
+        if (s_n == 0u) continue;
+        const StreamPlan sp = *plan_of(a, sid);
+        if (stage_table(sp, lds, w.status)) continue;
+        LdsTab tab{lds};
+        RoundIO io{lds + sp.L, s_rbase, sp.units, sp.n_units};
+        uint32_t* list = w.flist + sp.chunk_base;
+        uint32_t* next = w.flist2 + sp.chunk_base;
+        for (uint32_t round = 0;; ++round) {
+            __syncthreads();
+            const uint32_t n = s_n;
+            if (n == 0u) break;
+            for (uint32_t i = threadIdx.x; i < n; i += kWG) {        // capture + mark
+                const uint32_t g = list[i];
+                next[i] = chunk_exit(sp, w, g - sp.chunk_base - 1u);
+                w.fflag[g] = 1u;
             }
-        }
-        __threadfence();
-        __syncthreads();
-        if (threadIdx.x == 0) status[2] = 0u;
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += kWG) {        // consistency
-            const uint32_t g = list[i];
-            const StreamPlan& sp = *plan_of(a, stream_of_chunk(a, g));
-            const uint32_t c = g - sp.chunk_base;
-            if (w.wentry[g] != chunk_exit(sp, w, c - 1u)) {
-                uint32_t slot = atomicAdd(&status[2], 1u);
-                next[slot] = g;
+            __threadfence();
+            __syncthreads();
+            for (uint32_t b0 = 0; b0 < n; b0 += kWG) {               // walk again
+                const uint32_t i = b0 + threadIdx.x;
+                const uint32_t mine = i < n ? 1u : 0u;
+                const uint32_t g = mine ? list[i] : sp.chunk_base + 1u;
+                walk_rounds(sp, w, tab, io, mine, g - sp.chunk_base, mine ? next[i] : 0u);
             }
-            if (c + 1u < sp.n_chunks && !w.fflag[g + 1u] &&
-                w.wentry[g + 1u] != chunk_exit(sp, w, c)) {
-                uint32_t slot = atomicAdd(&status[2], 1u);
-                next[slot] = g + 1u;
+            __threadfence();
+            if (threadIdx.x == 0) s_n2 = 0u;
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < n; i += kWG) {        // consistency
+                const uint32_t g = list[i];
+                const uint32_t c = g - sp.chunk_base;
+                if (w.wentry[g] != chunk_exit(sp, w, c - 1u)) next[atomicAdd(&s_n2, 1u)] = g;
+                if (c + 1u < sp.n_chunks && !w.fflag[g + 1u] &&
+                    w.wentry[g + 1u] != chunk_exit(sp, w, c))
+                    next[atomicAdd(&s_n2, 1u)] = g + 1u;
             }
+            __threadfence();
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < n; i += kWG) w.fflag[list[i]] = 0u;
+            if (threadIdx.x == 0) {
+                s_n = s_n2;
+                atomicAdd(&w.status[3], 1u);                      // rounds (statistics)
+            }
+            __threadfence();
+            uint32_t* tmp = list; list = next; next = tmp;
         }
-        __threadfence();
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += kWG) w.fflag[list[i]] = 0u;
-        if (threadIdx.x == 0) {
-            status[1] = __hip_atomic_load(&status[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            status[3] += 1u;
-        }
-        __threadfence();
-        uint32_t* tmp = list; list = next; next = tmp;
     }
 }
 
@@ -686,7 +667,7 @@ void choose_geometry(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t req_
 struct Layout {
     uint64_t chunks = 0, cks = 0, bytes = 0;
     uint64_t o_exitv, o_own, o_went, o_wexit, o_wtot, o_mq, o_wcm, o_ff, o_coff;
-    uint64_t o_ck, o_ckc, o_wb, o_wbc, o_status, o_fl, o_fl2, o_tick, o_plans, o_bsum;
+    uint64_t o_ck, o_ckc, o_wb, o_wbc, o_status, o_fl, o_fl2, o_tick, o_plans, o_bsum, o_fcnt;
 };
 
 static uint64_t align256(uint64_t v) { return (v + 255u) & ~255ull; }
@@ -715,6 +696,7 @@ Layout make_layout(uint64_t chunks, uint64_t cks, uint64_t n_plans) {
     l.o_fl2 = take(4 * chunks + 4);
     l.o_tick = take(4 * 8);
     l.o_bsum = take(8 * (chunks / 2048 + 2));
+    l.o_fcnt = take(4 * (n_plans + 1));
     l.o_plans = take(sizeof(StreamPlan) * n_plans);
     l.bytes = o;
     return l;
@@ -741,6 +723,7 @@ Work make_work(void* base, const Layout& l) {
     w.flist2 = (uint32_t*)(b + l.o_fl2);
     w.ctick = (uint32_t*)(b + l.o_tick);
     w.bsum = (uint64_t*)(b + l.o_bsum);
+    w.fcnt = (uint32_t*)(b + l.o_fcnt);
     return w;
 }
 
@@ -751,13 +734,14 @@ static int launch_pipeline(const KArgs& a, uint32_t n_chunk_tasks, uint32_t n_it
     size_t lds_s = lds + (size_t)kWG * kLS * 4u;   // table + round buffer
     auto mark = [&](int i) { if (ev) (void)hipEventRecord((hipEvent_t)ev[i], st); };
     mark(0);
-    hipLaunchKernelGGL(k_init, dim3(1), dim3(64), 0, st, a.w, 64u);
+    hipLaunchKernelGGL(k_init, dim3((std::max<uint32_t>(a.n_streams, 64u) + 255u) / 256u), dim3(256),
+                       0, st, a.w, 64u, a.n_streams);
     mark(1);
     hipLaunchKernelGGL(k_sync, dim3(n_chunk_tasks), dim3(kWG), lds_s, st, a);
     mark(2);
     hipLaunchKernelGGL(k_seam, dim3(n_chunk_tasks), dim3(kWG), lds_s, st, a);
     mark(3);
-    hipLaunchKernelGGL(k_fixup, dim3(1), dim3(kWG), lds_s, st, a);
+    hipLaunchKernelGGL(k_fixup, dim3(std::min<uint32_t>(a.n_streams, 512u)), dim3(kWG), lds_s, st, a);
     mark(4);
     {
         uint32_t nb = (total_chunks + kScanBlock - 1) / kScanBlock;

@@ -101,3 +101,68 @@ class DeviceStream:
         _lib.check(_lib.lib().mans_decode_status(C.c_void_p(self.ws.data_ptr()), C.c_void_p(st),
                                                  C.byref(s)))
         return s.value
+
+
+class DeviceBatch:
+    """Many independent streams (each with its own table) resident on the device,
+    decoded together by mans_decode_batch.  Units of every stream start 16-byte
+    aligned and are padded to a multiple of four units; outputs are packed."""
+
+    def __init__(self, streams, device="cuda"):
+        torch = _torch()
+        self.streams = list(streams)
+        n = len(self.streams)
+        uoff = np.zeros(n + 1, np.int64)
+        toff = np.zeros(n + 1, np.int64)
+        ooff = np.zeros(n + 1, np.int64)
+        for i, es in enumerate(self.streams):
+            uoff[i + 1] = uoff[i] + ((es.n_units + 3) // 4 + 1) * 4
+            toff[i + 1] = toff[i] + es.L
+            ooff[i + 1] = ooff[i] + es.n_symbols
+        units = np.zeros(uoff[-1], np.uint32)
+        tables = np.zeros(toff[-1], np.uint32)
+        for i, es in enumerate(self.streams):
+            units[uoff[i]:uoff[i] + es.n_units] = es.units
+            tables[toff[i]:toff[i] + es.L] = es.table
+        self.uoff, self.toff, self.ooff = uoff, toff, ooff
+        self.units = torch.from_numpy(units.view(np.int32)).to(device)
+        self.tables = torch.from_numpy(tables.view(np.int32)).to(device)
+        self.out = torch.zeros(max(int(ooff[-1]), 1), dtype=torch.uint8, device=device)
+        self.descs = (_lib.StreamDesc * n)()
+        ub, tb, ob = self.units.data_ptr(), self.tables.data_ptr(), self.out.data_ptr()
+        for i, es in enumerate(self.streams):
+            d = self.descs[i]
+            d.units = ub + 4 * int(uoff[i])
+            d.n_units = es.n_units
+            d.out = ob + int(ooff[i])
+            d.n_out = es.n_symbols
+            d.table = tb + 4 * int(toff[i])
+            d.L = es.L
+            d.first_state = es.first_state
+            d.first_bit = es.first_bit
+        b = C.c_uint64()
+        _lib.check(_lib.lib().mans_batch_workspace_bytes(self.descs, n, C.byref(b)))
+        self.ws = torch.zeros(b.value, dtype=torch.uint8, device=device)
+
+    def decode(self, stream=None):
+        torch = _torch()
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        o = _lib.BatchOpts()
+        o.workspace = self.ws.data_ptr()
+        o.workspace_bytes = self.ws.numel()
+        o.stream = st
+        stats = _lib.BatchStats()
+        _lib.check(_lib.lib().mans_decode_batch(self.descs, len(self.streams), C.byref(o),
+                                                C.byref(stats)))
+        return stats
+
+    def output(self, i):
+        return self.out[int(self.ooff[i]):int(self.ooff[i + 1])]
+
+    def status(self, stream=None):
+        torch = _torch()
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        s = C.c_int()
+        _lib.check(_lib.lib().mans_decode_status(C.c_void_p(self.ws.data_ptr()), C.c_void_p(st),
+                                                 C.byref(s)))
+        return s.value
