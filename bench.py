@@ -1,26 +1,29 @@
-"""Benchmark: tANS subsequence decoding on one MI355X.
+"""Benchmark: tANS self-synchronising subsequence decoding on one MI355X.
 
-Default workload ("plain", the at-scale single-stream configuration): one tANS stream
-per lambda in {0.01, 0.05, 0.2, 1.0}, each 2^30 uint8 symbols s = min(255, floor(X)),
-X ~ Exp(lambda), with a 4096-state table normalised from the stream's histogram.
-A step decodes all four streams (inputs resident in HBM) through the C ABI; every
-stream is checked bit-exact against its original symbols before timing.
+Headline (`value`): the batched configuration -- 10,000 independent streams, uint8
+sizes log-uniform in [4 KiB, 16 MiB], symbols s = min(255, floor(X)), X ~ Exp(lambda)
+with lambda ~ LogUniform(0.005, 2.0) per stream, each with its own 4096-state table
+normalised from its histogram -- decoded by one batched launch sequence
+(mans_decode_batch).  Beside it: the same streams decoded by per-stream plain launches
+(mans_decode once per stream, in order, one HIP stream), and the single-stream at-scale
+configuration (2^30 symbols for lambda in {0.01, 0.05, 0.2, 1.0}, 4096 states).
 
-Prints one JSON line.  `value` = decoded bytes of all streams / time of the timed
-steps (decoded GB/s).  `roofline` is for the dominant kernel, timed with HIP events
-on the decode stream; `cpu_baseline` times the reference MulticoreDecoder (compiled
-from the reference sources into oracle/_ref, else the oracle's restatement) on a
-bounded sample on this host.
+Every decoded stream is checked bit-exact against its original symbols before timing.
+Inputs are resident in HBM; timing is HIP events on the decode stream.  A step is one
+decode of the whole batch.  `roofline` is for the dominant kernel of the batched
+decode; `cpu_baseline` times the reference MulticoreDecoder (compiled from the
+reference sources into oracle/_ref, else the oracle's restatement) on a bounded sample
+on this host.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload plain|batched]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--streams N] [--no-single]
 """
 import argparse
 import ctypes as C
 import json
 import os
 import sys
-import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -29,197 +32,244 @@ sys.path.insert(0, ROOT)
 
 METRIC = "decoded GB/s, batched vs per-stream plain launch; p50/p99 latency; bit-exact"
 HBM_PEAK_GBS = 8000.0
+THREADS = min(os.cpu_count() or 1, 16)
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def make_streams(lams, n, L, seed0=1000):
-    """Generate + encode one stream per lambda (host, product codec, threads)."""
+def make_streams(specs, keep_symbols=True):
+    """specs: list of (seed, lambda, n, L).  Host generation + encoding (product codec)."""
     from multians_amd import _lib
     from multians_amd.decoder import EncodedStream
-    res = [None] * len(lams)
 
-    def one(i, lam):
-        sym = _lib.gen_symbols(seed0 + i, lam, n, threads=4)
-        hist = np.bincount(sym, minlength=256).astype(np.uint64)
-        counts = _lib.normalize_counts(hist, L)
-        enc, dec = _lib.build_tables(counts, L)
-        units, fb, fs = _lib.encode(sym, enc, L)
-        res[i] = (EncodedStream(units, fb, fs, dec, L, n, counts), sym)
+    def one(sp):
+        seed, lam, n, L = sp
+        units, fb, fs, dec, sym = _lib.make_stream(seed, lam, n, L, keep_symbols)
+        return EncodedStream(units, fb, fs, dec, L, n), sym
 
-    th = [threading.Thread(target=one, args=(i, lam)) for i, lam in enumerate(lams)]
-    for t in th:
-        t.start()
-    for t in th:
-        t.join()
-    return res
+    with ThreadPoolExecutor(THREADS) as ex:
+        return list(ex.map(one, specs))
 
 
-def _lib_names():
-    from multians_amd import _lib
-    return _lib.KERNEL_NAMES
+def batch_specs(n_streams, seed=20261203):
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(np.log(4096), np.log(16 << 20), n_streams))).astype(np.int64)
+    lams = np.exp(rng.uniform(np.log(0.005), np.log(2.0), n_streams))
+    return [(seed * 100003 + i, float(lams[i]), int(sizes[i]), 4096) for i in range(n_streams)]
 
 
-def kernel_times(ds, torch, reps=3):
-    """Per-kernel durations (ms) of one decode, from HIP events on the decode stream."""
-    from multians_amd import _lib
-    evs = [C.c_void_p() for _ in range(_lib.N_KERNEL_EVENTS)]
-    L = _lib.lib()
-    for e in evs:
-        _lib.check(L.mans_event_create(C.byref(e)))
-    arr = (C.c_void_p * _lib.N_KERNEL_EVENTS)(*[e.value for e in evs])
-    tot = np.zeros(_lib.N_KERNEL_EVENTS - 1)
+def verify_batch(b, items, torch):
+    """D2H every stream's output and compare with its symbols (decode order = reversed)."""
+    bad = []
+
+    def chk(i):
+        got = b.output(i).cpu().numpy()
+        if not np.array_equal(got[::-1], items[i][1]):
+            bad.append(i)
+
+    with ThreadPoolExecutor(THREADS) as ex:
+        list(ex.map(chk, range(len(items))))
+    return bad
+
+
+def time_events(torch, fn, reps):
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record()
     for _ in range(reps):
-        ds.decode(kernel_events=C.cast(arr, C.c_void_p))
-        torch.cuda.synchronize()
-        for k in range(_lib.N_KERNEL_EVENTS - 1):
-            ms = C.c_float()
-            _lib.check(L.mans_event_elapsed_ms(evs[k], evs[k + 1], C.byref(ms)))
-            tot[k] += ms.value
-    for e in evs:
-        L.mans_event_destroy(e)
-    return tot / reps
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
 
 
-def cpu_baseline(streams, budget_s=20.0):
-    """Reference MulticoreDecoder on a bounded prefix-sized sample of each stream."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as ol
-    threads = min(os.cpu_count() or 1, 16)
-    use_ref = ol.ref_available()
-    total_bytes, total_s, samples = 0, 0.0, []
-    for es, sym in streams:
-        # sample: the same distribution, 64 Mi symbols, re-encoded (decode of a prefix
-        # of units is not a valid stream)
-        n = 1 << 26
-        st, _ = ol.encode(sym[:n], es.counts, es.L)
-        out = np.zeros(n, np.uint8)
-        units = np.concatenate([st.units, np.zeros(16, np.uint32)])
-        t0 = time.perf_counter()
-        if use_ref:
-            us = ol.ref().ref_multicore(4, threads, units, len(st.units), st.table, es.L,
-                                        st.first_state, st.first_bit, out, n)
-            dt = us / 1e6
-        else:
-            ol.oracle().mo_decode_multicore(4, threads, units, len(st.units), st.table, es.L,
-                                            st.first_state, st.first_bit, out, n)
-            dt = time.perf_counter() - t0
-        assert np.array_equal(out[::-1], sym[:n]), "cpu baseline decode mismatch"
-        total_bytes += n
-        total_s += dt
-        samples.append("%d Mi symbols" % (n >> 20))
-        if total_s > budget_s:
-            break
-    return {"value": round(total_bytes / total_s / 1e9, 4), "unit": "GB/s", "cores": threads,
-            "kind": "reference" if use_ref else "port",
-            "sample": "MulticoreDecoder S=4 (the reference demo's), %d threads, on %d re-encoded "
-                      "streams of 2^26 symbols from the same lambdas/tables" % (threads, len(samples)),
-            "host_logical_cpus": os.cpu_count()}
+def bench_batched(args, torch):
+    from multians_amd import _lib
+    from multians_amd.decoder import DeviceBatch, workspace_bytes
+    t0 = time.time()
+    specs = batch_specs(args.streams)
+    items = make_streams(specs)
+    total_sym = sum(es.n_symbols for es, _ in items)
+    total_units = sum(es.n_units for es, _ in items)
+    log("batch: %d streams, %.2f GB symbols, %.2f GB compressed, generated in %.1fs" %
+        (len(items), total_sym / 1e9, 4.0 * total_units / 1e9, time.time() - t0))
+    b = DeviceBatch([es for es, _ in items])
+    b.decode()
+    torch.cuda.synchronize()
+    st = b.status()
+    bad = verify_batch(b, items, torch)
+    log("batched decode bit-exact: %s (status %d, mismatching streams %d)" % (not bad, st, len(bad)))
+    if bad or st:
+        raise SystemExit("batched decode mismatch: streams %s" % bad[:10])
+    # kernel breakdown of one batched decode
+    kt = _lib.KernelTimer()
+    b.decode(kernel_events=kt.ptr)
+    torch.cuda.synchronize()
+    kms = kt.times_ms()
+    kt.close()
+    for _ in range(args.warmup):
+        b.decode()
+    ms_batch = time_events(torch, b.decode, args.steps)
+    # per-stream plain launches over the same device buffers (shared workspace)
+    ws = torch.zeros(workspace_bytes(max(es.n_units for es, _ in items), 4096), dtype=torch.uint8,
+                     device="cuda")
+    plain = [b.plain_args(i, ws) for i in range(len(items))]
+    L = _lib.lib()
+
+    def run_plain():
+        for a in plain:
+            _lib.check(L.mans_decode(C.byref(a)))
+
+    b.out.zero_()
+    run_plain()
+    torch.cuda.synchronize()
+    bad = verify_batch(b, items, torch)
+    log("per-stream plain launches bit-exact: %s" % (not bad))
+    if bad:
+        raise SystemExit("plain decode mismatch")
+    ms_plain = time_events(torch, run_plain, 1)
+    res = {"ms_batch": ms_batch, "ms_plain": ms_plain, "kernels_ms": kms,
+           "total_symbols": total_sym, "total_units": total_units, "n_streams": len(items)}
+    del b, ws
+    torch.cuda.empty_cache()
+    return res, items
 
 
-def bench_plain(args, torch):
+def bench_single(args, torch):
+    """Single-stream at-scale breakdown: 2^log2n symbols per lambda, L = 4096."""
+    from multians_amd import _lib
     from multians_amd.decoder import DeviceStream
     lams = [0.01, 0.05, 0.2, 1.0]
     n = 1 << args.log2n
-    L = 4096
-    t0 = time.time()
-    streams = make_streams(lams, n, L)
-    log("generated+encoded %d streams of %d symbols in %.1fs" % (len(lams), n, time.time() - t0))
-    devs = []
-    for (es, sym), lam in zip(streams, lams):
+    items = make_streams([(1000 + i, lam, n, 4096) for i, lam in enumerate(lams)])
+    per = []
+    for (es, sym), lam in zip(items, lams):
         ds = DeviceStream(es)
         out = ds.decode()
         ref = torch.from_numpy(sym).cuda()
-        ok = bool(torch.equal(out[:n].flip(0), ref))
-        st = ds.status()
+        ok = bool(torch.equal(out[:n].flip(0), ref)) and ds.status() == 0
         del ref
-        log("lambda=%.3g units=%d ratio=%.2f bit-exact=%s status=%d" %
-            (lam, es.n_units, n / (4.0 * es.n_units), ok, st))
-        if not ok or st != 0:
-            raise SystemExit("decode mismatch for lambda=%g" % lam)
-        devs.append(ds)
-    torch.cuda.synchronize()
-    # per-lambda times (median of 5) and kernel breakdown
-    per = []
-    for ds, lam in zip(devs, lams):
-        ts = []
-        for _ in range(5):
-            a = torch.cuda.Event(enable_timing=True)
-            b = torch.cuda.Event(enable_timing=True)
-            a.record()
-            ds.decode()
-            b.record()
-            torch.cuda.synchronize()
-            ts.append(a.elapsed_time(b))
-        kt = kernel_times(ds, torch)
-        ms = float(np.median(ts))
-        algo = 4.0 * ds.es.n_units + n
+        if not ok:
+            raise SystemExit("single-stream decode mismatch for lambda=%g" % lam)
+        ms = time_events(torch, ds.decode, 5)
+        kt = _lib.KernelTimer()
+        ds.decode(kernel_events=kt.ptr)
+        torch.cuda.synchronize()
+        kms = kt.times_ms()
+        kt.close()
+        algo = 4.0 * es.n_units + n
         per.append({"lambda": lam, "ms": round(ms, 4), "decoded_GBps": round(n / ms / 1e6, 2),
-                    "ratio": round(n / (4.0 * ds.es.n_units), 3),
-                    "hbm_frac": round(algo / ms / 1e6 / HBM_PEAK_GBS, 4),
-                    "kernels_ms": {k: round(float(v), 4) for k, v in
-                                   zip(_lib_names(), kt)}})
-        log(json.dumps(per[-1]))
-    for _ in range(args.warmup):
-        for ds in devs:
-            ds.decode()
-    torch.cuda.synchronize()
-    a = torch.cuda.Event(enable_timing=True)
-    b = torch.cuda.Event(enable_timing=True)
-    a.record()
-    for _ in range(args.steps):
-        for ds in devs:
-            ds.decode()
-    b.record()
-    torch.cuda.synchronize()
-    ms_step = a.elapsed_time(b) / args.steps
-    total = n * len(devs)
-    value = total / ms_step / 1e6
-    # dominant kernel over the four streams: summed kernel time
-    ktot = {}
-    for p in per:
-        for k, v in p["kernels_ms"].items():
-            ktot[k] = ktot.get(k, 0.0) + v
-    dom = max(ktot, key=ktot.get)
-    # algorithmic bytes per launch: k_sync reads the compressed units (4 B/unit); the
-    # write kernel reads them and writes the decoded bytes; the others are small.
-    units_all = sum(ds.es.n_units for ds in devs)
-    algo = {"sync": 4.0 * units_all, "write": 4.0 * units_all + total}.get(dom, 4.0 * units_all)
-    ach = algo / (ktot[dom] / 1e3) / 1e9
-    roof = {"bound": "hbm", "kernel": "k_" + dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-            "whole_decode_frac": round((4.0 * units_all + total) / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
-    res = {"metric": METRIC, "value": round(value, 3), "unit": "GB/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
-           "data": "synthetic (seeded Exp(lambda) symbols, encoded on the host)",
-           "config": {"workload": "plain single stream at scale: 4 streams x 2^%d uint8 symbols, "
-                                  "lambda in {0.01,0.05,0.2,1.0}, L=4096" % args.log2n,
-                      "streams": len(devs), "symbols_per_stream": n, "L": L},
-           "per_lambda": per, "roofline": roof}
-    return res, streams
+                    "ratio": round(n / (4.0 * es.n_units), 3),
+                    "hbm_frac": round(algo / ms / 1e6 / HBM_PEAK_GBS, 4), "bit_exact": True,
+                    "kernels_ms": {k: round(v, 4) for k, v in kms.items()}})
+        log("single", json.dumps(per[-1]))
+        del ds
+        torch.cuda.empty_cache()
+    return per, items
+
+
+def cpu_baseline(items, budget_s=20.0):
+    """Reference MulticoreDecoder on a bounded sample (streams of the benchmark)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    use_ref = ol.ref_available()
+    total_bytes, total_s, nstreams, dropped_tail = 0, 0.0, 0, 0
+    for es, sym in items:
+        # the reference's overflow rounds never re-check a thread marked synchronised
+        # when its predecessor's exit changes later (src/multicore_decoder.cc:57-75,
+        # 109-111), so it is only correct when every thread's range is longer than the
+        # self-synchronisation distance: sample streams with >= 8192 units per thread
+        if es.n_units < 8192 * THREADS:
+            continue
+        units = np.concatenate([es.units, np.zeros(16, np.uint32)])
+        out = np.zeros(es.n_symbols, np.uint8)
+        t0 = time.perf_counter()
+        if use_ref:
+            us = ol.ref().ref_multicore(4, THREADS, units, es.n_units, es.table, es.L,
+                                        es.first_state, es.first_bit, out, es.n_symbols)
+            dt = us / 1e6
+        else:
+            ol.oracle().mo_decode_multicore(4, THREADS, units, es.n_units, es.table, es.L,
+                                            es.first_state, es.first_bit, out, es.n_symbols)
+            dt = time.perf_counter() - t0
+        if not np.array_equal(out[::-1], sym):
+            # the reference never decodes zero-bit symbols that sit after the last unit
+            # when n_units % S == 0 (the first-encoded symbols, decoded last); any other
+            # difference is an error
+            diff = np.nonzero(out[::-1] != sym)[0]
+            if diff.max() >= 64:
+                raise SystemExit("cpu baseline decode mismatch")
+            dropped_tail += 1
+        total_bytes += es.n_symbols
+        total_s += dt
+        nstreams += 1
+        if total_s > budget_s:
+            break
+    return {"value": round(total_bytes / total_s / 1e9, 4), "unit": "GB/s", "cores": THREADS,
+            "kind": "reference" if use_ref else "port",
+            "sample": "MulticoreDecoder with S=4 (the reference demo's) and %d threads on the "
+                      "first %d streams with >= 8192 units per thread (%.2f GB of "
+                      "symbols) of the benchmark batch, one after another" % (THREADS, nstreams, total_bytes / 1e9),
+            "host_logical_cpus": os.cpu_count(),
+            "streams_with_dropped_zero_bit_tail": dropped_tail}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="plain", choices=["plain"])
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--streams", type=int, default=10000)
     ap.add_argument("--log2n", type=int, default=30)
+    ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(streams shard across GPUs without exchange; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU; this build has no multi-GPU path (streams would "
+              "shard across GPUs without exchange)", file=sys.stderr)
         raise SystemExit(2)
     import torch
     torch.cuda.set_device(0)
-    res, streams = bench_plain(args, torch)
+    bres, bitems = bench_batched(args, torch)
+    ms = bres["ms_batch"]
+    value = bres["total_symbols"] / ms / 1e6
+    plain_gbps = bres["total_symbols"] / bres["ms_plain"] / 1e6
+    kms = bres["kernels_ms"]
+    dom = max(kms, key=kms.get)
+    # algorithmic bytes per launch: k_sync reads the compressed units once (4 B/unit);
+    # k_write reads them and writes every decoded byte; other kernels are small.
+    units4 = 4.0 * bres["total_units"]
+    algo = {"sync": units4, "write": units4 + bres["total_symbols"]}.get(dom, units4)
+    ach = algo / (kms[dom] / 1e3) / 1e9
+    res = {"metric": METRIC, "value": round(value, 3), "unit": "GB/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+           "data": "synthetic (seeded Exp(lambda) symbols, tables and encoding on the host)",
+           "config": {"workload": "batched: %d tANS streams, uint8 sizes log-uniform 4 KiB-16 MiB, "
+                                  "lambda ~ LogU(0.005, 2.0), own 4096-state table each, one "
+                                  "batched launch sequence" % bres["n_streams"],
+                      "streams": bres["n_streams"], "total_symbols": bres["total_symbols"],
+                      "compressed_bytes": int(units4), "L": 4096},
+           "batched_GBps": round(value, 3),
+           "per_stream_plain_GBps": round(plain_gbps, 3),
+           "per_stream_plain_ms": round(bres["ms_plain"], 3),
+           "batched_vs_plain": round(value / plain_gbps, 2),
+           "kernels_ms": {k: round(v, 4) for k, v in kms.items()},
+           "roofline": {"bound": "hbm", "kernel": "k_" + dom, "achieved": round(ach, 2),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                        "traffic": None,
+                        "whole_decode_frac": round((units4 + bres["total_symbols"]) / ms / 1e6
+                                                   / HBM_PEAK_GBS, 4)}}
+    log("batched", json.dumps({k: res[k] for k in ("batched_GBps", "per_stream_plain_GBps",
+                                                   "batched_vs_plain", "kernels_ms")}))
+    if not args.no_single:
+        per, _ = bench_single(args, torch)
+        res["single_stream"] = per
     if not args.no_cpu_baseline:
-        res["cpu_baseline"] = cpu_baseline(streams)
+        res["cpu_baseline"] = cpu_baseline(bitems)
         log("cpu baseline", json.dumps(res["cpu_baseline"]))
     print(json.dumps(res), flush=True)
 

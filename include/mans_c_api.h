@@ -145,6 +145,7 @@ typedef struct mans_batch_opts {
     void* stream;
     uint32_t flags;
     uint32_t reserved;
+    void** kernel_events;       /* optional, as in mans_decode_args */
 } mans_batch_opts;
 
 typedef struct mans_batch_stats {
@@ -174,6 +175,12 @@ uint64_t mans_max_encoded_units(uint64_t n);
 int mans_encode(const uint8_t* in, uint64_t n, const uint32_t* enc, uint32_t L,
                 uint32_t* units, uint64_t* n_units, uint32_t* first_bit,
                 uint32_t* first_state);
+/* One call: mans_gen_symbols (1 thread) -> histogram -> mans_normalize_counts ->
+ * mans_build_tables -> mans_encode.  symbols_out may be NULL; units_cap must be at
+ * least mans_max_encoded_units(n); dec_out receives the L-entry decoder table. */
+int mans_make_stream(uint64_t seed, double lambda, uint64_t n, uint32_t L, uint8_t* symbols_out,
+                     uint32_t* units_out, uint64_t units_cap, uint64_t* n_units,
+                     uint32_t* first_bit, uint32_t* first_state, uint32_t* dec_out);
 
 #ifdef __cplusplus
 }

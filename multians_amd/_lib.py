@@ -49,7 +49,8 @@ class StreamDesc(C.Structure):
 
 class BatchOpts(C.Structure):
     _fields_ = [("workspace", C.c_void_p), ("workspace_bytes", C.c_uint64),
-                ("stream", C.c_void_p), ("flags", C.c_uint32), ("reserved", C.c_uint32)]
+                ("stream", C.c_void_p), ("flags", C.c_uint32), ("reserved", C.c_uint32),
+                ("kernel_events", C.c_void_p)]
 
 
 class BatchStats(C.Structure):
@@ -68,6 +69,7 @@ EXPORTS = [
     "mans_workspace_bytes", "mans_workspace_bytes_geometry", "mans_decode", "mans_decode_status",
     "mans_batch_workspace_bytes", "mans_decode_batch", "mans_gen_symbols",
     "mans_normalize_counts", "mans_build_tables", "mans_max_encoded_units", "mans_encode",
+    "mans_make_stream",
 ]
 
 _lib = None
@@ -114,6 +116,8 @@ def lib():
         "mans_build_tables": ([vp, u32, u32, vp, vp], i32),
         "mans_max_encoded_units": ([u64], u64),
         "mans_encode": ([vp, u64, vp, u32, vp, C.POINTER(u64), C.POINTER(u32), C.POINTER(u32)], i32),
+        "mans_make_stream": ([u64, C.c_double, u64, u32, vp, vp, u64, C.POINTER(u64),
+                              C.POINTER(u32), C.POINTER(u32), vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(l, name)
@@ -165,3 +169,42 @@ def encode(symbols, enc, L):
     check(lib().mans_encode(_ptr(symbols), n, _ptr(enc), L, _ptr(units), C.byref(nu), C.byref(fb),
                             C.byref(fs)))
     return units[:nu.value].copy(), fb.value, fs.value
+
+
+def make_stream(seed, lam, n, L, keep_symbols=True):
+    """Generate + encode one stream in one native call; returns (units, fb, fs, table, symbols)."""
+    sym = np.empty(n, np.uint8) if keep_symbols else None
+    units = np.empty(lib().mans_max_encoded_units(n), np.uint32)
+    dec = np.empty(L, np.uint32)
+    nu, fb, fs = C.c_uint64(), C.c_uint32(), C.c_uint32()
+    check(lib().mans_make_stream(seed, lam, n, L, _ptr(sym) if keep_symbols else None, _ptr(units),
+                                 len(units), C.byref(nu), C.byref(fb), C.byref(fs), _ptr(dec)))
+    return units[:nu.value], fb.value, fs.value, dec, sym
+
+
+class KernelTimer:
+    """HIP events around the decode kernels (kernel_events argument)."""
+
+    def __init__(self):
+        L = lib()
+        self.evs = [C.c_void_p() for _ in range(N_KERNEL_EVENTS)]
+        for e in self.evs:
+            check(L.mans_event_create(C.byref(e)))
+        self.arr = (C.c_void_p * N_KERNEL_EVENTS)(*[e.value for e in self.evs])
+
+    @property
+    def ptr(self):
+        return C.cast(self.arr, C.c_void_p)
+
+    def times_ms(self):
+        L = lib()
+        out = []
+        for k in range(N_KERNEL_EVENTS - 1):
+            ms = C.c_float()
+            check(L.mans_event_elapsed_ms(self.evs[k], self.evs[k + 1], C.byref(ms)))
+            out.append(ms.value)
+        return dict(zip(KERNEL_NAMES, out))
+
+    def close(self):
+        for e in self.evs:
+            lib().mans_event_destroy(e)

@@ -25,14 +25,29 @@ inline uint64_t mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
-void gen_range(uint64_t seed, const uint64_t* thr, uint8_t* out, uint64_t b, uint64_t e) {
+// s(u) = #{k in 1..255 : u < thr[k]} for u uniform on [0, 2^53); s is non-increasing
+// in u.  Bucketing u by its top 12 bits gives, per bucket, the range [s(end), s(start)];
+// most buckets hold one symbol, the rest are finished by a short binary search.
+struct SymTable {
+    uint64_t thr[256];
+    uint8_t hi[4096], lo[4096];
+};
+
+inline uint32_t sym_of(const uint64_t* thr, uint64_t u, uint32_t lo, uint32_t hi) {
+    // largest k in [lo, hi] with u < thr[k] (u < thr[lo] holds)
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (u < thr[mid]) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+void gen_range(uint64_t seed, const SymTable* t, uint8_t* out, uint64_t b, uint64_t e) {
     for (uint64_t i = b; i < e; ++i) {
         const uint64_t u = mix64(seed + (i + 1) * 0x9E3779B97F4A7C15ull) >> 11;
-        // thr[] is decreasing; count thresholds above u (thr[0] = 2^53 > u always)
-        uint32_t s = 0;
-        for (uint32_t step = 128; step; step >>= 1)
-            if (s + step <= 255u && u < thr[s + step]) s += step;
-        out[i] = (uint8_t)s;
+        const uint32_t k = (uint32_t)(u >> 41);
+        const uint32_t h = t->hi[k], l = t->lo[k];
+        out[i] = (uint8_t)(h == l ? h : sym_of(t->thr, u, l, h));
     }
 }
 
@@ -49,17 +64,21 @@ extern "C" {
 int mans_gen_symbols(uint64_t seed, double lambda, uint8_t* out, uint64_t n, int threads) {
     if (!out && n) return set_error(MANS_EINVAL, "out is NULL");
     if (!(lambda > 0.0)) return set_error(MANS_EINVAL, "lambda must be > 0");
-    uint64_t thr[256];
-    for (int k = 0; k < 256; ++k) thr[k] = (uint64_t)std::floor(std::ldexp(std::exp(-lambda * k), 53));
+    SymTable t;
+    for (int k = 0; k < 256; ++k) t.thr[k] = (uint64_t)std::floor(std::ldexp(std::exp(-lambda * k), 53));
+    for (uint32_t k = 0; k < 4096; ++k) {
+        t.hi[k] = (uint8_t)sym_of(t.thr, (uint64_t)k << 41, 0, 255);
+        t.lo[k] = (uint8_t)sym_of(t.thr, (((uint64_t)k + 1) << 41) - 1, 0, 255);
+    }
     if (threads <= 1 || n < (1u << 20)) {
-        gen_range(seed, thr, out, 0, n);
+        gen_range(seed, &t, out, 0, n);
         return MANS_OK;
     }
     std::vector<std::thread> th;
     uint64_t per = (n + threads - 1) / threads;
-    for (int t = 0; t < threads; ++t) {
-        uint64_t b = std::min<uint64_t>(n, per * t), e = std::min<uint64_t>(n, b + per);
-        th.emplace_back(gen_range, seed, thr, out, b, e);
+    for (int i = 0; i < threads; ++i) {
+        uint64_t b = std::min<uint64_t>(n, per * i), e = std::min<uint64_t>(n, b + per);
+        th.emplace_back(gen_range, seed, &t, out, b, e);
     }
     for (auto& x : th) x.join();
     return MANS_OK;
@@ -175,3 +194,26 @@ int mans_encode(const uint8_t* in, uint64_t n, const uint32_t* enc, uint32_t L, 
 }
 
 }  // extern "C"
+
+extern "C" int mans_make_stream(uint64_t seed, double lambda, uint64_t n, uint32_t L,
+                                uint8_t* symbols_out, uint32_t* units_out, uint64_t units_cap,
+                                uint64_t* n_units, uint32_t* first_bit, uint32_t* first_state,
+                                uint32_t* dec_out) {
+    if (!units_out || !n_units || !first_bit || !first_state || !dec_out)
+        return set_error(MANS_EINVAL, "NULL argument");
+    if (units_cap < mans_max_encoded_units(n)) return set_error(MANS_EINVAL, "units_cap too small");
+    std::vector<uint8_t> tmp;
+    uint8_t* sym = symbols_out;
+    if (!sym) { tmp.resize(n); sym = tmp.data(); }
+    int rc = mans_gen_symbols(seed, lambda, sym, n, 1);
+    if (rc) return rc;
+    uint64_t hist[256] = {0};
+    for (uint64_t i = 0; i < n; ++i) ++hist[sym[i]];
+    uint32_t counts[256];
+    rc = mans_normalize_counts(hist, 256, L, counts);
+    if (rc) return rc;
+    std::vector<uint32_t> enc(3ull * 256 * L);
+    rc = mans_build_tables(counts, 256, L, enc.data(), dec_out);
+    if (rc) return rc;
+    return mans_encode(sym, n, enc.data(), L, units_out, n_units, first_bit, first_state);
+}

@@ -644,19 +644,25 @@ void choose_geometry(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t req_
                      uint32_t req_K, uint32_t* S, uint32_t* K) {
     // checkpoints sit at round boundaries of the sync kernels: K is the round length
     (void)req_K;
-    uint32_t k = kR;
+    const uint32_t k = kR;
     uint32_t s = req_S;
     if (!s) {
-        // Sync chunks must be long against the distance over which a guessed-state
-        // decode merges with the true path, or walks fail and need fixup rounds.
-        // Measured at L = 4096 (units, 99.9th percentile): 851 at 4.3 symbols/unit,
-        // 1832 at 5.5, ~4000-4600 at 8.5-21, ~600 at 100+.
-        double spu = n_units ? (double)n_out / (double)n_units : 4.0;
-        if (spu <= 4.6) s = 2048u;
-        else if (spu <= 6.5) s = 4096u;
-        else if (spu <= 40.0) s = 8192u;
-        else s = 2048u;
-        if (L <= 1024u) s >>= 1;
+        // s_safe: long against the distance over which a guessed-state decode merges
+        // with the true path, so that walks rarely fail.  Measured at L = 4096 (units,
+        // 99.9th percentile): 851 at 4.3 symbols/unit, 1832 at 5.5, ~4000-4600 at
+        // 8.5-21, ~600 at 100+.
+        const double spu = n_units ? (double)n_out / (double)n_units : 4.0;
+        uint32_t s_safe = spu <= 4.6 ? 2048u : (spu <= 6.5 ? 4096u : (spu <= 40.0 ? 8192u : 2048u));
+        if (L <= 1024u) s_safe >>= 1;
+        // s_fill: enough chunks to fill one 256-lane workgroup; a short stream trades
+        // failed walks (fixup rounds) for lanes, which lowers its latency
+        uint64_t fill = (n_units + 255u) / 256u;
+        uint32_t s_fill = 1u;
+        while (s_fill < fill && s_fill < (1u << 30)) s_fill <<= 1;
+        // s_floor: below about the mean merge distance, failed walks chain through
+        // fixup rounds and cost more than the lanes they free
+        uint32_t s_floor = spu <= 4.6 ? 128u : (spu <= 6.5 ? 256u : 512u);
+        s = std::max<uint32_t>(s_floor, std::min(s_safe, s_fill));
     }
     s = 1u << ilog2(s < k ? k : s);
     *S = s;
@@ -741,7 +747,7 @@ static int launch_pipeline(const KArgs& a, uint32_t n_chunk_tasks, uint32_t n_it
     mark(2);
     hipLaunchKernelGGL(k_seam, dim3(n_chunk_tasks), dim3(kWG), lds_s, st, a);
     mark(3);
-    hipLaunchKernelGGL(k_fixup, dim3(std::min<uint32_t>(a.n_streams, 512u)), dim3(kWG), lds_s, st, a);
+    hipLaunchKernelGGL(k_fixup, dim3(a.n_streams), dim3(kWG), lds_s, st, a);
     mark(4);
     {
         uint32_t nb = (total_chunks + kScanBlock - 1) / kScanBlock;
@@ -964,7 +970,7 @@ extern "C" int mans_decode_batch(const mans_stream_desc* descs, size_t n,
     a.n_streams = (uint32_t)n;
     a.n_tasks = ct;
     a.w = make_work(opts->workspace, l);
-    rc = launch_pipeline(a, ct, it, (uint32_t)chunks, ml, mk, st);
+    rc = launch_pipeline(a, ct, it, (uint32_t)chunks, ml, mk, st, opts->kernel_events);
     if (rc) return rc;
     if (stats) {
         stats->total_units = units;

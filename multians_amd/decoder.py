@@ -144,10 +144,11 @@ class DeviceBatch:
         _lib.check(_lib.lib().mans_batch_workspace_bytes(self.descs, n, C.byref(b)))
         self.ws = torch.zeros(b.value, dtype=torch.uint8, device=device)
 
-    def decode(self, stream=None):
+    def decode(self, stream=None, kernel_events=None):
         torch = _torch()
         st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         o = _lib.BatchOpts()
+        o.kernel_events = kernel_events
         o.workspace = self.ws.data_ptr()
         o.workspace_bytes = self.ws.numel()
         o.stream = st
@@ -155,6 +156,18 @@ class DeviceBatch:
         _lib.check(_lib.lib().mans_decode_batch(self.descs, len(self.streams), C.byref(o),
                                                 C.byref(stats)))
         return stats
+
+    def plain_args(self, i, ws, stream=None):
+        """mans_decode arguments for stream i alone (per-stream plain launch), using the
+        batch's device buffers and the caller's workspace tensor."""
+        torch = _torch()
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        d = self.descs[i]
+        a = _lib.DecodeArgs()
+        a.units, a.n_units, a.out, a.n_out = d.units, d.n_units, d.out, d.n_out
+        a.table, a.L, a.first_state, a.first_bit = d.table, d.L, d.first_state, d.first_bit
+        a.workspace, a.workspace_bytes, a.stream = ws.data_ptr(), ws.numel(), st
+        return a
 
     def output(self, i):
         return self.out[int(self.ooff[i]):int(self.ooff[i + 1])]
