@@ -16,6 +16,7 @@ reference sources into oracle/_ref, else the oracle's restatement) on a bounded 
 on this host.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--streams N] [--no-single]
+                    [--no-plain] [--no-cpu-baseline] [--calibrate-copy MIB]
 """
 import argparse
 import ctypes as C
@@ -113,6 +114,12 @@ def bench_batched(args, torch):
     for _ in range(args.warmup):
         b.decode()
     ms_batch = time_events(torch, b.decode, args.steps)
+    res = {"ms_batch": ms_batch, "ms_plain": None, "kernels_ms": kms,
+           "total_symbols": total_sym, "total_units": total_units, "n_streams": len(items)}
+    if args.no_plain:
+        del b
+        torch.cuda.empty_cache()
+        return res, items
     # per-stream plain launches over the same device buffers (shared workspace)
     ws = torch.zeros(workspace_bytes(max(es.n_units for es, _ in items), 4096), dtype=torch.uint8,
                      device="cuda")
@@ -130,9 +137,7 @@ def bench_batched(args, torch):
     log("per-stream plain launches bit-exact: %s" % (not bad))
     if bad:
         raise SystemExit("plain decode mismatch")
-    ms_plain = time_events(torch, run_plain, 1)
-    res = {"ms_batch": ms_batch, "ms_plain": ms_plain, "kernels_ms": kms,
-           "total_symbols": total_sym, "total_units": total_units, "n_streams": len(items)}
+    res["ms_plain"] = time_events(torch, run_plain, 1)
     del b, ws
     torch.cuda.empty_cache()
     return res, items
@@ -217,6 +222,21 @@ def cpu_baseline(items, budget_s=20.0):
             "streams_with_dropped_zero_bit_tail": dropped_tail}
 
 
+def profiled_traffic(kernel, n_streams):
+    """HBM bytes per launch of `kernel` on the 10,000-stream batch, measured with
+    rocprofv3 FETCH_SIZE/WRITE_SIZE counter runs (scaled by a calibration kernel of
+    known traffic) and recorded in profiles/batch10k_kernels.md; None if absent."""
+    path = os.path.join(ROOT, "profiles", "batch10k_kernels.md")
+    if n_streams != 10000 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes "):
+            vals = dict(kv.split("=") for kv in line.split()[1:])
+            v = vals.get(kernel)
+            return float(v) if v and float(v) > 0 else None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,6 +246,11 @@ def main():
     ap.add_argument("--log2n", type=int, default=30)
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the per-stream plain-launch leg")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run one elementwise kernel that reads and writes MIB MiB (known "
+                         "HBM traffic, for scaling FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; this build has no multi-GPU path (streams would "
@@ -233,10 +258,17 @@ def main():
         raise SystemExit(2)
     import torch
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        src = torch.ones(args.calibrate_copy << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.add(src, 1, out=dst)       # elementwise kernel: reads and writes MIB MiB
+        torch.cuda.synchronize()
+        log("calibration copy: %d MiB read + %d MiB written" % (args.calibrate_copy, args.calibrate_copy))
+        del src, dst
     bres, bitems = bench_batched(args, torch)
     ms = bres["ms_batch"]
     value = bres["total_symbols"] / ms / 1e6
-    plain_gbps = bres["total_symbols"] / bres["ms_plain"] / 1e6
+    plain_gbps = bres["total_symbols"] / bres["ms_plain"] / 1e6 if bres["ms_plain"] else None
     kms = bres["kernels_ms"]
     dom = max(kms, key=kms.get)
     # algorithmic bytes per launch: k_sync reads the compressed units once (4 B/unit);
@@ -254,13 +286,14 @@ def main():
                       "streams": bres["n_streams"], "total_symbols": bres["total_symbols"],
                       "compressed_bytes": int(units4), "L": 4096},
            "batched_GBps": round(value, 3),
-           "per_stream_plain_GBps": round(plain_gbps, 3),
-           "per_stream_plain_ms": round(bres["ms_plain"], 3),
-           "batched_vs_plain": round(value / plain_gbps, 2),
+           "per_stream_plain_GBps": round(plain_gbps, 3) if plain_gbps else None,
+           "per_stream_plain_ms": round(bres["ms_plain"], 3) if plain_gbps else None,
+           "batched_vs_plain": round(value / plain_gbps, 2) if plain_gbps else None,
            "kernels_ms": {k: round(v, 4) for k, v in kms.items()},
            "roofline": {"bound": "hbm", "kernel": "k_" + dom, "achieved": round(ach, 2),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                        "traffic": None,
+                        "traffic": profiled_traffic("k_" + dom, bres["n_streams"]),
+                        "traffic_source": "profiles/batch10k_kernels.md (rocprofv3 --pmc)",
                         "whole_decode_frac": round((units4 + bres["total_symbols"]) / ms / 1e6
                                                    / HBM_PEAK_GBS, 4)}}
     log("batched", json.dumps({k: res[k] for k in ("batched_GBps", "per_stream_plain_GBps",

@@ -1,6 +1,6 @@
 // multians.h -- umbrella header (reference include/multians.h:11-30).  The GPU classes
-// are always available; the reference's CPU MulticoreDecoder is not part of this
-// library (it is the checker/CPU baseline under oracle/).
+// are always available.  The reference's CPU MulticoreDecoder is not part of this
+// library; with -DMULTI its header is taken from the reference's own include path.
 #ifndef MULTIANS_H_
 #define MULTIANS_H_
 
@@ -17,5 +17,9 @@
 #include "cuhd_input_buffer.h"
 #include "cuhd_output_buffer.h"
 #include "cuhd_util.h"
+
+#ifdef MULTI
+#include "multicore_decoder.h"
+#endif
 
 #endif

@@ -200,3 +200,36 @@ def test_golden_vectors():
         assert _units_equal(np.array(case["units"], np.uint32), st.units, st.first_bit)
         mc = ol.decode_multicore(st, S=case["S"], T=case["T"])
         assert mc.tobytes().hex() == case["multicore_hex"]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_normalize_invariants(seed):
+    """mo_normalize (the build's own normaliser; the reference's breaks on absent
+    symbols, src/ans_table_generator.cc:84-85, 274-283): counts sum to L, absent
+    symbols get 0, present ones >= 1, exact multiples are reproduced exactly,
+    scaling the histogram changes nothing, and every symbol but the one that takes
+    the rounding surplus is within one count of its share."""
+    rng = np.random.default_rng(seed)
+    L = [256, 1024, 4096][seed % 3]
+    hist = rng.integers(0, 1 << 20, 256).astype(np.uint64)
+    hist[rng.random(256) < 0.3] = 0
+    hist[rng.integers(0, 256, 3)] = rng.integers(1, 4, 3)     # rare symbols
+    c = ol.normalize(hist, L)
+    assert int(c.sum()) == L
+    assert np.array_equal(c == 0, hist == 0)
+    assert np.array_equal(ol.normalize(hist * np.uint64(3), L), c)
+    share = hist.astype(np.float64) * L / float(hist.sum())
+    big = int(np.argmax(c))
+    others = np.arange(256) != big
+    assert np.all(np.abs(c[others] - np.maximum(share[others], (hist[others] > 0))) < 1.0 + 1e-9)
+    # exact multiples: counts proportional to the histogram come back unchanged
+    exact = np.zeros(256, np.uint64)
+    exact[:16] = np.arange(1, 17, dtype=np.uint64)
+    c2 = ol.normalize(exact * np.uint64(7), 136)
+    assert np.array_equal(c2[:16], np.arange(1, 17)) and int(c2.sum()) == 136
+    # a single present symbol takes every state; too many present symbols is an error
+    one = np.zeros(256, np.uint64)
+    one[200] = 5
+    assert ol.normalize(one, L)[200] == L
+    with pytest.raises(ValueError):
+        ol.normalize(np.ones(256, np.uint64), 128)
