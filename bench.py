@@ -6,7 +6,8 @@ with lambda ~ LogUniform(0.005, 2.0) per stream, each with its own 4096-state ta
 normalised from its histogram -- decoded by one batched launch sequence
 (mans_decode_batch).  Beside it: the same streams decoded by per-stream plain launches
 (mans_decode once per stream, in order, one HIP stream), and the single-stream at-scale
-configuration (2^30 symbols for lambda in {0.01, 0.05, 0.2, 1.0}, 4096 states).
+configuration (2^30 symbols for lambda in {0.01, 0.05, 0.2, 1.0}, 4096 states), and the
+served configuration (continuous batching under Poisson arrivals; p50/p99 latency).
 
 Every decoded stream is checked bit-exact against its original symbols before timing.
 Inputs are resident in HBM; timing is HIP events on the decode stream.  A step is one
@@ -17,6 +18,7 @@ on this host.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--streams N] [--no-single]
                     [--no-plain] [--no-cpu-baseline] [--calibrate-copy MIB]
+                    [--serve-seconds S] [--serve-pool N]
 """
 import argparse
 import ctypes as C
@@ -176,6 +178,44 @@ def bench_single(args, torch):
     return per, items
 
 
+def bench_served(args, torch):
+    """Served configuration: open-loop Poisson arrivals at 70% of the measured batched
+    capacity on the request mix (L 1024/4096 50/50, uint8 sizes log-uniform 1 KiB-64 MiB,
+    lambda ~ LogU(0.005, 2.0)), admitted under a 64 GB HBM budget, continuous batching."""
+    from multians_amd import serve
+    t0 = time.time()
+    specs = serve.request_mix(args.serve_pool)
+    items = make_streams(specs)
+    pool = serve.StreamPool([es for es, _ in items], [sym for _, sym in items])
+    del items
+    log("serve pool: %d streams, %.2f GB symbols, built in %.1fs" %
+        (pool.n, pool.n_sym.sum() / 1e9, time.time() - t0))
+    srv = serve.Server(pool, 64e9, max_inflight=3)
+    # capacity: every pool stream at t = 0 (one batch), every result checked
+    cap = srv.run(np.zeros(pool.n), np.arange(pool.n), check_every=1)
+    if cap.mismatches:
+        raise SystemExit("served capacity run: %d mismatching streams" % cap.mismatches)
+    cap = srv.run(np.zeros(pool.n), np.arange(pool.n))
+    capacity = cap.symbols / cap.wall_s
+    rate = 0.7 * capacity / float(pool.n_sym.mean())
+    arr = serve.poisson_arrivals(rate, args.serve_seconds)
+    ids = np.random.default_rng(11).integers(0, pool.n, len(arr))
+    log("serve: capacity %.1f GB/s, offered %.1f GB/s = %.0f requests/s for %gs" %
+        (capacity / 1e9, 0.7 * capacity / 1e9, rate, args.serve_seconds))
+    st = srv.run(arr, ids, check_every=16)
+    if st.mismatches:
+        raise SystemExit("served run: %d mismatching requests" % st.mismatches)
+    out = st.summary()
+    out.update({"capacity_GBps": round(capacity / 1e9, 3),
+                "offered_GBps": round(0.7 * capacity / 1e9, 3),
+                "offered_requests_per_s": round(rate, 1), "duration_s": args.serve_seconds,
+                "pool_streams": pool.n, "budget_GB": 64, "max_inflight": 3,
+                "latency": "arrival to completion observed on the host; inputs resident"})
+    del srv, pool
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(items, budget_s=20.0):
     """Reference MulticoreDecoder on a bounded sample (streams of the benchmark)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -246,6 +286,10 @@ def main():
     ap.add_argument("--log2n", type=int, default=30)
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
+                    help="duration of the served run (0 = skip)")
+    ap.add_argument("--serve-pool", type=int, default=1000,
+                    help="distinct streams the served requests draw from")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the per-stream plain-launch leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -301,6 +345,12 @@ def main():
     if not args.no_single:
         per, _ = bench_single(args, torch)
         res["single_stream"] = per
+    if args.serve_seconds > 0:
+        sv = bench_served(args, torch)
+        res["served"] = sv
+        res["latency_p50_ms"] = sv["p50_ms"]
+        res["latency_p99_ms"] = sv["p99_ms"]
+        log("served", json.dumps(sv))
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(bitems)
         log("cpu baseline", json.dumps(res["cpu_baseline"]))

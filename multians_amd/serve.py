@@ -1,0 +1,336 @@
+"""Continuous-batching decode server over mans_decode_batch.
+
+Requests are decodes of device-resident tANS streams (each with its own table).  The
+loop admits arrived requests under an HBM budget, coalesces everything admitted since
+the last launch into the next batched launch, and keeps up to `max_inflight` batches
+on the device at once, each on its own HIP stream, so a batch that arrives while others
+run overlaps them.  A request's latency runs from its arrival to the moment its batch's
+completion event is observed on the host.
+
+Memory: compressed inputs and tables are resident (a stream pool); the budget covers
+what a request holds while it is admitted -- its output bytes and its share of the batch
+workspace -- carved from one device arena.
+"""
+import bisect
+import ctypes as C
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+_ALIGN = 256
+
+
+def _up(v, a=_ALIGN):
+    return (v + a - 1) // a * a
+
+
+class DeviceArena:
+    """First-fit allocator over one device byte tensor (offsets, 256-byte aligned)."""
+
+    def __init__(self, nbytes, device="cuda"):
+        import torch
+        self.size = _up(int(nbytes))
+        self.buf = torch.empty(self.size, dtype=torch.uint8, device=device)
+        self.base = self.buf.data_ptr()
+        self.free_off = [0]           # sorted offsets of free blocks
+        self.free_len = {0: self.size}
+        self.used = 0
+        self.peak = 0
+
+    def alloc(self, n):
+        n = _up(max(int(n), 1))
+        for i, off in enumerate(self.free_off):
+            ln = self.free_len[off]
+            if ln >= n:
+                del self.free_off[i]
+                del self.free_len[off]
+                if ln > n:
+                    self.free_off.insert(i, off + n)
+                    self.free_len[off + n] = ln - n
+                self.used += n
+                self.peak = max(self.peak, self.used)
+                return off, n
+        return None
+
+    def free(self, off, n):
+        self.used -= n
+        i = bisect.bisect_left(self.free_off, off)
+        # merge with the following and the preceding free block
+        if i < len(self.free_off) and off + n == self.free_off[i]:
+            nxt = self.free_off.pop(i)
+            n += self.free_len.pop(nxt)
+        if i > 0 and self.free_off[i - 1] + self.free_len[self.free_off[i - 1]] == off:
+            prev = self.free_off[i - 1]
+            self.free_len[prev] += n
+            return
+        self.free_off.insert(i, off)
+        self.free_len[off] = n
+
+    def largest_free(self):
+        return max(self.free_len.values()) if self.free_len else 0
+
+
+class StreamPool:
+    """Encoded streams resident in device memory: packed units (each stream 16-byte
+    aligned, padded to whole 16-byte blocks), packed tables, and optionally the
+    original symbols (decode order) for checking results."""
+
+    def __init__(self, streams, symbols=None, device="cuda"):
+        import torch
+        self.streams = list(streams)
+        n = len(self.streams)
+        self.n = n
+        uoff = np.zeros(n + 1, np.int64)
+        toff = np.zeros(n + 1, np.int64)
+        for i, es in enumerate(self.streams):
+            uoff[i + 1] = uoff[i] + ((es.n_units + 3) // 4 + 1) * 4
+            toff[i + 1] = toff[i] + es.L
+        units = np.zeros(uoff[-1], np.uint32)
+        tables = np.zeros(toff[-1], np.uint32)
+        for i, es in enumerate(self.streams):
+            units[uoff[i]:uoff[i] + es.n_units] = es.units
+            tables[toff[i]:toff[i] + es.L] = es.table
+        self.units = torch.from_numpy(units.view(np.int32)).to(device)
+        self.tables = torch.from_numpy(tables.view(np.int32)).to(device)
+        del units, tables
+        self.uptr = self.units.data_ptr() + 4 * uoff[:-1]
+        self.tptr = self.tables.data_ptr() + 4 * toff[:-1]
+        self.n_units = np.array([es.n_units for es in self.streams], np.int64)
+        self.n_sym = np.array([es.n_symbols for es in self.streams], np.int64)
+        self.L = np.array([es.L for es in self.streams], np.int64)
+        self.first_state = np.array([es.first_state for es in self.streams], np.int64)
+        self.first_bit = np.array([es.first_bit for es in self.streams], np.int64)
+        self.sym = None
+        if symbols is not None:
+            soff = np.zeros(n + 1, np.int64)
+            soff[1:] = np.cumsum(self.n_sym)
+            host = np.empty(int(soff[-1]), np.uint8)
+            for i, s in enumerate(symbols):
+                host[soff[i]:soff[i + 1]] = s[::-1]          # decode order
+            self.sym = torch.from_numpy(host).to(device)
+            self.soff = soff
+
+    def expected(self, i):
+        return self.sym[int(self.soff[i]):int(self.soff[i + 1])]
+
+
+@dataclass
+class Request:
+    rid: int
+    stream: int           # index into the pool
+    t_arrival: float      # seconds from the start of the run
+    t_admit: float = 0.0
+    t_done: float = 0.0
+    out_off: int = 0      # arena offset of the output
+
+
+@dataclass
+class _Batch:
+    reqs: list
+    block: tuple
+    event: object
+    hip_stream: object
+    descs: object
+    t_launch: float
+    check: object = None
+
+
+@dataclass
+class ServeStats:
+    latencies_ms: np.ndarray
+    symbols: int
+    wall_s: float
+    n_batches: int
+    batch_sizes: list = field(default_factory=list)
+    peak_bytes: int = 0
+    checked: int = 0
+    mismatches: int = -1
+
+    def summary(self):
+        lat = self.latencies_ms
+        return {
+            "requests": int(len(lat)),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3) if len(lat) else None,
+            "p99_ms": round(float(np.percentile(lat, 99)), 3) if len(lat) else None,
+            "max_ms": round(float(lat.max()), 3) if len(lat) else None,
+            "decoded_GBps": round(self.symbols / self.wall_s / 1e9, 3),
+            "batches": self.n_batches,
+            "mean_batch": round(float(np.mean(self.batch_sizes)), 1) if self.batch_sizes else 0,
+            "peak_hbm_GB": round(self.peak_bytes / 1e9, 3),
+            "checked": self.checked,
+            "mismatches": self.mismatches,
+        }
+
+
+class Server:
+    """Continuous-batching decode loop over a StreamPool."""
+
+    def __init__(self, pool, budget_bytes, max_inflight=3, max_batch=4096, device="cuda"):
+        import torch
+        self.torch = torch
+        self.pool = pool
+        self.arena = DeviceArena(budget_bytes, device)
+        self.max_inflight = max_inflight
+        self.max_batch = max_batch
+        self.streams = [torch.cuda.Stream(device=device) for _ in range(max_inflight)]
+        self.lib = _lib.lib()
+
+    def _batch_block(self, reqs):
+        """Bytes of one batch's block: outputs (256-aligned each) + workspace."""
+        p = self.pool
+        n = len(reqs)
+        descs = (_lib.StreamDesc * n)()
+        out_bytes = 0
+        for j, r in enumerate(reqs):
+            i = r.stream
+            d = descs[j]
+            d.units = int(p.uptr[i])
+            d.n_units = int(p.n_units[i])
+            d.out = 0
+            d.n_out = int(p.n_sym[i])
+            d.table = int(p.tptr[i])
+            d.L = int(p.L[i])
+            d.first_state = int(p.first_state[i])
+            d.first_bit = int(p.first_bit[i])
+            r.out_off = out_bytes
+            out_bytes += _up(d.n_out)
+        # workspace sizing only reads sizes and pointers, so out may be a placeholder
+        for j in range(n):
+            descs[j].out = 1 << 12
+        b = C.c_uint64()
+        _lib.check(self.lib.mans_batch_workspace_bytes(descs, n, C.byref(b)))
+        return descs, out_bytes, int(b.value)
+
+    def _launch(self, reqs, t_now, hs, check_every):
+        descs, out_bytes, ws_bytes = self._batch_block(reqs)
+        blk = self.arena.alloc(out_bytes + ws_bytes)
+        if blk is None:
+            return None
+        off, nb = blk
+        base = self.arena.base + off
+        for j, r in enumerate(reqs):
+            descs[j].out = base + r.out_off
+            r.out_off += off
+        o = _lib.BatchOpts()
+        o.workspace = base + out_bytes
+        o.workspace_bytes = ws_bytes
+        o.stream = hs.cuda_stream
+        _lib.check(self.lib.mans_decode_batch(descs, len(reqs), C.byref(o), None))
+        check = None
+        if check_every and self.pool.sym is not None:
+            torch = self.torch
+            with torch.cuda.stream(hs):
+                for r in reqs:
+                    if r.rid % check_every:
+                        continue
+                    n = int(self.pool.n_sym[r.stream])
+                    got = self.arena.buf[r.out_off:r.out_off + n]
+                    bad = (got != self.pool.expected(r.stream)).any().to(torch.int32)
+                    check = bad if check is None else check + bad
+        ev = self.torch.cuda.Event()
+        ev.record(hs)
+        return _Batch(reqs, blk, ev, hs, descs, t_now, check)
+
+    def run(self, arrivals, stream_ids, check_every=0, poll_sleep=20e-6):
+        """arrivals: sorted arrival times (s); stream_ids: pool index per request.
+        Runs until every request is done.  Returns ServeStats."""
+        torch = self.torch
+        reqs = [Request(k, int(s), float(t)) for k, (t, s) in enumerate(zip(arrivals, stream_ids))]
+        pending = deque()
+        inflight = []
+        free_streams = list(self.streams)
+        nxt = 0
+        done = []
+        sizes = []
+        n_batches = 0
+        checked = 0
+        bad_total = torch.zeros((), dtype=torch.int32, device=self.arena.buf.device)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        while nxt < len(reqs) or pending or inflight:
+            now = time.perf_counter() - t0
+            while nxt < len(reqs) and reqs[nxt].t_arrival <= now:
+                pending.append(reqs[nxt])
+                nxt += 1
+            # completions
+            still = []
+            for b in inflight:
+                if b.event.query():
+                    t = time.perf_counter() - t0
+                    for r in b.reqs:
+                        r.t_done = t
+                    done.extend(b.reqs)
+                    if b.check is not None:
+                        bad_total += b.check
+                        checked += sum(1 for r in b.reqs if r.rid % max(check_every, 1) == 0)
+                    self.arena.free(*b.block)
+                    free_streams.append(b.hip_stream)
+                else:
+                    still.append(b)
+            inflight = still
+            # launch everything admitted so far as one batch on a free stream
+            if pending and free_streams:
+                take = []
+                need = 0
+                room = self.arena.largest_free()
+                while pending and len(take) < self.max_batch:
+                    r = pending[0]
+                    n = int(self.pool.n_sym[r.stream])
+                    # outputs plus a workspace bound (about 1.5 B per unit at K = 32)
+                    est = _up(n) + 2 * int(self.pool.n_units[r.stream]) + 4096
+                    if take and need + est > room:
+                        break
+                    take.append(pending.popleft())
+                    need += est
+                if take:
+                    b = self._launch(take, now, free_streams[-1], check_every)
+                    if b is None and not inflight:
+                        raise RuntimeError("request of %d symbols does not fit the HBM budget"
+                                           % int(self.pool.n_sym[take[0].stream]))
+                    if b is None:
+                        for r in reversed(take):
+                            pending.appendleft(r)
+                    else:
+                        free_streams.pop()
+                        for r in take:
+                            r.t_admit = now
+                        inflight.append(b)
+                        sizes.append(len(take))
+                        n_batches += 1
+                        continue
+            if nxt < len(reqs) and not pending:
+                wait = reqs[nxt].t_arrival - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, poll_sleep) if inflight else wait)
+            else:
+                time.sleep(poll_sleep)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        lat = np.array([(r.t_done - r.t_arrival) * 1e3 for r in done])
+        syms = int(sum(int(self.pool.n_sym[r.stream]) for r in done))
+        st = ServeStats(lat, syms, wall, n_batches, sizes, self.arena.peak, checked,
+                        int(bad_total.item()) if check_every else -1)
+        return st
+
+
+def poisson_arrivals(rate_per_s, duration_s, seed=7):
+    """Open-loop arrival times of a Poisson process on [0, duration_s)."""
+    rng = np.random.default_rng(seed)
+    n = int(rate_per_s * duration_s * 1.2) + 16
+    t = np.cumsum(rng.exponential(1.0 / rate_per_s, n))
+    return t[t < duration_s]
+
+
+def request_mix(n, seed=20261204, min_bytes=1 << 10, max_bytes=64 << 20, lam_lo=0.005,
+                lam_hi=2.0, states=(1024, 4096)):
+    """Served-configuration request streams: (seed, lambda, n_symbols, L) per stream,
+    size log-uniform in [min_bytes, max_bytes], lambda log-uniform, L drawn 50/50."""
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(np.log(min_bytes), np.log(max_bytes), n))).astype(np.int64)
+    lams = np.exp(rng.uniform(np.log(lam_lo), np.log(lam_hi), n))
+    Ls = rng.choice(np.array(states), n)
+    return [(seed * 7919 + i, float(lams[i]), int(sizes[i]), int(Ls[i])) for i in range(n)]

@@ -1,0 +1,49 @@
+"""Serving loop on the GPU: requests arriving while others run are admitted, batched
+and decoded; every result equals the original symbols, which is also what each stream
+decodes to alone."""
+import numpy as np
+import pytest
+
+
+@pytest.mark.gpu
+def test_served_results_match_alone_and_mix():
+    import torch
+    from multians_amd import serve
+    from multians_amd.decoder import DeviceStream
+    import bench
+
+    specs = serve.request_mix(40, seed=5, max_bytes=1 << 20)
+    items = bench.make_streams(specs)
+    pool = serve.StreamPool([es for es, _ in items], [sym for _, sym in items])
+    srv = serve.Server(pool, 1 << 30, max_inflight=3)
+    arr = serve.poisson_arrivals(3000.0, 0.4, seed=3)
+    ids = np.random.default_rng(4).integers(0, pool.n, len(arr))
+    print("serve: %d requests over %d streams" % (len(arr), pool.n), flush=True)
+    st = srv.run(arr, ids, check_every=1)
+    s = st.summary()
+    print(s, flush=True)
+    assert s["requests"] == len(arr)
+    assert st.checked == len(arr) and st.mismatches == 0
+    assert s["batches"] >= 2                      # arrivals spread over several launches
+    assert srv.arena.used == 0
+    # the same streams decoded alone
+    for i in range(0, pool.n, 7):
+        es, sym = items[i]
+        got = DeviceStream(es).decode()[:es.n_symbols]
+        assert torch.equal(got, pool.expected(i))
+        assert np.array_equal(got.cpu().numpy()[::-1], sym)
+
+
+@pytest.mark.gpu
+def test_served_budget_smaller_than_backlog():
+    """A budget that holds only a few requests at once still finishes every request."""
+    from multians_amd import serve
+    import bench
+
+    specs = serve.request_mix(12, seed=9, min_bytes=200 << 10, max_bytes=1 << 20)
+    items = bench.make_streams(specs)
+    pool = serve.StreamPool([es for es, _ in items], [sym for _, sym in items])
+    srv = serve.Server(pool, 3 << 20, max_inflight=2)
+    st = srv.run(np.zeros(60), np.arange(60) % pool.n, check_every=1)
+    assert st.summary()["requests"] == 60 and st.mismatches == 0
+    assert srv.arena.peak <= 3 << 20 and st.n_batches >= 10
