@@ -149,6 +149,10 @@ struct RLane {
 // Decode the lane's steps of round r (those starting in units [rs, min(rs+kR, u_stop))).
 // Branch-free step: the unit window (lo, hi) and the next unit n1 shift on a crossing
 // by selects; n1 is re-read from LDS every step (off the dependency chain).
+// Software-pipelined: the table word of the next step is requested as soon as the next
+// state index exists, before the window bookkeeping of the current step, so the only
+// dependent chain per step is LDS read -> bfe -> add -> next LDS read.  The last
+// iteration requests one word it does not use (an in-range index).
 __device__ __forceinline__ void round_decode(const LdsTab& tab, const uint32_t* lbuf, RLane& L,
                                              uint32_t r) {
     if (!L.active) return;
@@ -159,11 +163,17 @@ __device__ __forceinline__ void round_decode(const LdsTab& tab, const uint32_t* 
     const uint32_t ue = re - rs;
     uint32_t o = L.o, xi = L.xi, cnt = L.cnt;
     uint32_t lo = lbuf[ui], hi = lbuf[ui + 1u], n1 = lbuf[ui + 2u];
+    uint32_t w = tab[xi];
+    // opaque copy: keeps the compiler from merging this load with the in-loop one into
+    // a single load at the loop head, which would undo the pipelining
+    asm volatile("" : "+v"(w));
     do {
-        const uint32_t w = tab[xi];
         const uint32_t win = funnel_shr(hi, lo, o);
+        const uint32_t nb = w & 31u;          // w is dead once the next read is issued
         xi = (w >> 16) + low_bits(win, w);
-        o += w & 31u;
+        w = tab[xi];
+        __builtin_amdgcn_sched_barrier(0);   // issue the read before the bookkeeping
+        o += nb;
         ++cnt;
         const bool c = o >= 32u;
         o &= 31u;
@@ -534,42 +544,55 @@ __device__ __forceinline__ void pass_load(const uint32_t* units, uint32_t n, uin
 }
 
 __device__ __forceinline__ void item_write(const LdsTab& tab, const uint32_t* lbuf, uint32_t entry,
-                                           uint8_t* dst, uint32_t cnt) {
+                                           uint8_t* dst_generic, uint32_t cnt) {
+    // global (not flat) stores: flat stores also count on the LDS counter and would
+    // force a full LDS wait every step
+    typedef uint8_t __attribute__((address_space(1)))* gptr_u8;
+    typedef uint32_t __attribute__((address_space(1)))* gptr_w;
+    gptr_u8 dst = (gptr_u8)dst_generic;
     uint32_t ui = entry_off(entry) >> 5;
     uint32_t o = entry_off(entry) & 31u;
     uint32_t lo = lbuf[ui], hi = lbuf[ui + 1u], n1 = lbuf[ui + 2u];
     uint32_t xi = entry_state(entry);
+    uint32_t wcur = tab[xi];
+    asm volatile("" : "+v"(wcur));     // as in round_decode
+    // software-pipelined like round_decode: the next table word is requested first
     auto next_sym = [&]() -> uint32_t {
-        const uint32_t w = tab[xi];
+        const uint32_t w = wcur;
         const uint32_t win = funnel_shr(hi, lo, o);
+        const uint32_t nb = w & 31u;
+        const uint32_t sym = w >> 8;
         xi = (w >> 16) + low_bits(win, w);
-        o += w & 31u;
+        wcur = tab[xi];
+        __builtin_amdgcn_sched_barrier(0);
+        o += nb;
         const bool c = o >= 32u;
         o &= 31u;
         lo = c ? hi : lo;
         hi = c ? n1 : hi;
         ui += c ? 1u : 0u;
         n1 = lbuf[min(ui + 2u, kRL - 1u)];
-        return w;
+        __builtin_amdgcn_sched_barrier(0);   // keep the unit read in this step
+        return sym;                           // symbol in bits 0-7
     };
     uint32_t i = 0;
     const uint32_t head = (uint32_t)((4u - ((uintptr_t)dst & 3u)) & 3u);
     const uint32_t h = head < cnt ? head : cnt;
-    for (; i < h; ++i) dst[i] = (uint8_t)(next_sym() >> 8);
-    uint32_t* dw = (uint32_t*)(dst + i);
+    for (; i < h; ++i) dst[i] = (uint8_t)next_sym();
+    gptr_w dw = (gptr_w)(dst + i);
     const uint32_t nd = (cnt - i) >> 2;
     for (uint32_t d = 0; d < nd; ++d) {
         uint32_t acc = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const uint32_t w = next_sym();
-            // acc = (acc >> 8) | (symbol << 24): byte 1 of w goes on top
-            acc = __builtin_amdgcn_perm(w, acc, 0x05030201u);
+            const uint32_t sym = next_sym();
+            // acc = (acc >> 8) | (symbol << 24): byte 0 of sym goes on top
+            acc = __builtin_amdgcn_perm(sym, acc, 0x04030201u);
         }
         dw[d] = acc;
     }
     i += nd << 2;
-    for (; i < cnt; ++i) dst[i] = (uint8_t)(next_sym() >> 8);
+    for (; i < cnt; ++i) dst[i] = (uint8_t)next_sym();
 }
 
 __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint32_t per_wg) {
