@@ -101,10 +101,14 @@ __global__ void k_init(Work w, uint32_t n_status, uint32_t n_streams) {
 // round r is decoded and written to LDS after it, so global-memory latency overlaps
 // a whole round of decoding.  A round is exactly one checkpoint interval (kR = K),
 // so checkpoints and merge tests happen between rounds, outside the step loop.
-constexpr uint32_t kR = 32;    // units decoded per round (= checkpoint stride)
-constexpr uint32_t kRL = 36;   // units staged per lane per round (kR + lookahead)
-constexpr uint32_t kLS = 36;   // LDS stride per lane in units (16-byte multiple)
-constexpr uint32_t kPieces = kRL / 4;   // 16-byte loads per lane segment (9)
+#ifndef MANS_ROUND_UNITS
+#define MANS_ROUND_UNITS 16
+#endif
+constexpr uint32_t kR = MANS_ROUND_UNITS;  // units decoded per round (= checkpoint stride)
+constexpr uint32_t kRL = kR + 4;   // units staged per lane per round (kR + lookahead)
+constexpr uint32_t kLS = kRL;      // LDS stride per lane in units (16-byte multiple)
+constexpr uint32_t kPieces = kRL / 4;   // 16-byte loads per lane segment
+static_assert(kR % 4 == 0 && kR >= 8, "round length");
 
 struct RoundIO {
     uint32_t* buf;        // LDS, kWG * kLS units
@@ -115,14 +119,21 @@ struct RoundIO {
 
 struct Prefetch { uint4 v[kPieces]; };
 
-// issue the loads of round r for all lanes: pieces 0-7 of lane t/8 + 32j by thread t
-// (eight threads per 128-byte lane segment), piece 8 of lane t by thread t
+// Piece j of thread t: the (lane, piece) pairs of all lane segments are numbered
+// lane-major, kPieces per lane, and thread t takes numbers t, t + kWG, ...; consecutive
+// threads read consecutive 16-byte pieces of a segment.
+__device__ __forceinline__ void piece_of(uint32_t j, uint32_t* lane, uint32_t* piece) {
+    const uint32_t idx = threadIdx.x + kWG * j;
+    *lane = idx / kPieces;
+    *piece = idx - *lane * kPieces;
+}
+
+// issue the loads of round r for all lanes
 __device__ __forceinline__ void round_load(const RoundIO& io, uint32_t r, Prefetch& pf) {
-    const uint32_t t = threadIdx.x;
 #pragma unroll
     for (uint32_t j = 0; j < kPieces; ++j) {
-        const uint32_t lane = j < 8u ? (t >> 3) + 32u * j : t;
-        const uint32_t piece = j < 8u ? (t & 7u) : 8u;
+        uint32_t lane, piece;
+        piece_of(j, &lane, &piece);
         const uint32_t rb = io.rbase[lane];
         const uint32_t b = rb + r * kR + 4u * piece;
         const U4 u = load_block(io.units, rb == kNone ? 0u : io.n, b);
@@ -131,11 +142,10 @@ __device__ __forceinline__ void round_load(const RoundIO& io, uint32_t r, Prefet
 }
 
 __device__ __forceinline__ void round_store(const RoundIO& io, const Prefetch& pf) {
-    const uint32_t t = threadIdx.x;
 #pragma unroll
     for (uint32_t j = 0; j < kPieces; ++j) {
-        const uint32_t lane = j < 8u ? (t >> 3) + 32u * j : t;
-        const uint32_t piece = j < 8u ? (t & 7u) : 8u;
+        uint32_t lane, piece;
+        piece_of(j, &lane, &piece);
         *reinterpret_cast<uint4*>(io.buf + lane * kLS + 4u * piece) = pf.v[j];
     }
 }
@@ -529,17 +539,36 @@ __global__ __launch_bounds__(kWG) void k_scan_down(KArgs a, uint32_t total, cons
 // next pass's units load while the current pass decodes, every lane decodes its
 // item's step count from the item's true entry, and symbols are packed four to a
 // dword store (byte stores for the unaligned head and tail).
-constexpr uint32_t kMaxItemUnits = kR;
+// A write item spans kWF checkpoint intervals (kWI units): items longer than the
+// sync kernels' rounds amortise the per-pass work (offsets, barrier) over more steps.
+constexpr uint32_t kWF = kR >= 32u ? 1u : 32u / kR;
+constexpr uint32_t kWI = kR * kWF;          // units per item
+constexpr uint32_t kWL = kWI + 4u;          // units staged per lane
+constexpr uint32_t kWPieces = kWL / 4u;
+
+__host__ __device__ __forceinline__ uint32_t n_write_items(uint32_t n_ck) {
+    return (n_ck + kWF - 1u) / kWF;
+}
+
+struct WPrefetch { uint4 v[kWPieces]; };
 
 __device__ __forceinline__ void pass_load(const uint32_t* units, uint32_t n, uint32_t u_first,
-                                          Prefetch& pf) {
-    const uint32_t t = threadIdx.x;
+                                          WPrefetch& pf) {
 #pragma unroll
-    for (uint32_t j = 0; j < kPieces; ++j) {
-        const uint32_t lane = j < 8u ? (t >> 3) + 32u * j : t;
-        const uint32_t piece = j < 8u ? (t & 7u) : 8u;
-        const U4 u = load_block(units, n, u_first + lane * kR + 4u * piece);
+    for (uint32_t j = 0; j < kWPieces; ++j) {
+        const uint32_t idx = threadIdx.x + kWG * j;
+        const uint32_t lane = idx / kWPieces, piece = idx - lane * kWPieces;
+        const U4 u = load_block(units, n, u_first + lane * kWI + 4u * piece);
         pf.v[j] = make_uint4(u.x, u.y, u.z, u.w);
+    }
+}
+
+__device__ __forceinline__ void pass_store(uint32_t* buf, const WPrefetch& pf) {
+#pragma unroll
+    for (uint32_t j = 0; j < kWPieces; ++j) {
+        const uint32_t idx = threadIdx.x + kWG * j;
+        const uint32_t lane = idx / kWPieces, piece = idx - lane * kWPieces;
+        *reinterpret_cast<uint4*>(buf + lane * kWL + 4u * piece) = pf.v[j];
     }
 }
 
@@ -571,7 +600,7 @@ __device__ __forceinline__ void item_write(const LdsTab& tab, const uint32_t* lb
         lo = c ? hi : lo;
         hi = c ? n1 : hi;
         ui += c ? 1u : 0u;
-        n1 = lbuf[min(ui + 2u, kRL - 1u)];
+        n1 = lbuf[min(ui + 2u, kWL - 1u)];
         __builtin_amdgcn_sched_barrier(0);   // keep the unit read in this step
         return sym;                           // symbol in bits 0-7
     };
@@ -606,10 +635,10 @@ __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint3
     // prefetch the first pass
     if (threadIdx.x == 0) s_stream = stream_of_task<true>(a, p0);
     __syncthreads();
-    Prefetch pf;
+    WPrefetch pf;
     {
         const StreamPlan sp = *plan_of(a, __builtin_amdgcn_readfirstlane(s_stream));
-        pass_load(sp.units, sp.n_units, ((p0 - sp.task_item) * kWG) << sp.log2K, pf);
+        pass_load(sp.units, sp.n_units, (p0 - sp.task_item) * kWG * kWI, pf);
     }
     for (uint32_t t = p0; t < p1; ++t) {
         __syncthreads();
@@ -622,31 +651,31 @@ __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint3
             (void)stage_table(sp, lds, a.w.status);    // ends with a barrier
             if (threadIdx.x == 0) s_have = sid;
         }
-        const uint32_t q0 = (t - sp.task_item) * kWG;
-        const uint32_t nitems = min((uint32_t)kWG, sp.n_ck - q0);
-        RoundIO io{buf, nullptr, sp.units, sp.n_units};
-        round_store(io, pf);
-        const uint32_t q = q0 + threadIdx.x;
+        // items i0 .. i0+nitems-1 of the stream; item i starts at checkpoint i*kWF
+        const uint32_t i0 = (t - sp.task_item) * kWG;
+        const uint32_t nitems = min((uint32_t)kWG, n_write_items(sp.n_ck) - i0);
+        pass_store(buf, pf);
         uint32_t entry = 0;
-        if (threadIdx.x < nitems) s_off[threadIdx.x] = checkpoint_offset(sp, a.w, q, &entry);
+        if (threadIdx.x < nitems)
+            s_off[threadIdx.x] = checkpoint_offset(sp, a.w, (i0 + threadIdx.x) * kWF, &entry);
         if (threadIdx.x == 0) {
             uint32_t e2;
-            s_off[nitems] = (q0 + nitems < sp.n_ck) ? checkpoint_offset(sp, a.w, q0 + nitems, &e2)
-                                                    : sp.n_out;
+            const uint32_t qe = (i0 + nitems) * kWF;
+            s_off[nitems] = qe < sp.n_ck ? checkpoint_offset(sp, a.w, qe, &e2) : sp.n_out;
         }
         __syncthreads();
         if (t + 1u < p1) {
             // the next pass may belong to the next stream
             const uint32_t ns = stream_of_task<true>(a, t + 1u);
             const StreamPlan np = *plan_of(a, __builtin_amdgcn_readfirstlane(ns));
-            pass_load(np.units, np.n_units, ((t + 1u - np.task_item) * kWG) << np.log2K, pf);
+            pass_load(np.units, np.n_units, (t + 1u - np.task_item) * kWG * kWI, pf);
         }
         if (threadIdx.x < nitems) {
             uint64_t off = s_off[threadIdx.x], end = s_off[threadIdx.x + 1];
             if (end > sp.n_out) end = sp.n_out;
             if (off < end) {
                 LdsTab tab{lds};
-                item_write(tab, buf + threadIdx.x * kLS, entry, sp.out + off, (uint32_t)(end - off));
+                item_write(tab, buf + threadIdx.x * kWL, entry, sp.out + off, (uint32_t)(end - off));
             }
         }
     }
@@ -786,7 +815,8 @@ static int launch_pipeline(const KArgs& a, uint32_t n_chunk_tasks, uint32_t n_it
         uint32_t per = (n_item_tasks + nwg - 1) / nwg;
         nwg = (n_item_tasks + per - 1) / per;
         (void)maxK;
-        hipLaunchKernelGGL(k_write, dim3(nwg), dim3(kWG), lds_s, st, a, n_item_tasks, per);
+        const size_t lds_w = lds + (size_t)kWG * kWL * 4u;
+        hipLaunchKernelGGL(k_write, dim3(nwg), dim3(kWG), lds_w, st, a, n_item_tasks, per);
     }
     mark(6);
     hipError_t e = hipGetLastError();
@@ -816,8 +846,8 @@ static int check_stream_args(const uint32_t* units, uint64_t n_units, const uint
 extern "C" int mans_workspace_bytes(uint64_t n_units, uint32_t L, uint64_t* bytes) {
     (void)L;
     if (!bytes) return set_error(MANS_EINVAL, "bytes is NULL");
-    // smallest geometry any tuning choice uses: K = 8, S = 256
-    uint64_t chunks = n_units / 256u + 2u, cks = n_units / 8u + 2u;
+    // smallest geometry the automatic choice uses: S = 128; K (the round length) >= 8
+    uint64_t chunks = n_units / 128u + 2u, cks = n_units / 8u + 2u;
     *bytes = make_layout(chunks, cks, 1).bytes;
     return MANS_OK;
 }
@@ -825,7 +855,10 @@ extern "C" int mans_workspace_bytes(uint64_t n_units, uint32_t L, uint64_t* byte
 extern "C" int mans_workspace_bytes_geometry(uint64_t n_units, uint32_t chunk_units,
                                              uint32_t checkpoint_units, uint64_t* bytes) {
     if (!bytes || !chunk_units || !checkpoint_units) return set_error(MANS_EINVAL, "bad geometry");
-    uint64_t chunks = n_units / chunk_units + 2u, cks = n_units / checkpoint_units + 2u;
+    // checkpoints always sit at round boundaries (kR units), whatever K is asked for
+    const uint32_t k = std::min<uint32_t>(checkpoint_units, kR);
+    const uint32_t c = std::max<uint32_t>(chunk_units, kR);
+    uint64_t chunks = n_units / c + 2u, cks = n_units / k + 2u;
     *bytes = make_layout(chunks, cks, 1).bytes;
     return MANS_OK;
 }
@@ -869,7 +902,7 @@ extern "C" int mans_decode(const mans_decode_args* d) {
     a.n_streams = 1;
     a.w = make_work(d->workspace, l);
     uint32_t nct = (sp.n_chunks + kWG - 1) / kWG;
-    uint32_t nit = (sp.n_ck + kWG - 1) / kWG;
+    uint32_t nit = (n_write_items(sp.n_ck) + kWG - 1) / kWG;
     a.n_tasks = nct;
     rc = launch_pipeline(a, nct, nit, sp.n_chunks, sp.L, sp.K, (hipStream_t)d->stream, d->kernel_events);
     if (rc) return rc;
@@ -937,7 +970,7 @@ static int build_plans(const mans_stream_desc* d, size_t n, std::vector<StreamPl
         cb += std::max<uint32_t>(sp.n_chunks, 1u);
         qb += std::max<uint32_t>(sp.n_ck, 1u);
         tc += std::max<uint64_t>((sp.n_chunks + kWG - 1) / kWG, 1);
-        ti += std::max<uint64_t>((sp.n_ck + kWG - 1) / kWG, 1);
+        ti += std::max<uint64_t>((n_write_items(sp.n_ck) + kWG - 1) / kWG, 1);
         tu += d[i].n_units;
         ts += d[i].n_out;
         ml = std::max(ml, sp.L);
