@@ -161,6 +161,34 @@ int mans_batch_workspace_bytes(const mans_stream_desc* descs, size_t n, uint64_t
 int mans_decode_batch(const mans_stream_desc* descs, size_t n, const mans_batch_opts* opts,
                       mans_batch_stats* stats);
 
+/* hipGraph form of mans_decode_batch (no reference counterpart: the reference launches
+ * its phases one by one from the host).  Plans the batch, uploads the plans into
+ * opts->workspace (synchronously), captures the decode's kernel sequence into a graph
+ * and instantiates it.  Each mans_graph_launch then decodes the same streams with no
+ * host-side planning: the device buffers named in descs and the workspace must stay
+ * allocated and unchanged in size; their contents (new units of the same lengths, new
+ * tables) may change between launches.  opts->kernel_events is ignored. */
+int mans_batch_graph_create(const mans_stream_desc* descs, size_t n, const mans_batch_opts* opts,
+                            void** graph);
+int mans_graph_launch(void* graph, void* stream);
+int mans_graph_destroy(void* graph);
+
+/* Chunk-length rules (the autotuner's output; no reference counterpart: the reference
+ * takes one subsequence size from its caller).  A stream uses the first rule with
+ * L <= L_max, symbols per unit (n_out / n_units) <= spu_max and n_units <= units_max;
+ * streams no rule matches use the built-in choice.  chunk_units is rounded down to a
+ * power of two and must be >= 128.  n == 0 restores the built-in choice.  The rules
+ * are process-wide; set them before decoding. */
+typedef struct mans_geometry_rule {
+    uint32_t L_max;
+    uint32_t chunk_units;
+    double spu_max;
+    uint64_t units_max;
+} mans_geometry_rule;
+int mans_set_geometry_rules(const mans_geometry_rule* rules, size_t n);
+/* The chunk length the decoder picks for a stream (rules, else built-in). */
+int mans_chunk_units(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t* chunk_units);
+
 /* --------------------------------------------------------- host-side codec */
 /* Symbols s = min(255, floor(X)), X ~ Exp(lambda); element i depends only on (seed, i). */
 int mans_gen_symbols(uint64_t seed, double lambda, uint8_t* out, uint64_t n, int threads);

@@ -53,6 +53,11 @@ class BatchOpts(C.Structure):
                 ("kernel_events", C.c_void_p)]
 
 
+class GeometryRule(C.Structure):
+    _fields_ = [("L_max", C.c_uint32), ("chunk_units", C.c_uint32), ("spu_max", C.c_double),
+                ("units_max", C.c_uint64)]
+
+
 class BatchStats(C.Structure):
     _fields_ = [("total_units", C.c_uint64), ("total_symbols", C.c_uint64),
                 ("n_streams", C.c_uint32), ("n_chunks", C.c_uint32),
@@ -69,7 +74,8 @@ EXPORTS = [
     "mans_workspace_bytes", "mans_workspace_bytes_geometry", "mans_decode", "mans_decode_status",
     "mans_batch_workspace_bytes", "mans_decode_batch", "mans_gen_symbols",
     "mans_normalize_counts", "mans_build_tables", "mans_max_encoded_units", "mans_encode",
-    "mans_make_stream",
+    "mans_make_stream", "mans_batch_graph_create", "mans_graph_launch", "mans_graph_destroy",
+    "mans_set_geometry_rules", "mans_chunk_units",
 ]
 
 _lib = None
@@ -118,6 +124,12 @@ def lib():
         "mans_encode": ([vp, u64, vp, u32, vp, C.POINTER(u64), C.POINTER(u32), C.POINTER(u32)], i32),
         "mans_make_stream": ([u64, C.c_double, u64, u32, vp, vp, u64, C.POINTER(u64),
                               C.POINTER(u32), C.POINTER(u32), vp], i32),
+        "mans_batch_graph_create": ([C.POINTER(StreamDesc), C.c_size_t, C.POINTER(BatchOpts),
+                                     C.POINTER(vp)], i32),
+        "mans_graph_launch": ([vp, vp], i32),
+        "mans_graph_destroy": ([vp], i32),
+        "mans_set_geometry_rules": ([C.POINTER(GeometryRule), C.c_size_t], i32),
+        "mans_chunk_units": ([u64, u64, u32, C.POINTER(u32)], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(l, name)
@@ -208,3 +220,19 @@ class KernelTimer:
     def close(self):
         for e in self.evs:
             lib().mans_event_destroy(e)
+
+
+def set_geometry_rules(rules):
+    """rules: iterable of (L_max, spu_max, units_max, chunk_units); [] = built-in choice."""
+    rules = list(rules)
+    arr = (GeometryRule * max(len(rules), 1))()
+    for i, (L_max, spu_max, units_max, chunk) in enumerate(rules):
+        arr[i].L_max, arr[i].spu_max = int(L_max), float(spu_max)
+        arr[i].units_max, arr[i].chunk_units = int(units_max), int(chunk)
+    check(lib().mans_set_geometry_rules(arr, len(rules)))
+
+
+def chunk_units(n_units, n_out, L):
+    v = C.c_uint32()
+    check(lib().mans_chunk_units(int(n_units), int(n_out), int(L), C.byref(v)))
+    return v.value

@@ -16,10 +16,14 @@
 //                 compares each checkpoint with the stored one; equality means the
 //                 two paths have merged (decoding is deterministic in (p, state)),
 //                 so everything stored from there on is on the walker's path.
-//                 Checkpoints before the merge are stored separately (wb*).
-//   fixup       : a walk that crosses its whole chunk without merging leaves the
-//                 next chunk's walk start unproven; that chunk is walked again
-//                 from the proven exit (rounds until no walk fails).
+//                 Checkpoints before the merge are stored separately (wb*).  A walk
+//                 that crosses its chunk without merging continues into the next
+//                 chunks (lane_walk_chain) when its own start is likely proven.
+//   check       : every chunk whose recorded walk start differs from its
+//                 predecessor's exit is listed.
+//   fixup       : listed chunks are walked again from their predecessor's exit, in
+//                 rounds, until every chunk is consistent; by induction from chunk 0
+//                 every exit is then on the true path.
 //   write pass  : every checkpoint is now known on the true path together with its
 //                 output offset (exclusive scan of per-chunk step counts), so item
 //                 q decodes exactly off(q+1)-off(q) steps from checkpoint q and
@@ -205,6 +209,10 @@ MANS_HD void bcursor_init(BCursor& c, const uint32_t* units, uint32_t n, uint32_
     c.hi = k0 == 3u ? c.B.x : pick4(c.A, k0 + 1u);
     c.k = k0 + 2u;
     c.nb = bb + 8u;
+    if (k0 >= 2u) {                           // A is already consumed: refill it now
+        c.A = load_block(units, n, c.nb);
+        c.nb += 4u;
+    }
 }
 
 MANS_HD void bcursor_advance(BCursor& c) {
@@ -368,6 +376,88 @@ MANS_HD uint32_t lane_walk(const StreamPlan& sp, const Work& w, const Tab& tab, 
     w.mq[gc] = kNone;
     w.wexit[gc] = pack_entry(xi, cur.o);
     w.wtotal[gc] = cnt;
+    return 0u;
+}
+
+// Walk from `entry` at the start of chunk c (c >= 1) through chunks c .. c_end-1 until it
+// merges with the stored spec path, giving every chunk it enters a full record (the
+// chunk-crossing form of lane_walk used by the sync kernels).  Returns 1 if merged.
+// With stop_after_first != 0 the walk ends at the end of chunk c even if it has not
+// merged (the sync kernel's rule for a walk whose predecessor's walk did not merge).
+template <typename Tab>
+MANS_HD uint32_t lane_walk_chain(const StreamPlan& sp, const Work& w, const Tab& tab, uint32_t c,
+                                 uint32_t c_end, uint32_t entry, uint32_t stop_after_first = 0u) {
+    if (stop_after_first && c_end > c + 1u) c_end = c + 1u;
+    const uint32_t u0 = c * sp.S;
+    uint32_t u_stop = c_end * sp.S;
+    if (u_stop > sp.n_units) u_stop = sp.n_units;
+    const uint64_t qb = sp.ck_base;
+    uint32_t gc = sp.chunk_base + c;
+    const uint32_t kmask = sp.K - 1u, smask = sp.S - 1u;
+    w.wentry[gc] = entry;
+    uint32_t q0 = u0 >> sp.log2K;
+    if (w.ck[qb + q0] == entry) {
+        w.mq[gc] = q0;
+        w.wcm[gc] = 0u;
+        return 1u;
+    }
+    w.wb[qb + q0] = entry;
+    w.wbc[qb + q0] = 0u;
+    BCursor cur;
+    uint32_t xi = entry_state(entry);
+    bcursor_init(cur, sp.units, sp.n_units, u0, entry_off(entry));
+    uint32_t cnt = 0u;
+    while (cur.u < u_stop) {
+        uint32_t crossed;
+        bstep(tab, cur, xi, crossed);
+        ++cnt;
+        if (crossed && (cur.u & kmask) == 0u && cur.u < u_stop) {
+            const uint32_t v = pack_entry(xi, cur.o);
+            if ((cur.u & smask) == 0u) {          // chunk boundary
+                w.mq[gc] = kNone;
+                w.wexit[gc] = v;
+                w.wtotal[gc] = cnt;
+                ++gc;
+                w.wentry[gc] = v;
+                cnt = 0u;
+            }
+            const uint32_t q = cur.u >> sp.log2K;
+            if (w.ck[qb + q] == v) {
+                w.mq[gc] = q;
+                w.wcm[gc] = cnt;
+                return 1u;
+            }
+            w.wb[qb + q] = v;
+            w.wbc[qb + q] = cnt;
+        }
+    }
+    w.mq[gc] = kNone;
+    w.wexit[gc] = pack_entry(xi, cur.o);
+    w.wtotal[gc] = cnt;
+    return 0u;
+}
+
+// 1 if a walk from `entry` at the start of chunk c merges with the stored spec path
+// inside chunk c (reads only).
+template <typename Tab>
+MANS_HD uint32_t lane_merges_in_chunk(const StreamPlan& sp, const Work& w, const Tab& tab,
+                                      uint32_t c, uint32_t entry) {
+    const uint32_t u0 = c * sp.S;
+    uint32_t u_stop = u0 + sp.S;
+    if (u_stop > sp.n_units) u_stop = sp.n_units;
+    const uint64_t qb = sp.ck_base;
+    const uint32_t kmask = sp.K - 1u;
+    if (w.ck[qb + (u0 >> sp.log2K)] == entry) return 1u;
+    BCursor cur;
+    uint32_t xi = entry_state(entry);
+    bcursor_init(cur, sp.units, sp.n_units, u0, entry_off(entry));
+    while (cur.u < u_stop) {
+        uint32_t crossed;
+        bstep(tab, cur, xi, crossed);
+        if (crossed && (cur.u & kmask) == 0u && cur.u < u_stop &&
+            w.ck[qb + (cur.u >> sp.log2K)] == pack_entry(xi, cur.o))
+            return 1u;
+    }
     return 0u;
 }
 

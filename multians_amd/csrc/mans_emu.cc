@@ -1,10 +1,9 @@
 // mans_emu.cc -- CPU emulation of the decode pipeline (TESTS ONLY, libmans_emu.so).
 //
 // Runs the per-lane bodies of mans_core.h sequentially, phase by phase, in the same
-// order the kernels of mans_kernels.hip run them, so the algorithm (spec, walk,
-// fixup rounds, scan, count-driven write) can be checked against the oracle without
-// a GPU.  Lanes of one kernel touch disjoint state, so a sequential loop over lanes
-// reproduces what the kernel computes.  The product never loads this library.
+// order the kernels of mans_kernels.hip run them, so the algorithm (spec, chained
+// walks, consistency check, fixup rounds, scan, count-driven write) can be checked
+// against the oracle without a GPU.  The product never loads this library.
 #include <cstring>
 #include <vector>
 
@@ -72,15 +71,33 @@ int mans_emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint6
     } else {
         for (uint32_t c = 0; c < nc; ++c) lane_spec(sp, w, ht, c);
     }
-    // k_walk
-    for (uint32_t c = 1; c < nc; ++c) {
-        uint32_t ok = lane_walk(sp, w, ht, c, exitv[c - 1]);
-        st_merges += ok;
-        ff[c] = 0;
+    // k_sync walks: in a range of R = 256 chunks, lane c walks from its exit into c+1 and
+    // on to the range end until it merges.  Lanes run in lockstep rounds on the GPU, so
+    // where walks overlap the one that started further back writes last; running the
+    // lanes from the back of the range to its front reproduces that.
+    // A walk leaves its first chunk only if the previous lane's walk merged inside its
+    // own chunk (the range's first walk always may).
+    const uint32_t R = 256u;
+    std::vector<uint8_t> own_merged(nc, 0);
+    for (uint32_t t = 1; t < nc; ++t) own_merged[t] = (uint8_t)lane_merges_in_chunk(sp, w, ht, t, exitv[t - 1]);
+    for (uint32_t c0 = 0; c0 < nc; c0 += R) {
+        const uint32_t c_end = c0 + R < nc ? c0 + R : nc;
+        for (uint32_t t = c_end; t-- > c0 + 1u;) {
+            const uint32_t stop = (t >= c0 + 2u && !own_merged[t - 1]) ? 1u : 0u;
+            st_merges += lane_walk_chain(sp, w, ht, t, c_end, exitv[t - 1], stop);
+        }
     }
-    // initial fixup targets: successors of failed walks whose exit changed
-    for (uint32_t c = 1; c + 1 < nc; ++c)
-        if (mq[c] == kNone && wexit[c] != exitv[c]) fl[status[1]++] = c + 1;
+    // k_seam: the walk into the first chunk of each later range, from the previous
+    // range's exit as the walks left it
+    for (uint32_t c0 = R; c0 < nc; c0 += R) {
+        const uint32_t c_end = c0 + R < nc ? c0 + R : nc;
+        st_merges += lane_walk_chain(sp, w, ht, c0, c_end, chunk_exit(sp, w, c0 - 1u));
+    }
+    // k_check: every chunk whose walk start differs from its predecessor's exit
+    for (uint32_t c = 0; c < nc; ++c) {
+        ff[c] = 0;
+        if (c >= 1 && went[c] != chunk_exit(sp, w, c - 1)) fl[status[1]++] = c;
+    }
     st_fail = status[1];
     // k_fixup (same phases as the kernel)
     uint32_t* list = fl.data();

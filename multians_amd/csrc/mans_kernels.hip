@@ -135,8 +135,8 @@ __device__ __forceinline__ void round_load(const RoundIO& io, uint32_t r, Prefet
         uint32_t lane, piece;
         piece_of(j, &lane, &piece);
         const uint32_t rb = io.rbase[lane];
-        const uint32_t b = rb + r * kR + 4u * piece;
-        const U4 u = load_block(io.units, rb == kNone ? 0u : io.n, b);
+        U4 u{0u, 0u, 0u, 0u};
+        if (rb != kNone) u = load_block(io.units, io.n, rb + r * kR + 4u * piece);
         pf.v[j] = make_uint4(u.x, u.y, u.z, u.w);
     }
 }
@@ -197,9 +197,16 @@ __device__ __forceinline__ void round_decode(const LdsTab& tab, const uint32_t* 
 
 // Run rounds (at most nr_max, uniform) until no lane is active.  After each round,
 // at_end(L, re) is called for lanes that reached a round boundary re < u_stop (their
-// state is the entry of unit re); it may deactivate the lane.
-template <typename F>
-__device__ void run_rounds(const LdsTab& tab, RoundIO& io, RLane& L, uint32_t nr_max, F&& at_end) {
+// state is the entry of unit re); it may deactivate the lane.  at_stop(L) is called in
+// the round in which a lane reaches the end of its region.  A lane that stops has its
+// unit loads switched off from the next round on.
+struct NoStop {
+    __device__ void operator()(RLane&) const {}
+};
+
+template <typename F, typename G = NoStop>
+__device__ void run_rounds(const LdsTab& tab, RoundIO& io, RLane& L, uint32_t nr_max, F&& at_end,
+                           G&& at_stop = G()) {
     __syncthreads();                       // rbase[] visible
     Prefetch pf;
     round_load(io, 0u, pf);
@@ -215,7 +222,9 @@ __device__ void run_rounds(const LdsTab& tab, RoundIO& io, RLane& L, uint32_t nr
                 if (L.u == re) at_end(L, re);
             } else {
                 L.active = 0u;             // region done
+                at_stop(L);
             }
+            if (!L.active) io.rbase[threadIdx.x] = kNone;
         }
         if (!__syncthreads_or(L.active)) break;
     }
@@ -280,6 +289,87 @@ __device__ uint32_t walk_rounds(const StreamPlan& sp, const Work& w, const LdsTa
     return merged;
 }
 
+// Walk from `entry` at the start of chunk c (c >= 1) and, while it has not merged with
+// the stored spec path, on through the following chunks up to chunk c_end (exclusive),
+// for the lanes with want != 0.  Each chunk the walk enters gets its full record
+// (wentry, wb/wbc up to the merge, then mq/wcm or wexit/wtotal), written in the round
+// the walk reaches that point.  Walks of one workgroup run in lockstep rounds, so when
+// several walks pass through one chunk the one that started further back writes last;
+// that walk's start is the exit recorded for its predecessor chunk, so the records of
+// a range stay consistent.  Returns 1 if the walk merged.
+// nr_max (uniform over the workgroup) bounds the rounds of every lane's walk.
+// With mrg (LDS, one word per lane, zeroed by the caller) the lanes publish when their
+// walk merges, and a walk leaves its first chunk only if the previous lane's walk merged
+// inside its own chunk: otherwise its start (the previous chunk's spec exit) is not on
+// the true path, and continuing would overwrite good records with a wrong path (the
+// previous lane's continuing walk covers those chunks instead).
+__device__ uint32_t walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& tab,
+                               RoundIO& io, uint32_t want, uint32_t c, uint32_t c_end,
+                               uint32_t entry, uint32_t nr_max, uint32_t* mrg = nullptr) {
+    const uint64_t qb = sp.ck_base;
+    uint32_t gc = sp.chunk_base + c;       // chunk the walk is in
+    const uint32_t gc0 = gc;
+    RLane L;
+    L.active = 0u;
+    uint32_t merged = 0u;
+    if (want) {
+        const uint32_t u0 = c * sp.S;
+        L.u_stop = min(c_end * sp.S, sp.n_units);
+        L.rbase = u0;
+        w.wentry[gc] = entry;
+        const uint32_t q0 = u0 >> sp.log2K;
+        if (ld_coherent(&w.ck[qb + q0]) == entry) {
+            w.mq[gc] = q0;
+            w.wcm[gc] = 0u;
+            merged = 1u;
+            if (mrg) mrg[threadIdx.x] = 1u;
+        } else {
+            w.wb[qb + q0] = entry;
+            w.wbc[qb + q0] = 0u;
+            L.active = 1u;
+            L.xi = entry_state(entry);
+            L.u = u0 + (entry_off(entry) >> 5);
+            L.o = entry_off(entry) & 31u;
+            L.cnt = 0u;
+        }
+    }
+    io.rbase[threadIdx.x] = L.active ? L.rbase : kNone;
+    const uint32_t smask = sp.S - 1u;
+    run_rounds(tab, io, L, nr_max,
+        [&](RLane& l, uint32_t re) {
+            const uint32_t v = pack_entry(l.xi, l.o);
+            if ((re & smask) == 0u) {      // chunk boundary: close this chunk
+                w.mq[gc] = kNone;
+                w.wexit[gc] = v;
+                w.wtotal[gc] = l.cnt;
+                if (mrg && gc == gc0 && threadIdx.x > 0u && !mrg[threadIdx.x - 1u]) {
+                    l.active = 0u;         // start not proven: stop here
+                    return;
+                }
+                ++gc;                      // enter the next
+                w.wentry[gc] = v;
+                l.cnt = 0u;
+            }
+            const uint32_t q = re >> sp.log2K;
+            if (ld_coherent(&w.ck[qb + q]) == v) {
+                w.mq[gc] = q;
+                w.wcm[gc] = l.cnt;
+                merged = 1u;
+                if (mrg && gc == gc0) mrg[threadIdx.x] = 1u;
+                l.active = 0u;
+                return;
+            }
+            w.wb[qb + q] = v;
+            w.wbc[qb + q] = l.cnt;
+        },
+        [&](RLane& l) {                    // end of the range without a merge
+            w.mq[gc] = kNone;
+            w.wexit[gc] = pack_entry(l.xi, l.o);
+            w.wtotal[gc] = l.cnt;
+        });
+    return merged;
+}
+
 // Spec pass for 256 consecutive chunks of one stream, then the walks into chunks
 // c0+1 .. c0+255 (their predecessors' exits are this workgroup's own results).
 __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
@@ -341,18 +431,15 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
     }
     __threadfence();
     __syncthreads();
-    // ---- walks into c+1 from this lane's exit
+    // ---- walks from this lane's exit into c+1, continuing until they merge, at most
+    // to the end of this workgroup's range (consistency is checked by k_check)
+    const uint32_t c_end = min(c0 + (uint32_t)kWG, sp.n_chunks);
     const uint32_t tgt = c + 1u;
-    const uint32_t want = (threadIdx.x + 1u < kWG && tgt < sp.n_chunks) ? 1u : 0u;
-    const uint32_t ok = walk_rounds(sp, w, tab, io, want, tgt, my_exit);
-    if (want) {
-        const uint32_t gt = sp.chunk_base + tgt;
-        w.fflag[gt] = 0u;
-        if (!ok && tgt + 1u < sp.n_chunks && w.wexit[gt] != ld_coherent(&w.exitv[gt])) {
-            uint32_t slot = atomicAdd(&w.fcnt[s_stream], 1u);
-            w.flist[sp.chunk_base + slot] = gt + 1u;
-        }
-    }
+    const uint32_t want = tgt < c_end ? 1u : 0u;
+    const uint32_t nr = c0 + 1u < c_end ? ((c_end - c0 - 1u) * sp.S + kR - 1u) / kR : 0u;
+    __shared__ uint32_t s_mrg[kWG];
+    s_mrg[threadIdx.x] = 0u;               // visible after run_rounds' first barrier
+    (void)walk_chain(sp, w, tab, io, want, tgt, c_end, my_exit, nr, s_mrg);
 }
 
 // Walks into the first chunk of each workgroup range (chunks c0 = 256k, k >= 1), from
@@ -374,15 +461,30 @@ __global__ __launch_bounds__(kWG) void k_seam(KArgs a) {
     LdsTab tab{lds};
     RoundIO io{lds + sp.L, s_rbase, sp.units, sp.n_units};
     const Work& w = a.w;
-    const uint32_t gt = sp.chunk_base + tgt;
-    const uint32_t entry = want ? w.exitv[gt - 1u] : 0u;
-    const uint32_t ok = walk_rounds(sp, w, tab, io, want, tgt, entry);
-    if (want) {
-        w.fflag[gt] = 0u;
-        if (!ok && tgt + 1u < sp.n_chunks && w.wexit[gt] != w.exitv[gt]) {
-            uint32_t slot = atomicAdd(&w.fcnt[s_stream], 1u);
-            w.flist[sp.chunk_base + slot] = gt + 1u;
-        }
+    // the previous range's exit as its walks left it (another seam walk may still change
+    // it; k_check finds any chunk whose walk started from a different exit)
+    const uint32_t entry = want ? chunk_exit(sp, w, tgt - 1u) : 0u;
+    const uint32_t c_end = want ? min(tgt + (uint32_t)kWG, sp.n_chunks) : tgt;
+    (void)walk_chain(sp, w, tab, io, want, tgt, c_end, entry, (kWG * sp.S) / kR);
+}
+
+// Consistency check over all chunks (grid as k_sync): a chunk whose recorded walk start
+// differs from its predecessor's exit goes on its stream's fixup list.  Clears fflag.
+__global__ __launch_bounds__(kWG) void k_check(KArgs a) {
+    __shared__ uint32_t s_stream;
+    const uint32_t t = blockIdx.x;
+    if (threadIdx.x == 0) s_stream = stream_of_task<false>(a, t);
+    __syncthreads();
+    const uint32_t sid = __builtin_amdgcn_readfirstlane(s_stream);
+    const StreamPlan sp = *plan_of(a, sid);
+    const Work& w = a.w;
+    const uint32_t c = (t - sp.task_chunk) * kWG + threadIdx.x;
+    if (c >= sp.n_chunks) return;
+    const uint32_t g = sp.chunk_base + c;
+    w.fflag[g] = 0u;
+    if (c >= 1u && w.wentry[g] != chunk_exit(sp, w, c - 1u)) {
+        const uint32_t slot = atomicAdd(&w.fcnt[sid], 1u);
+        w.flist[sp.chunk_base + slot] = g;
     }
 }
 
@@ -408,10 +510,17 @@ __global__ __launch_bounds__(kWG) void k_fixup(KArgs a) {
         RoundIO io{lds + sp.L, s_rbase, sp.units, sp.n_units};
         uint32_t* list = w.flist + sp.chunk_base;
         uint32_t* next = w.flist2 + sp.chunk_base;
+        if (threadIdx.x == 0) {                                    // statistics
+            atomicAdd(&w.status[5], 1u);
+            atomicMax(&w.status[6], s_n);
+        }
         for (uint32_t round = 0;; ++round) {
             __syncthreads();
             const uint32_t n = s_n;
-            if (n == 0u) break;
+            if (n == 0u) {
+                if (threadIdx.x == 0) atomicMax(&w.status[4], round);
+                break;
+            }
             for (uint32_t i = threadIdx.x; i < n; i += kWG) {        // capture + mark
                 const uint32_t g = list[i];
                 next[i] = chunk_exit(sp, w, g - sp.chunk_base - 1u);
@@ -692,12 +801,36 @@ static uint32_t ilog2(uint32_t v) { uint32_t r = 0; while ((1u << (r + 1)) <= v)
 // the typical distance over which a guessed-state decode merges with the true
 // path (tens to hundreds of units at L = 4096), so the spec and walk passes stay
 // close to one decode of the stream.
+static std::mutex g_rules_mu;
+static std::vector<mans_geometry_rule> g_rules;
+
+// Built-in rules: the autotuner's table for MI355X (multians_amd/tune.py); empty until
+// a tuning run has filled it.
+static const mans_geometry_rule kTunedRules[] = {
+#include "tuned_rules.inc"
+    {0u, 0u, 0.0, 0ull}};
+
+static uint32_t rule_chunk(uint64_t n_units, uint64_t n_out, uint32_t L) {
+    const double spu = n_units ? (double)n_out / (double)n_units : 4.0;
+    {
+        std::lock_guard<std::mutex> lk(g_rules_mu);
+        for (const mans_geometry_rule& r : g_rules)
+            if (L <= r.L_max && spu <= r.spu_max && n_units <= r.units_max) return r.chunk_units;
+        if (!g_rules.empty()) return 0u;
+    }
+    for (const mans_geometry_rule& r : kTunedRules)
+        if (r.chunk_units && L <= r.L_max && spu <= r.spu_max && n_units <= r.units_max)
+            return r.chunk_units;
+    return 0u;
+}
+
 void choose_geometry(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t req_S,
                      uint32_t req_K, uint32_t* S, uint32_t* K) {
     // checkpoints sit at round boundaries of the sync kernels: K is the round length
     (void)req_K;
     const uint32_t k = kR;
     uint32_t s = req_S;
+    if (!s) s = rule_chunk(n_units, n_out, L);
     if (!s) {
         // s_safe: long against the distance over which a guessed-state decode merges
         // with the true path, so that walks rarely fail.  Measured at L = 4096 (units,
@@ -798,6 +931,7 @@ static int launch_pipeline(const KArgs& a, uint32_t n_chunk_tasks, uint32_t n_it
     hipLaunchKernelGGL(k_sync, dim3(n_chunk_tasks), dim3(kWG), lds_s, st, a);
     mark(2);
     hipLaunchKernelGGL(k_seam, dim3(n_chunk_tasks), dim3(kWG), lds_s, st, a);
+    hipLaunchKernelGGL(k_check, dim3(n_chunk_tasks), dim3(kWG), 0, st, a);
     mark(3);
     hipLaunchKernelGGL(k_fixup, dim3(a.n_streams), dim3(kWG), lds_s, st, a);
     mark(4);
@@ -1036,5 +1170,105 @@ extern "C" int mans_decode_batch(const mans_stream_desc* descs, size_t n,
         stats->n_items = (uint32_t)cks;
         stats->status = 0;
     }
+    return MANS_OK;
+}
+
+// ------------------------------------------------------------ graphs and rules
+namespace {
+struct GraphHandle {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+}  // namespace
+
+extern "C" int mans_batch_graph_create(const mans_stream_desc* descs, size_t n,
+                                       const mans_batch_opts* opts, void** graph) {
+    if (!descs || !n || !opts || !graph) return set_error(MANS_EINVAL, "bad batch");
+    *graph = nullptr;
+    std::vector<StreamPlan> plans;
+    uint64_t chunks, cks, units, syms;
+    uint32_t ct, it, ml, mk;
+    int rc = build_plans(descs, n, plans, &chunks, &cks, &ct, &it, &ml, &mk, &units, &syms);
+    if (rc) return rc;
+    Layout l = make_layout(chunks, cks, n);
+    if (!opts->workspace || opts->workspace_bytes < l.bytes)
+        return set_error(MANS_ENOMEM, "batch workspace of %llu bytes < %llu needed",
+                         (unsigned long long)opts->workspace_bytes, (unsigned long long)l.bytes);
+    StreamPlan* dplans = (StreamPlan*)((char*)opts->workspace + l.o_plans);
+    hipError_t e = hipMemcpy(dplans, plans.data(), sizeof(StreamPlan) * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return set_error(MANS_EHIP, "plan upload: %s", hipGetErrorString(e));
+    KArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.plans = dplans;
+    a.n_streams = (uint32_t)n;
+    a.n_tasks = ct;
+    a.w = make_work(opts->workspace, l);
+    hipStream_t cs = nullptr;
+    e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e != hipSuccess) return set_error(MANS_EHIP, "capture stream: %s", hipGetErrorString(e));
+    GraphHandle* g = new GraphHandle();
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        rc = launch_pipeline(a, ct, it, (uint32_t)chunks, ml, mk, cs, nullptr);
+        hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
+        if (!rc && e2 != hipSuccess) rc = set_error(MANS_EHIP, "end capture: %s", hipGetErrorString(e2));
+    } else {
+        rc = set_error(MANS_EHIP, "begin capture: %s", hipGetErrorString(e));
+    }
+    if (!rc) {
+        e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = set_error(MANS_EHIP, "instantiate: %s", hipGetErrorString(e));
+    }
+    (void)hipStreamDestroy(cs);
+    if (rc) {
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        delete g;
+        return rc;
+    }
+    *graph = g;
+    return MANS_OK;
+}
+
+extern "C" int mans_graph_launch(void* graph, void* stream) {
+    GraphHandle* g = (GraphHandle*)graph;
+    if (!g || !g->exec) return set_error(MANS_EINVAL, "graph is NULL");
+    hipError_t e = hipGraphLaunch(g->exec, (hipStream_t)stream);
+    if (e != hipSuccess) return set_error(MANS_EHIP, "graph launch: %s", hipGetErrorString(e));
+    return MANS_OK;
+}
+
+extern "C" int mans_graph_destroy(void* graph) {
+    GraphHandle* g = (GraphHandle*)graph;
+    if (!g) return MANS_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return MANS_OK;
+}
+
+extern "C" int mans_set_geometry_rules(const mans_geometry_rule* rules, size_t n) {
+    if (n && !rules) return set_error(MANS_EINVAL, "rules is NULL");
+    std::vector<mans_geometry_rule> v;
+    for (size_t i = 0; i < n; ++i) {
+        mans_geometry_rule r = rules[i];
+        if (r.chunk_units < 128u || r.chunk_units > (1u << 24))
+            return set_error(MANS_EINVAL, "rule %zu: chunk_units %u outside [128, 2^24]", i,
+                             r.chunk_units);
+        uint32_t p = 128u;
+        while (p * 2u <= r.chunk_units) p *= 2u;
+        r.chunk_units = p;
+        v.push_back(r);
+    }
+    std::lock_guard<std::mutex> lk(g_rules_mu);
+    g_rules.swap(v);
+    return MANS_OK;
+}
+
+extern "C" int mans_chunk_units(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t* chunk_units) {
+    if (!chunk_units) return set_error(MANS_EINVAL, "chunk_units is NULL");
+    uint32_t S, K;
+    choose_geometry(n_units, n_out, L, 0, 0, &S, &K);
+    *chunk_units = S;
     return MANS_OK;
 }

@@ -157,6 +157,17 @@ class DeviceBatch:
                                                 C.byref(stats)))
         return stats
 
+    def graph(self):
+        """Capture this batch's decode into a hipGraph (see mans_batch_graph_create);
+        returns a BatchGraph whose launch() replays it."""
+        o = _lib.BatchOpts()
+        o.workspace = self.ws.data_ptr()
+        o.workspace_bytes = self.ws.numel()
+        h = C.c_void_p()
+        _lib.check(_lib.lib().mans_batch_graph_create(self.descs, len(self.streams), C.byref(o),
+                                                      C.byref(h)))
+        return BatchGraph(h, self)
+
     def plain_args(self, i, ws, stream=None):
         """mans_decode arguments for stream i alone (per-stream plain launch), using the
         batch's device buffers and the caller's workspace tensor."""
@@ -179,3 +190,27 @@ class DeviceBatch:
         _lib.check(_lib.lib().mans_decode_status(C.c_void_p(self.ws.data_ptr()), C.c_void_p(st),
                                                  C.byref(s)))
         return s.value
+
+
+class BatchGraph:
+    """An instantiated hipGraph of one batch's decode; keeps the batch's buffers alive."""
+
+    def __init__(self, handle, batch):
+        self.h = handle
+        self.batch = batch
+
+    def launch(self, stream=None):
+        torch = _torch()
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        _lib.check(_lib.lib().mans_graph_launch(self.h, C.c_void_p(st)))
+
+    def close(self):
+        if self.h:
+            _lib.lib().mans_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

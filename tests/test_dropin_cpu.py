@@ -97,3 +97,41 @@ def test_product_codec_matches_oracle(lam, L):
     enc, _ = _lib.build_tables(counts, L)
     _, encarr, _ = ol.build_tables(counts, L)
     assert np.array_equal(enc.reshape(256, L, 3), encarr)
+
+
+def test_geometry_rules_pick_chunk_length():
+    """Autotuner rules: first matching rule wins, lengths round down to a power of two,
+    short lengths are refused, and clearing the rules restores the built-in choice."""
+    from multians_amd import _lib
+    builtin = _lib.chunk_units(1 << 20, 4 << 20, 4096)
+    try:
+        _lib.set_geometry_rules([(1024, 100.0, 1 << 30, 300),
+                                 (4096, 5.0, 1 << 20, 1000),
+                                 (4096, 100.0, 1 << 30, 4096)])
+        assert _lib.chunk_units(1 << 20, 4 << 20, 4096) == 512      # second rule
+        assert _lib.chunk_units(1 << 20, 4 << 20, 1024) == 256      # first rule
+        assert _lib.chunk_units(1 << 20, 20 << 20, 4096) == 4096    # spu 20 -> third
+        assert _lib.chunk_units((1 << 20) + 1, 4 << 20, 4096) == 4096
+        with pytest.raises(_lib.MansError):
+            _lib.set_geometry_rules([(4096, 5.0, 1 << 20, 64)])
+        assert _lib.chunk_units(1 << 20, 4 << 20, 4096) == 512      # unchanged on error
+    finally:
+        _lib.set_geometry_rules([])
+    assert _lib.chunk_units(1 << 20, 4 << 20, 4096) == builtin
+
+
+def test_tuner_rule_table_reproduces_builtin_choice():
+    """The autotuner's rule table, filled with the built-in constants, picks exactly the
+    built-in chunk length for every stream shape (so tuned tables only move the
+    constants)."""
+    from multians_amd import _lib, tune
+    rng = np.random.default_rng(4)
+    shapes = [(int(np.exp(rng.uniform(0, np.log(1 << 32)))), float(np.exp(rng.uniform(0, 5))),
+               int(rng.choice([256, 1024, 2048, 4096, 8192]))) for _ in range(3000)]
+    builtin = [_lib.chunk_units(n, int(n * spu), L) for n, spu, L in shapes]
+    try:
+        _lib.set_geometry_rules(tune.heuristic_rules(tune.FLOOR0, tune.SAFE0))
+        tuned = [_lib.chunk_units(n, int(n * spu), L) for n, spu, L in shapes]
+    finally:
+        _lib.set_geometry_rules([])
+    assert tuned == builtin

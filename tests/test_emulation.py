@@ -96,3 +96,19 @@ def test_partial_output(emu):
     for n_out in [1, 17, 4096, 49999]:
         part, *_ = _run(emu, st, 64, 8, n_out=n_out)
         assert np.array_equal(part, full[:n_out])
+
+
+@pytest.mark.parametrize("lam", [0.05, 0.3, 1.0, 4.0])
+@pytest.mark.parametrize("S,K", [(8, 4), (16, 8), (32, 16)])
+def test_chained_walks_many_ranges(emu, lam, S, K):
+    """Chunks far shorter than the merge distance: walks chain through many chunks,
+    ranges of 256 chunks meet at seams, and the consistency check + fixup rounds must
+    still yield the true path at every checkpoint."""
+    sym = ol.gen_symbols(77, lam, 400_000 if lam < 1.0 else 3_000_000)
+    st, _ = ol.encode(sym, L=4096)
+    out, stats, ent, off = _run(emu, st, S, K)
+    assert stats[0] > 256                        # several ranges
+    assert np.array_equal(out[::-1], sym)
+    _, entries, counts, _, _ = ol.decode_seq(st, S=K)
+    assert np.array_equal(ent, entries)
+    assert np.array_equal(off, np.concatenate([[0], np.cumsum(counts.astype(np.uint64))[:-1]]))

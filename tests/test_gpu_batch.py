@@ -57,3 +57,30 @@ def test_batch_repeat_is_stable():
     assert torch.equal(first, b.out)
     for i, (es, sym) in enumerate(items):
         assert np.array_equal(b.output(i).cpu().numpy()[::-1], sym)
+
+
+def test_batch_graph_replays_match_eager():
+    """A captured batch decode, replayed several times with eager work in between
+    (output cleared, another decode on the same device), gives the eager result on
+    every replay."""
+    import torch
+    from multians_amd.decoder import DeviceBatch, DeviceStream
+    print("case graph replays", flush=True)
+    items = _mixed(60, 21, max_log2=21)
+    b = DeviceBatch([es for es, _ in items])
+    b.decode()
+    torch.cuda.synchronize()
+    eager = b.out.clone()
+    g = b.graph()
+    other = _mixed(3, 22)
+    for rep in range(4):
+        b.out.zero_()
+        ds = DeviceStream(other[rep % 3][0])      # eager work between replays
+        ds.decode()
+        g.launch()
+        torch.cuda.synchronize()
+        assert b.status() == 0
+        assert torch.equal(b.out, eager), "replay %d" % rep
+    for i, (es, sym) in enumerate(items):
+        assert np.array_equal(b.output(i).cpu().numpy()[::-1], sym)
+    g.close()
