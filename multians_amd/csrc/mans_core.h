@@ -289,6 +289,25 @@ struct Work {
     uint32_t* fcnt;      // per stream: entries in its fixup list
 };
 
+// Storage slot of checkpoint q of a stream.  Chunk c = q / P (P = S/K checkpoints per
+// chunk) lies in the 256-chunk range starting at c0 = c & ~255 (one sync workgroup), and
+// checkpoint r = q % P of lane l = c - c0 is stored at ck_base + c0*P + r*width + l,
+// width = min(256, n_chunks - c0): the lanes of a workgroup store a round's checkpoints
+// side by side (full-line writes).  A stream's slots span n_chunks * P entries.
+MANS_HD uint64_t ck_slot(const StreamPlan& sp, uint32_t q) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t lp = (uint32_t)__builtin_ctz(sp.S) - sp.log2K;
+#else
+    uint32_t lp = 0;
+    while ((sp.K << lp) < sp.S) ++lp;
+#endif
+    const uint32_t c = q >> lp, r = q & ((1u << lp) - 1u);
+    const uint32_t c0 = c & ~255u, l = c & 255u;
+    const uint32_t rest = sp.n_chunks - c0;
+    const uint32_t width = rest < 256u ? rest : 256u;
+    return sp.ck_base + ((uint64_t)c0 << lp) + (uint64_t)r * width + l;
+}
+
 // ---------------------------------------------------------------- lane bodies
 
 // Spec pass for chunk c of stream sp (lane body).
@@ -308,11 +327,10 @@ MANS_HD void lane_spec(const StreamPlan& sp, const Work& w, const Tab& tab, uint
         xi = sp.first_state - L;   // guessed state (the reference guesses the same)
     }
     bcursor_init(cur, sp.units, sp.n_units, u0, off0);
-    const uint64_t qb = sp.ck_base;
     const uint32_t gc = sp.chunk_base + c;
     const uint32_t kmask = sp.K - 1u;
-    w.ck[qb + (u0 >> sp.log2K)] = pack_entry(xi, off0);
-    w.ckc[qb + (u0 >> sp.log2K)] = 0u;
+    w.ck[ck_slot(sp, u0 >> sp.log2K)] = pack_entry(xi, off0);
+    w.ckc[ck_slot(sp, u0 >> sp.log2K)] = 0u;
     if (c == 0u) {
         w.mq[gc] = 0u;
         w.wcm[gc] = 0u;
@@ -324,7 +342,7 @@ MANS_HD void lane_spec(const StreamPlan& sp, const Work& w, const Tab& tab, uint
         bstep(tab, cur, xi, crossed);
         ++cnt;
         if (crossed && (cur.u & kmask) == 0u && cur.u < u_stop) {
-            uint64_t q = qb + (cur.u >> sp.log2K);
+            uint64_t q = ck_slot(sp, cur.u >> sp.log2K);
             w.ck[q] = pack_entry(xi, cur.o);
             w.ckc[q] = cnt;
         }
@@ -341,18 +359,17 @@ MANS_HD uint32_t lane_walk(const StreamPlan& sp, const Work& w, const Tab& tab, 
     const uint32_t u0 = c * sp.S;
     uint32_t u_stop = u0 + sp.S;
     if (u_stop > sp.n_units) u_stop = sp.n_units;
-    const uint64_t qb = sp.ck_base;
     const uint32_t gc = sp.chunk_base + c;
     const uint32_t kmask = sp.K - 1u;
     w.wentry[gc] = entry;
     uint32_t q0 = u0 >> sp.log2K;
-    if (w.ck[qb + q0] == entry) {
+    if (w.ck[ck_slot(sp, q0)] == entry) {
         w.mq[gc] = q0;
         w.wcm[gc] = 0u;
         return 1u;
     }
-    w.wb[qb + q0] = entry;
-    w.wbc[qb + q0] = 0u;
+    w.wb[ck_slot(sp, q0)] = entry;
+    w.wbc[ck_slot(sp, q0)] = 0u;
     BCursor cur;
     uint32_t xi = entry_state(entry);
     bcursor_init(cur, sp.units, sp.n_units, u0, entry_off(entry));
@@ -364,13 +381,13 @@ MANS_HD uint32_t lane_walk(const StreamPlan& sp, const Work& w, const Tab& tab, 
         if (crossed && (cur.u & kmask) == 0u && cur.u < u_stop) {
             uint32_t q = cur.u >> sp.log2K;
             uint32_t v = pack_entry(xi, cur.o);
-            if (w.ck[qb + q] == v) {
+            if (w.ck[ck_slot(sp, q)] == v) {
                 w.mq[gc] = q;
                 w.wcm[gc] = cnt;
                 return 1u;
             }
-            w.wb[qb + q] = v;
-            w.wbc[qb + q] = cnt;
+            w.wb[ck_slot(sp, q)] = v;
+            w.wbc[ck_slot(sp, q)] = cnt;
         }
     }
     w.mq[gc] = kNone;
@@ -391,18 +408,17 @@ MANS_HD uint32_t lane_walk_chain(const StreamPlan& sp, const Work& w, const Tab&
     const uint32_t u0 = c * sp.S;
     uint32_t u_stop = c_end * sp.S;
     if (u_stop > sp.n_units) u_stop = sp.n_units;
-    const uint64_t qb = sp.ck_base;
     uint32_t gc = sp.chunk_base + c;
     const uint32_t kmask = sp.K - 1u, smask = sp.S - 1u;
     w.wentry[gc] = entry;
     uint32_t q0 = u0 >> sp.log2K;
-    if (w.ck[qb + q0] == entry) {
+    if (w.ck[ck_slot(sp, q0)] == entry) {
         w.mq[gc] = q0;
         w.wcm[gc] = 0u;
         return 1u;
     }
-    w.wb[qb + q0] = entry;
-    w.wbc[qb + q0] = 0u;
+    w.wb[ck_slot(sp, q0)] = entry;
+    w.wbc[ck_slot(sp, q0)] = 0u;
     BCursor cur;
     uint32_t xi = entry_state(entry);
     bcursor_init(cur, sp.units, sp.n_units, u0, entry_off(entry));
@@ -422,13 +438,13 @@ MANS_HD uint32_t lane_walk_chain(const StreamPlan& sp, const Work& w, const Tab&
                 cnt = 0u;
             }
             const uint32_t q = cur.u >> sp.log2K;
-            if (w.ck[qb + q] == v) {
+            if (w.ck[ck_slot(sp, q)] == v) {
                 w.mq[gc] = q;
                 w.wcm[gc] = cnt;
                 return 1u;
             }
-            w.wb[qb + q] = v;
-            w.wbc[qb + q] = cnt;
+            w.wb[ck_slot(sp, q)] = v;
+            w.wbc[ck_slot(sp, q)] = cnt;
         }
     }
     w.mq[gc] = kNone;
@@ -445,9 +461,8 @@ MANS_HD uint32_t lane_merges_in_chunk(const StreamPlan& sp, const Work& w, const
     const uint32_t u0 = c * sp.S;
     uint32_t u_stop = u0 + sp.S;
     if (u_stop > sp.n_units) u_stop = sp.n_units;
-    const uint64_t qb = sp.ck_base;
     const uint32_t kmask = sp.K - 1u;
-    if (w.ck[qb + (u0 >> sp.log2K)] == entry) return 1u;
+    if (w.ck[ck_slot(sp, u0 >> sp.log2K)] == entry) return 1u;
     BCursor cur;
     uint32_t xi = entry_state(entry);
     bcursor_init(cur, sp.units, sp.n_units, u0, entry_off(entry));
@@ -455,7 +470,7 @@ MANS_HD uint32_t lane_merges_in_chunk(const StreamPlan& sp, const Work& w, const
         uint32_t crossed;
         bstep(tab, cur, xi, crossed);
         if (crossed && (cur.u & kmask) == 0u && cur.u < u_stop &&
-            w.ck[qb + (cur.u >> sp.log2K)] == pack_entry(xi, cur.o))
+            w.ck[ck_slot(sp, cur.u >> sp.log2K)] == pack_entry(xi, cur.o))
             return 1u;
     }
     return 0u;
@@ -472,7 +487,7 @@ MANS_HD uint32_t chunk_count(const StreamPlan& sp, const Work& w, uint32_t c) {
     const uint32_t gc = sp.chunk_base + c;
     uint32_t m = w.mq[gc];
     if (m == kNone) return w.wtotal[gc];
-    return w.wcm[gc] + (w.own_total[gc] - w.ckc[sp.ck_base + m]);
+    return w.wcm[gc] + (w.own_total[gc] - w.ckc[ck_slot(sp, m)]);
 }
 
 // True entry and chunk-relative step count at checkpoint q.
@@ -480,14 +495,14 @@ MANS_HD void checkpoint_truth(const StreamPlan& sp, const Work& w, uint32_t q, u
                               uint32_t* cum) {
     const uint32_t c = (q << sp.log2K) / sp.S;
     const uint32_t gc = sp.chunk_base + c;
-    const uint64_t qq = sp.ck_base + q;
+    const uint64_t qq = ck_slot(sp, q);
     uint32_t m = w.mq[gc];
     if (m == kNone || q < m) {
         *entry = w.wb[qq];
         *cum = w.wbc[qq];
     } else {
         *entry = w.ck[qq];
-        *cum = w.wcm[gc] + (w.ckc[qq] - w.ckc[sp.ck_base + m]);
+        *cum = w.wcm[gc] + (w.ckc[qq] - w.ckc[ck_slot(sp, m)]);
     }
 }
 

@@ -54,7 +54,8 @@ int mans_emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint6
     std::vector<uint32_t> exitv(nc), own(nc), went(nc), wexit(nc), wtot(nc), mq(nc), wcm(nc),
         ff(nc), fl(nc + 1), fl2(nc + 1), status(64), fcnt(2);
     std::vector<uint64_t> coff(nc);
-    std::vector<uint32_t> ck(nq), ckc(nq), wb(nq), wbc(nq);
+    const uint64_t ns = (uint64_t)nc * (S / K);     // checkpoint slots (see ck_slot)
+    std::vector<uint32_t> ck(ns), ckc(ns), wb(ns), wbc(ns);
     Work w;
     w.exitv = exitv.data(); w.own_total = own.data(); w.wentry = went.data();
     w.wexit = wexit.data(); w.wtotal = wtot.data(); w.mq = mq.data(); w.wcm = wcm.data();

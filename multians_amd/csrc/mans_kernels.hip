@@ -238,7 +238,6 @@ __device__ __forceinline__ uint32_t ld_coherent(const uint32_t* p) {
 // Same semantics as mans::lane_walk with K == kR.  Returns 1 if merged.
 __device__ uint32_t walk_rounds(const StreamPlan& sp, const Work& w, const LdsTab& tab,
                                 RoundIO& io, uint32_t want, uint32_t c, uint32_t entry) {
-    const uint64_t qb = sp.ck_base;
     const uint32_t gc = sp.chunk_base + c;
     RLane L;
     L.active = 0u;
@@ -249,13 +248,13 @@ __device__ uint32_t walk_rounds(const StreamPlan& sp, const Work& w, const LdsTa
         L.rbase = u0;
         w.wentry[gc] = entry;
         const uint32_t q0 = u0 >> sp.log2K;
-        if (ld_coherent(&w.ck[qb + q0]) == entry) {
+        if (ld_coherent(&w.ck[ck_slot(sp, q0)]) == entry) {
             w.mq[gc] = q0;
             w.wcm[gc] = 0u;
             merged = 1u;
         } else {
-            w.wb[qb + q0] = entry;
-            w.wbc[qb + q0] = 0u;
+            w.wb[ck_slot(sp, q0)] = entry;
+            w.wbc[ck_slot(sp, q0)] = 0u;
             L.active = 1u;
             L.xi = entry_state(entry);
             L.u = u0 + (entry_off(entry) >> 5);
@@ -267,7 +266,7 @@ __device__ uint32_t walk_rounds(const StreamPlan& sp, const Work& w, const LdsTa
     run_rounds(tab, io, L, (sp.S + kR - 1u) / kR, [&](RLane& l, uint32_t re) {
         const uint32_t q = re >> sp.log2K;
         const uint32_t v = pack_entry(l.xi, l.o);
-        if (ld_coherent(&w.ck[qb + q]) == v) {
+        if (ld_coherent(&w.ck[ck_slot(sp, q)]) == v) {
             mq = q;
             wcm = l.cnt;
             merged = 1u;
@@ -275,8 +274,8 @@ __device__ uint32_t walk_rounds(const StreamPlan& sp, const Work& w, const LdsTa
             l.active = 0u;
             return;
         }
-        w.wb[qb + q] = v;
-        w.wbc[qb + q] = l.cnt;
+        w.wb[ck_slot(sp, q)] = v;
+        w.wbc[ck_slot(sp, q)] = l.cnt;
     });
     if (want && late) {                 // merged during the rounds
         w.mq[gc] = mq;
@@ -306,7 +305,6 @@ __device__ uint32_t walk_rounds(const StreamPlan& sp, const Work& w, const LdsTa
 __device__ uint32_t walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& tab,
                                RoundIO& io, uint32_t want, uint32_t c, uint32_t c_end,
                                uint32_t entry, uint32_t nr_max, uint32_t* mrg = nullptr) {
-    const uint64_t qb = sp.ck_base;
     uint32_t gc = sp.chunk_base + c;       // chunk the walk is in
     const uint32_t gc0 = gc;
     RLane L;
@@ -318,14 +316,14 @@ __device__ uint32_t walk_chain(const StreamPlan& sp, const Work& w, const LdsTab
         L.rbase = u0;
         w.wentry[gc] = entry;
         const uint32_t q0 = u0 >> sp.log2K;
-        if (ld_coherent(&w.ck[qb + q0]) == entry) {
+        if (ld_coherent(&w.ck[ck_slot(sp, q0)]) == entry) {
             w.mq[gc] = q0;
             w.wcm[gc] = 0u;
             merged = 1u;
             if (mrg) mrg[threadIdx.x] = 1u;
         } else {
-            w.wb[qb + q0] = entry;
-            w.wbc[qb + q0] = 0u;
+            w.wb[ck_slot(sp, q0)] = entry;
+            w.wbc[ck_slot(sp, q0)] = 0u;
             L.active = 1u;
             L.xi = entry_state(entry);
             L.u = u0 + (entry_off(entry) >> 5);
@@ -351,7 +349,7 @@ __device__ uint32_t walk_chain(const StreamPlan& sp, const Work& w, const LdsTab
                 l.cnt = 0u;
             }
             const uint32_t q = re >> sp.log2K;
-            if (ld_coherent(&w.ck[qb + q]) == v) {
+            if (ld_coherent(&w.ck[ck_slot(sp, q)]) == v) {
                 w.mq[gc] = q;
                 w.wcm[gc] = l.cnt;
                 merged = 1u;
@@ -359,8 +357,8 @@ __device__ uint32_t walk_chain(const StreamPlan& sp, const Work& w, const LdsTab
                 l.active = 0u;
                 return;
             }
-            w.wb[qb + q] = v;
-            w.wbc[qb + q] = l.cnt;
+            w.wb[ck_slot(sp, q)] = v;
+            w.wbc[ck_slot(sp, q)] = l.cnt;
         },
         [&](RLane& l) {                    // end of the range without a merge
             w.mq[gc] = kNone;
@@ -390,15 +388,14 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
         // zero-bit cycles: only a single checkpoint interval is decodable (count-driven)
         if (c == 0u && sp.n_chunks) {
             const uint32_t off0 = 32u - sp.first_bit;
-            w.ck[sp.ck_base] = pack_entry(sp.first_state - sp.L, off0);
-            w.ckc[sp.ck_base] = 0u;
+            w.ck[ck_slot(sp, 0u)] = pack_entry(sp.first_state - sp.L, off0);
+            w.ckc[ck_slot(sp, 0u)] = 0u;
             w.mq[sp.chunk_base] = 0u;
             w.wcm[sp.chunk_base] = 0u;
         }
         return;
     }
     const bool valid = c < sp.n_chunks;
-    const uint64_t qb = sp.ck_base;
     // ---- spec
     RLane L;
     L.active = valid ? 1u : 0u;
@@ -411,8 +408,8 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
         L.u = u0 + (off0 >> 5);
         L.o = off0 & 31u;
         L.cnt = 0u;
-        w.ck[qb + (u0 >> sp.log2K)] = pack_entry(L.xi, off0);
-        w.ckc[qb + (u0 >> sp.log2K)] = 0u;
+        w.ck[ck_slot(sp, u0 >> sp.log2K)] = pack_entry(L.xi, off0);
+        w.ckc[ck_slot(sp, u0 >> sp.log2K)] = 0u;
         if (c == 0u) {
             w.mq[sp.chunk_base] = 0u;
             w.wcm[sp.chunk_base] = 0u;
@@ -421,8 +418,8 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
     }
     s_rbase[threadIdx.x] = valid ? L.rbase : kNone;
     run_rounds(tab, io, L, (sp.S + kR - 1u) / kR, [&](RLane& l, uint32_t re) {
-        w.ck[qb + (re >> sp.log2K)] = pack_entry(l.xi, l.o);
-        w.ckc[qb + (re >> sp.log2K)] = l.cnt;
+        w.ck[ck_slot(sp, re >> sp.log2K)] = pack_entry(l.xi, l.o);
+        w.ckc[ck_slot(sp, re >> sp.log2K)] = l.cnt;
     });
     const uint32_t my_exit = pack_entry(L.xi, L.o);
     if (valid) {
@@ -849,10 +846,17 @@ void choose_geometry(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t req_
         uint32_t s_floor = spu <= 4.6 ? 128u : (spu <= 6.5 ? 256u : 512u);
         s = std::max<uint32_t>(s_floor, std::min(s_safe, s_fill));
     }
+    // a chunk longer than the stream gains nothing (and would waste checkpoint slots)
+    uint32_t cap = k;
+    while (cap < n_units && cap < (1u << 30)) cap <<= 1;
+    if (s > cap) s = cap;
     s = 1u << ilog2(s < k ? k : s);
     *S = s;
     *K = k;
 }
+
+// Checkpoint slots of a stream (see ck_slot): n_chunks * S / K.
+static uint64_t ck_slots(uint64_t n_chunks, uint32_t S, uint32_t K) { return n_chunks * (S / K); }
 
 // Workspace layout for a set of plans (shared by single and batch paths).
 struct Layout {
@@ -989,10 +993,11 @@ extern "C" int mans_workspace_bytes(uint64_t n_units, uint32_t L, uint64_t* byte
 extern "C" int mans_workspace_bytes_geometry(uint64_t n_units, uint32_t chunk_units,
                                              uint32_t checkpoint_units, uint64_t* bytes) {
     if (!bytes || !chunk_units || !checkpoint_units) return set_error(MANS_EINVAL, "bad geometry");
-    // checkpoints always sit at round boundaries (kR units), whatever K is asked for
-    const uint32_t k = std::min<uint32_t>(checkpoint_units, kR);
-    const uint32_t c = std::max<uint32_t>(chunk_units, kR);
-    uint64_t chunks = n_units / c + 2u, cks = n_units / k + 2u;
+    // the geometry mans_decode will use for these requests (checkpoints always sit at
+    // round boundaries, kR units, whatever K is asked for)
+    uint32_t S, K;
+    choose_geometry(n_units, 1u, 4096u, chunk_units, checkpoint_units, &S, &K);
+    uint64_t chunks = (n_units + S - 1u) / S + 2u, cks = ck_slots(chunks, S, K) + 2u;
     *bytes = make_layout(chunks, cks, 1).bytes;
     return MANS_OK;
 }
@@ -1025,7 +1030,7 @@ extern "C" int mans_decode(const mans_decode_args* d) {
     sp.ck_base = 0;
     sp.task_chunk = 0;
     sp.task_item = 0;
-    Layout l = make_layout(sp.n_chunks, sp.n_ck, 1);
+    Layout l = make_layout(sp.n_chunks, ck_slots(sp.n_chunks, S, K), 1);
     if (!d->workspace || d->workspace_bytes < l.bytes)
         return set_error(MANS_ENOMEM, "workspace of %llu bytes < %llu needed",
                          (unsigned long long)d->workspace_bytes, (unsigned long long)l.bytes);
@@ -1102,7 +1107,7 @@ static int build_plans(const mans_stream_desc* d, size_t n, std::vector<StreamPl
         sp.task_item = (uint32_t)ti;
         if (!d[i].n_out) sp.n_chunks = 0;
         cb += std::max<uint32_t>(sp.n_chunks, 1u);
-        qb += std::max<uint32_t>(sp.n_ck, 1u);
+        qb += std::max<uint64_t>(ck_slots(sp.n_chunks, S, K), 1u);
         tc += std::max<uint64_t>((sp.n_chunks + kWG - 1) / kWG, 1);
         ti += std::max<uint64_t>((n_write_items(sp.n_ck) + kWG - 1) / kWG, 1);
         tu += d[i].n_units;
