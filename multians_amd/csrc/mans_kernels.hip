@@ -649,8 +649,10 @@ __global__ __launch_bounds__(kWG) void k_scan_down(KArgs a, uint32_t total, cons
 // sync kernels' rounds amortise the per-pass work (offsets, barrier) over more steps.
 constexpr uint32_t kWF = kR >= 32u ? 1u : 32u / kR;
 constexpr uint32_t kWI = kR * kWF;          // units per item
-constexpr uint32_t kWL = kWI + 4u;          // units staged per lane
-constexpr uint32_t kWPieces = kWL / 4u;
+constexpr uint32_t kWPieces = kWI / 4u + 1u;  // 16-byte loads per lane (item + lookahead)
+// LDS stride per lane: the item plus the two units a step may look ahead (8-byte
+// aligned, stored as 8-byte halves); the shorter stride lets three workgroups share a CU
+constexpr uint32_t kWL = kWI + 2u;
 
 __host__ __device__ __forceinline__ uint32_t n_write_items(uint32_t n_ck) {
     return (n_ck + kWF - 1u) / kWF;
@@ -674,7 +676,9 @@ __device__ __forceinline__ void pass_store(uint32_t* buf, const WPrefetch& pf) {
     for (uint32_t j = 0; j < kWPieces; ++j) {
         const uint32_t idx = threadIdx.x + kWG * j;
         const uint32_t lane = idx / kWPieces, piece = idx - lane * kWPieces;
-        *reinterpret_cast<uint4*>(buf + lane * kWL + 4u * piece) = pf.v[j];
+        uint2* d = reinterpret_cast<uint2*>(buf + lane * kWL + 4u * piece);
+        d[0] = make_uint2(pf.v[j].x, pf.v[j].y);
+        if (piece + 1u < kWPieces) d[1] = make_uint2(pf.v[j].z, pf.v[j].w);
     }
 }
 
