@@ -714,13 +714,11 @@ __device__ __forceinline__ void item_write(const LdsTab& tab, const uint32_t* lb
         __builtin_amdgcn_sched_barrier(0);   // keep the unit read in this step
         return sym;                           // symbol in bits 0-7
     };
-    uint32_t i = 0;
-    const uint32_t head = (uint32_t)((4u - ((uintptr_t)dst & 3u)) & 3u);
-    const uint32_t h = head < cnt ? head : cnt;
-    for (; i < h; ++i) dst[i] = (uint8_t)next_sym();
-    gptr_w dw = (gptr_w)(dst + i);
-    const uint32_t nd = (cnt - i) >> 2;
-    for (uint32_t d = 0; d < nd; ++d) {
+    // Symbols are packed four to a dword and stored 16 bytes at a time: the lanes of a
+    // wave write 64 separate regions, so every store instruction costs one cache-line
+    // request per lane in the address unit, whatever its width.  Bytes, then dwords up
+    // to 16-byte alignment, then 16-byte stores, then the tail.
+    auto next_word = [&]() -> uint32_t {
         uint32_t acc = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -728,9 +726,34 @@ __device__ __forceinline__ void item_write(const LdsTab& tab, const uint32_t* lb
             // acc = (acc >> 8) | (symbol << 24): byte 0 of sym goes on top
             acc = __builtin_amdgcn_perm(sym, acc, 0x04030201u);
         }
-        dw[d] = acc;
+        return acc;
+    };
+    uint32_t i = 0;
+    const uint32_t head = (uint32_t)((4u - ((uintptr_t)dst & 3u)) & 3u);
+    const uint32_t h = head < cnt ? head : cnt;
+    for (; i < h; ++i) dst[i] = (uint8_t)next_sym();
+    gptr_w dw = (gptr_w)(dst + i);
+    uint32_t nd = (cnt - i) >> 2;                  // whole dwords left
+    const uint32_t wa = (uint32_t)(((16u - ((uintptr_t)(dst + i) & 15u)) & 15u) >> 2);
+    const uint32_t ha = wa < nd ? wa : nd;
+    for (uint32_t d = 0; d < ha; ++d) dw[d] = next_word();
+    dw += ha;
+    nd -= ha;
+    typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+    typedef v4u __attribute__((address_space(1)))* gptr_q;
+    gptr_q dq = (gptr_q)dw;
+    const uint32_t nq = nd >> 2;
+    for (uint32_t d = 0; d < nq; ++d) {
+        v4u v;
+        v.x = next_word();
+        v.y = next_word();
+        v.z = next_word();
+        v.w = next_word();
+        dq[d] = v;
     }
-    i += nd << 2;
+    dw += nq << 2;
+    for (uint32_t d = 0; d < (nd & 3u); ++d) dw[d] = next_word();
+    i += (ha + nd) << 2;
     for (; i < cnt; ++i) dst[i] = (uint8_t)next_sym();
 }
 
