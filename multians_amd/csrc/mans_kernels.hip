@@ -295,21 +295,24 @@ __device__ uint32_t walk_rounds(const StreamPlan& sp, const Work& w, const LdsTa
 // the walk reaches that point.  Walks of one workgroup run in lockstep rounds, so when
 // several walks pass through one chunk the one that started further back writes last;
 // that walk's start is the exit recorded for its predecessor chunk, so the records of
-// a range stay consistent.  Returns 1 if the walk merged.
-// nr_max (uniform over the workgroup) bounds the rounds of every lane's walk.
+// a range stay consistent.
+// nr_max (uniform over the workgroup) bounds the rounds of every lane's walk.  Once all
+// walks have left their first chunk, the ones still running are compacted into the lowest
+// lanes every four rounds (their state moves through LDS), so a long tail of a few walks
+// keeps one wave issuing instead of four.
 // With mrg (LDS, one word per lane, zeroed by the caller) the lanes publish when their
 // walk merges, and a walk leaves its first chunk only if the previous lane's walk merged
 // inside its own chunk: otherwise its start (the previous chunk's spec exit) is not on
 // the true path, and continuing would overwrite good records with a wrong path (the
 // previous lane's continuing walk covers those chunks instead).
-__device__ uint32_t walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& tab,
-                               RoundIO& io, uint32_t want, uint32_t c, uint32_t c_end,
-                               uint32_t entry, uint32_t nr_max, uint32_t* mrg = nullptr) {
+__device__ void walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& tab,
+                           RoundIO& io, uint32_t want, uint32_t c, uint32_t c_end,
+                           uint32_t entry, uint32_t nr_max, uint32_t* mrg = nullptr) {
+    __shared__ uint32_t s_wany[kWG / 64], s_wcnt[kWG / 64];
     uint32_t gc = sp.chunk_base + c;       // chunk the walk is in
-    const uint32_t gc0 = gc;
+    uint32_t gc0 = gc;                     // first chunk (kNone once lanes are compacted)
     RLane L;
     L.active = 0u;
-    uint32_t merged = 0u;
     if (want) {
         const uint32_t u0 = c * sp.S;
         L.u_stop = min(c_end * sp.S, sp.n_units);
@@ -319,7 +322,6 @@ __device__ uint32_t walk_chain(const StreamPlan& sp, const Work& w, const LdsTab
         if (ld_coherent(&w.ck[ck_slot(sp, q0)]) == entry) {
             w.mq[gc] = q0;
             w.wcm[gc] = 0u;
-            merged = 1u;
             if (mrg) mrg[threadIdx.x] = 1u;
         } else {
             w.wb[ck_slot(sp, q0)] = entry;
@@ -333,39 +335,95 @@ __device__ uint32_t walk_chain(const StreamPlan& sp, const Work& w, const LdsTab
     }
     io.rbase[threadIdx.x] = L.active ? L.rbase : kNone;
     const uint32_t smask = sp.S - 1u;
-    run_rounds(tab, io, L, nr_max,
-        [&](RLane& l, uint32_t re) {
-            const uint32_t v = pack_entry(l.xi, l.o);
-            if ((re & smask) == 0u) {      // chunk boundary: close this chunk
-                w.mq[gc] = kNone;
-                w.wexit[gc] = v;
-                w.wtotal[gc] = l.cnt;
-                if (mrg && gc == gc0 && threadIdx.x > 0u && !mrg[threadIdx.x - 1u]) {
-                    l.active = 0u;         // start not proven: stop here
-                    return;
+    const uint32_t r_first = sp.S / kR;    // rounds of the first chunk
+    const uint32_t wave = threadIdx.x >> 6;
+    __syncthreads();                       // rbase[] visible
+    Prefetch pf;
+    round_load(io, 0u, pf);
+    for (uint32_t r = 0; r < nr_max; ++r) {
+        __syncthreads();                   // previous round's reads of buf done
+        round_store(io, pf);
+        __syncthreads();
+        if (r + 1u < nr_max) round_load(io, r + 1u, pf);
+        round_decode(tab, io.buf + threadIdx.x * kLS, L, r);
+        if (L.active) {
+            const uint32_t re = L.rbase + (r + 1u) * kR;
+            if (re < L.u_stop) {
+                if (L.u == re) {
+                    const uint32_t v = pack_entry(L.xi, L.o);
+                    bool go = true;
+                    if ((re & smask) == 0u) {      // chunk boundary: close this chunk
+                        w.mq[gc] = kNone;
+                        w.wexit[gc] = v;
+                        w.wtotal[gc] = L.cnt;
+                        if (mrg && gc == gc0 && threadIdx.x > 0u && !mrg[threadIdx.x - 1u]) {
+                            L.active = 0u;         // start not proven: stop here
+                            go = false;
+                        } else {
+                            ++gc;                  // enter the next
+                            w.wentry[gc] = v;
+                            L.cnt = 0u;
+                        }
+                    }
+                    if (go) {
+                        const uint32_t q = re >> sp.log2K;
+                        if (ld_coherent(&w.ck[ck_slot(sp, q)]) == v) {
+                            w.mq[gc] = q;
+                            w.wcm[gc] = L.cnt;
+                            if (mrg && gc == gc0) mrg[threadIdx.x] = 1u;
+                            L.active = 0u;
+                        } else {
+                            w.wb[ck_slot(sp, q)] = v;
+                            w.wbc[ck_slot(sp, q)] = L.cnt;
+                        }
+                    }
                 }
-                ++gc;                      // enter the next
-                w.wentry[gc] = v;
-                l.cnt = 0u;
+            } else {                       // end of the range without a merge
+                L.active = 0u;
+                w.mq[gc] = kNone;
+                w.wexit[gc] = pack_entry(L.xi, L.o);
+                w.wtotal[gc] = L.cnt;
             }
-            const uint32_t q = re >> sp.log2K;
-            if (ld_coherent(&w.ck[ck_slot(sp, q)]) == v) {
-                w.mq[gc] = q;
-                w.wcm[gc] = l.cnt;
-                merged = 1u;
-                if (mrg && gc == gc0) mrg[threadIdx.x] = 1u;
-                l.active = 0u;
-                return;
-            }
-            w.wb[ck_slot(sp, q)] = v;
-            w.wbc[ck_slot(sp, q)] = l.cnt;
-        },
-        [&](RLane& l) {                    // end of the range without a merge
-            w.mq[gc] = kNone;
-            w.wexit[gc] = pack_entry(l.xi, l.o);
-            w.wtotal[gc] = l.cnt;
-        });
-    return merged;
+            if (!L.active) io.rbase[threadIdx.x] = kNone;
+        }
+        const uint32_t n_act = (uint32_t)__syncthreads_count(L.active);
+        if (n_act == 0u) break;
+        // Compaction: once every walk has left its first chunk, the walks still running
+        // are moved to the lowest lanes every few rounds, so that a long walk tail
+        // keeps as few waves issuing as its number of walks needs.
+        if (r + 1u < r_first || ((r + 1u) & 3u) != 0u || r + 1u >= nr_max) continue;
+        const uint64_t bal = __ballot(L.active);
+        if ((threadIdx.x & 63u) == 0u) {
+            s_wany[wave] = bal ? 1u : 0u;
+            s_wcnt[wave] = (uint32_t)__popcll(bal);
+        }
+        __syncthreads();
+        uint32_t waves_busy = 0u, before = 0u;
+        for (uint32_t k = 0; k < kWG / 64u; ++k) {
+            waves_busy += s_wany[k];
+            if (k < wave) before += s_wcnt[k];
+        }
+        if ((n_act + 63u) / 64u >= waves_busy) continue;       // uniform decision
+        uint32_t* x = io.buf;                                    // round data is dead here
+        if (L.active) {
+            const uint32_t pos = before + __builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            uint32_t* d = x + pos * 8u;
+            d[0] = L.u; d[1] = L.o; d[2] = L.xi; d[3] = L.cnt;
+            d[4] = L.u_stop; d[5] = L.rbase; d[6] = gc;
+        }
+        __syncthreads();
+        L.active = threadIdx.x < n_act ? 1u : 0u;
+        if (L.active) {
+            const uint32_t* d = x + threadIdx.x * 8u;
+            L.u = d[0]; L.o = d[1]; L.xi = d[2]; L.cnt = d[3];
+            L.u_stop = d[4]; L.rbase = d[5]; gc = d[6];
+        }
+        gc0 = kNone;                       // every walk is past its first chunk
+        io.rbase[threadIdx.x] = L.active ? L.rbase : kNone;
+        __syncthreads();                   // new rbase[] visible, exchange reads done
+        round_load(io, r + 1u, pf);        // reload the next round for the new lanes
+    }
 }
 
 // Spec pass for 256 consecutive chunks of one stream, then the walks into chunks
@@ -436,7 +494,7 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
     const uint32_t nr = c0 + 1u < c_end ? ((c_end - c0 - 1u) * sp.S + kR - 1u) / kR : 0u;
     __shared__ uint32_t s_mrg[kWG];
     s_mrg[threadIdx.x] = 0u;               // visible after run_rounds' first barrier
-    (void)walk_chain(sp, w, tab, io, want, tgt, c_end, my_exit, nr, s_mrg);
+    walk_chain(sp, w, tab, io, want, tgt, c_end, my_exit, nr, s_mrg);
 }
 
 // Walks into the first chunk of each workgroup range (chunks c0 = 256k, k >= 1), from
@@ -462,7 +520,7 @@ __global__ __launch_bounds__(kWG) void k_seam(KArgs a) {
     // it; k_check finds any chunk whose walk started from a different exit)
     const uint32_t entry = want ? chunk_exit(sp, w, tgt - 1u) : 0u;
     const uint32_t c_end = want ? min(tgt + (uint32_t)kWG, sp.n_chunks) : tgt;
-    (void)walk_chain(sp, w, tab, io, want, tgt, c_end, entry, (kWG * sp.S) / kR);
+    walk_chain(sp, w, tab, io, want, tgt, c_end, entry, (kWG * sp.S) / kR);
 }
 
 // Consistency check over all chunks (grid as k_sync): a chunk whose recorded walk start
