@@ -18,7 +18,7 @@ on this host.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--streams N] [--no-single]
                     [--no-plain] [--no-cpu-baseline] [--calibrate-copy MIB]
-                    [--serve-seconds S] [--serve-pool N]
+                    [--serve-seconds S] [--serve-pool N] [--stress]
 """
 import argparse
 import ctypes as C
@@ -178,6 +178,42 @@ def bench_single(args, torch):
     return per, items
 
 
+def bench_stress(args, torch):
+    """Stress configuration: one stream of 2^32 uint8 symbols (4 GiB, 64-bit offsets) for
+    lambda = 0.002 (symbol 255 holds 60% of the mass) and lambda = 3.0 (ratio > 7), 4096
+    states; bit-exact on the device, then timed."""
+    from multians_amd import _lib
+    from multians_amd.decoder import DeviceStream, EncodedStream
+    n = 1 << 32
+    out = []
+    for i, lam in enumerate([0.002, 3.0]):
+        t0 = time.time()
+        sym = _lib.gen_symbols(4000 + i, lam, n, threads=THREADS)
+        counts = _lib.normalize_counts(_lib.histogram(sym, threads=THREADS), 4096)
+        enc, dec = _lib.build_tables(counts, 4096)
+        units, fb, fs = _lib.encode(sym, enc, 4096)
+        del enc
+        es = EncodedStream(units, fb, fs, dec, 4096, n, counts)
+        log("stress lambda=%g: %d units (ratio %.2f), prepared in %.0fs" %
+            (lam, es.n_units, n / (4.0 * es.n_units), time.time() - t0))
+        ds = DeviceStream(es)
+        o = ds.decode()
+        torch.cuda.synchronize()
+        ref = torch.from_numpy(sym).cuda()
+        del sym
+        ok = ds.status() == 0 and bool(torch.equal(o[:n], ref.flip(0)))
+        del ref
+        if not ok:
+            raise SystemExit("stress decode mismatch for lambda=%g" % lam)
+        ms = time_events(torch, ds.decode, 3)
+        out.append({"lambda": lam, "symbols": n, "ratio": round(n / (4.0 * es.n_units), 3),
+                    "ms": round(ms, 3), "decoded_GBps": round(n / ms / 1e6, 2), "bit_exact": True})
+        log("stress", json.dumps(out[-1]))
+        del ds, o, es, units
+        torch.cuda.empty_cache()
+    return out
+
+
 def bench_served(args, torch):
     """Served configuration: open-loop Poisson arrivals at 70% of the measured batched
     capacity on the request mix (L 1024/4096 50/50, uint8 sizes log-uniform 1 KiB-64 MiB,
@@ -290,6 +326,8 @@ def main():
                     help="duration of the served run (0 = skip)")
     ap.add_argument("--serve-pool", type=int, default=1000,
                     help="distinct streams the served requests draw from")
+    ap.add_argument("--stress", action="store_true",
+                    help="also decode two 2^32-symbol streams (lambda 0.002 and 3.0)")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the per-stream plain-launch leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -345,6 +383,8 @@ def main():
     if not args.no_single:
         per, _ = bench_single(args, torch)
         res["single_stream"] = per
+    if args.stress:
+        res["stress"] = bench_stress(args, torch)
     if args.serve_seconds > 0:
         sv = bench_served(args, torch)
         res["served"] = sv

@@ -192,6 +192,10 @@ int mans_chunk_units(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t* chu
 /* --------------------------------------------------------- host-side codec */
 /* Symbols s = min(255, floor(X)), X ~ Exp(lambda); element i depends only on (seed, i). */
 int mans_gen_symbols(uint64_t seed, double lambda, uint8_t* out, uint64_t n, int threads);
+/* hist[256] = occurrences of each byte value in in[0..n) (threads > 1: parallel).
+ * Replaces the counting step of ANSTableGenerator::generate_distribution_from_buffer
+ * (src/ans_table_generator.cc:61-95). */
+int mans_histogram(const uint8_t* in, uint64_t n, uint64_t* hist, int threads);
 /* Histogram -> counts summing to L (absent -> 0, present -> >= 1). */
 int mans_normalize_counts(const uint64_t* hist, uint32_t nsym, uint32_t L, uint32_t* counts);
 /* Encoder table (3 u32 per (symbol, state): next_state, code, length; symbol-major)

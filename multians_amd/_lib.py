@@ -75,7 +75,7 @@ EXPORTS = [
     "mans_batch_workspace_bytes", "mans_decode_batch", "mans_gen_symbols",
     "mans_normalize_counts", "mans_build_tables", "mans_max_encoded_units", "mans_encode",
     "mans_make_stream", "mans_batch_graph_create", "mans_graph_launch", "mans_graph_destroy",
-    "mans_set_geometry_rules", "mans_chunk_units",
+    "mans_set_geometry_rules", "mans_chunk_units", "mans_histogram",
 ]
 
 _lib = None
@@ -130,6 +130,7 @@ def lib():
         "mans_graph_destroy": ([vp], i32),
         "mans_set_geometry_rules": ([C.POINTER(GeometryRule), C.c_size_t], i32),
         "mans_chunk_units": ([u64, u64, u32, C.POINTER(u32)], i32),
+        "mans_histogram": ([vp, u64, vp, i32], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(l, name)
@@ -155,6 +156,13 @@ def gen_symbols(seed, lam, n, threads=8):
     out = np.empty(n, np.uint8)
     check(lib().mans_gen_symbols(seed, lam, _ptr(out), n, threads))
     return out
+
+
+def histogram(symbols, threads=8):
+    symbols = np.ascontiguousarray(symbols, np.uint8)
+    h = np.zeros(256, np.uint64)
+    check(lib().mans_histogram(_ptr(symbols), len(symbols), _ptr(h), threads))
+    return h
 
 
 def normalize_counts(hist, L):

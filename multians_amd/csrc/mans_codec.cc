@@ -6,6 +6,8 @@
 //                       get_decoder_table (src/ans_table_generator.cc:164-288)
 //   mans_encode         ANSEncoder::encode (src/ans_encoder.cc:13-97) + the unit
 //                       reversal of CUHDInputBuffer (src/multians_input_buffer.cc:29)
+//   mans_histogram      symbol counts of a buffer (the counting step of
+//                       ANSTableGenerator::generate_distribution_from_buffer)
 //   mans_normalize_counts  counts from a histogram (absent symbols get 0, present >= 1)
 //   mans_gen_symbols    s = min(255, floor(X)), X ~ Exp(lambda), counter-based
 #include <algorithm>
@@ -81,6 +83,29 @@ int mans_gen_symbols(uint64_t seed, double lambda, uint8_t* out, uint64_t n, int
         th.emplace_back(gen_range, seed, &t, out, b, e);
     }
     for (auto& x : th) x.join();
+    return MANS_OK;
+}
+
+int mans_histogram(const uint8_t* in, uint64_t n, uint64_t* hist, int threads) {
+    if ((!in && n) || !hist) return set_error(MANS_EINVAL, "NULL argument");
+    if (threads < 1) threads = 1;
+    if (n < (1u << 22)) threads = 1;
+    std::vector<std::vector<uint64_t>> part(threads, std::vector<uint64_t>(256, 0));
+    auto run = [&](int t) {
+        const uint64_t per = (n + threads - 1) / threads;
+        const uint64_t b = std::min<uint64_t>(n, per * t), e = std::min<uint64_t>(n, b + per);
+        uint64_t* h = part[t].data();
+        for (uint64_t i = b; i < e; ++i) ++h[in[i]];
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < threads; ++t) th.emplace_back(run, t);
+    run(0);
+    for (auto& x : th) x.join();
+    for (int s = 0; s < 256; ++s) {
+        uint64_t v = 0;
+        for (int t = 0; t < threads; ++t) v += part[t][s];
+        hist[s] = v;
+    }
     return MANS_OK;
 }
 

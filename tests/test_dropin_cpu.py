@@ -145,3 +145,13 @@ def test_tuner_rule_table_matches_builtin_choice():
     finally:
         _lib.set_geometry_rules([])
     assert tuned == builtin
+
+
+def test_histogram_matches_bincount():
+    from multians_amd import _lib
+    rng = np.random.default_rng(8)
+    for n in (0, 1, 1000, (1 << 22) + 77):
+        sym = rng.integers(0, 256, n).astype(np.uint8)
+        ref = np.bincount(sym, minlength=256).astype(np.uint64)
+        for t in (1, 5):
+            assert np.array_equal(_lib.histogram(sym, threads=t), ref)
