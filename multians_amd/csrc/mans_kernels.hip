@@ -35,10 +35,17 @@ struct LdsTab {
 struct KArgs {
     StreamPlan one;                 // used when plans == nullptr
     const StreamPlan* plans;        // batch
+    const uint32_t* order;          // batch: chunk tasks, longest expected first
     uint32_t n_streams;
     uint32_t n_tasks;
     Work w;
 };
+
+// Chunk task of workgroup b: batches run their tasks longest-expected first, so the
+// long ones do not start last and hold the end of the kernel.
+__device__ __forceinline__ uint32_t chunk_task(const KArgs& a, uint32_t b) {
+    return a.order ? a.order[b] : b;
+}
 
 __device__ __forceinline__ const StreamPlan* plan_of(const KArgs& a, uint32_t s) {
     return a.plans ? &a.plans[s] : &a.one;
@@ -432,7 +439,7 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_stream;
     __shared__ uint32_t s_rbase[kWG];
-    const uint32_t t = blockIdx.x;
+    const uint32_t t = chunk_task(a, blockIdx.x);
     if (threadIdx.x == 0) s_stream = stream_of_task<false>(a, t);
     __syncthreads();
     const StreamPlan sp = *plan_of(a, __builtin_amdgcn_readfirstlane(s_stream));
@@ -503,7 +510,7 @@ __global__ __launch_bounds__(kWG) void k_seam(KArgs a) {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_stream;
     __shared__ uint32_t s_rbase[kWG];
-    const uint32_t t = blockIdx.x;
+    const uint32_t t = chunk_task(a, blockIdx.x);
     if (threadIdx.x == 0) s_stream = stream_of_task<false>(a, t);
     __syncthreads();
     const StreamPlan sp = *plan_of(a, __builtin_amdgcn_readfirstlane(s_stream));
@@ -527,7 +534,7 @@ __global__ __launch_bounds__(kWG) void k_seam(KArgs a) {
 // differs from its predecessor's exit goes on its stream's fixup list.  Clears fflag.
 __global__ __launch_bounds__(kWG) void k_check(KArgs a) {
     __shared__ uint32_t s_stream;
-    const uint32_t t = blockIdx.x;
+    const uint32_t t = chunk_task(a, blockIdx.x);
     if (threadIdx.x == 0) s_stream = stream_of_task<false>(a, t);
     __syncthreads();
     const uint32_t sid = __builtin_amdgcn_readfirstlane(s_stream);
@@ -977,7 +984,8 @@ Layout make_layout(uint64_t chunks, uint64_t cks, uint64_t n_plans) {
     l.o_tick = take(4 * 8);
     l.o_bsum = take(8 * (chunks / 2048 + 2));
     l.o_fcnt = take(4 * (n_plans + 1));
-    l.o_plans = take(sizeof(StreamPlan) * n_plans);
+    // plans, then the chunk-task order (at most chunks/256 + n_plans tasks)
+    l.o_plans = take(sizeof(StreamPlan) * n_plans + 4 * (chunks / 256 + n_plans + 1));
     l.bytes = o;
     return l;
 }
@@ -1213,6 +1221,32 @@ static int build_plans(const mans_stream_desc* d, size_t n, std::vector<StreamPl
     return MANS_OK;
 }
 
+// Plans plus the chunk-task order (longest expected first) as one host image for the
+// workspace's plan region.  A task's expected length is its chunks' serial decode plus a
+// merge tail, in steps: (S + 2048) * symbols per unit.
+static std::vector<char> plan_image(const std::vector<StreamPlan>& plans, uint32_t n_tasks) {
+    const size_t n = plans.size();
+    std::vector<std::pair<double, uint32_t>> cost;
+    cost.reserve(n_tasks);
+    for (size_t i = 0; i < n; ++i) {
+        const StreamPlan& p = plans[i];
+        const uint32_t nt = i + 1 < n ? plans[i + 1].task_chunk - p.task_chunk
+                                      : n_tasks - p.task_chunk;
+        const double spu = p.n_units ? (double)p.n_out / (double)p.n_units : 0.0;
+        const double c = ((double)p.S + 2048.0) * spu;
+        for (uint32_t k = 0; k < nt; ++k) cost.push_back({c, p.task_chunk + k});
+    }
+    std::stable_sort(cost.begin(), cost.end(),
+                     [](const std::pair<double, uint32_t>& x, const std::pair<double, uint32_t>& y) {
+                         return x.first > y.first;
+                     });
+    std::vector<char> img(sizeof(StreamPlan) * n + 4 * (size_t)n_tasks);
+    std::memcpy(img.data(), plans.data(), sizeof(StreamPlan) * n);
+    uint32_t* ord = (uint32_t*)(img.data() + sizeof(StreamPlan) * n);
+    for (uint32_t k = 0; k < n_tasks; ++k) ord[k] = cost[k].second;
+    return img;
+}
+
 }  // namespace mans
 
 extern "C" int mans_batch_workspace_bytes(const mans_stream_desc* descs, size_t n, uint64_t* bytes) {
@@ -1240,13 +1274,14 @@ extern "C" int mans_decode_batch(const mans_stream_desc* descs, size_t n,
                          (unsigned long long)opts->workspace_bytes, (unsigned long long)l.bytes);
     hipStream_t st = (hipStream_t)opts->stream;
     StreamPlan* dplans = (StreamPlan*)((char*)opts->workspace + l.o_plans);
-    hipError_t e = hipMemcpyAsync(dplans, plans.data(), sizeof(StreamPlan) * n,
-                                  hipMemcpyHostToDevice, st);
+    const std::vector<char> img = plan_image(plans, ct);
+    hipError_t e = hipMemcpyAsync(dplans, img.data(), img.size(), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return set_error(MANS_EHIP, "plan upload: %s", hipGetErrorString(e));
     KArgs a;
     std::memset(&a, 0, sizeof(a));
     a.plans = dplans;
+    a.order = (const uint32_t*)(dplans + n);
     a.n_streams = (uint32_t)n;
     a.n_tasks = ct;
     a.w = make_work(opts->workspace, l);
@@ -1285,11 +1320,13 @@ extern "C" int mans_batch_graph_create(const mans_stream_desc* descs, size_t n,
         return set_error(MANS_ENOMEM, "batch workspace of %llu bytes < %llu needed",
                          (unsigned long long)opts->workspace_bytes, (unsigned long long)l.bytes);
     StreamPlan* dplans = (StreamPlan*)((char*)opts->workspace + l.o_plans);
-    hipError_t e = hipMemcpy(dplans, plans.data(), sizeof(StreamPlan) * n, hipMemcpyHostToDevice);
+    const std::vector<char> img = plan_image(plans, ct);
+    hipError_t e = hipMemcpy(dplans, img.data(), img.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return set_error(MANS_EHIP, "plan upload: %s", hipGetErrorString(e));
     KArgs a;
     std::memset(&a, 0, sizeof(a));
     a.plans = dplans;
+    a.order = (const uint32_t*)(dplans + n);
     a.n_streams = (uint32_t)n;
     a.n_tasks = ct;
     a.w = make_work(opts->workspace, l);
