@@ -36,6 +36,7 @@ struct KArgs {
     StreamPlan one;                 // used when plans == nullptr
     const StreamPlan* plans;        // batch
     const uint32_t* order;          // batch: chunk tasks, longest expected first
+    const uint32_t* sorder;         // batch: streams, longest expected first
     uint32_t n_streams;
     uint32_t n_tasks;
     Work w;
@@ -561,7 +562,8 @@ __global__ __launch_bounds__(kWG) void k_fixup(KArgs a) {
     __shared__ uint32_t s_n, s_n2;
     __shared__ uint32_t s_rbase[kWG];
     const Work& w = a.w;
-    for (uint32_t sid = blockIdx.x; sid < a.n_streams; sid += gridDim.x) {
+    for (uint32_t si = blockIdx.x; si < a.n_streams; si += gridDim.x) {
+        const uint32_t sid = a.sorder ? a.sorder[si] : si;
         __syncthreads();
         if (threadIdx.x == 0) s_n = ld_coherent(&w.fcnt[sid]);
         __syncthreads();
@@ -823,24 +825,34 @@ __device__ __forceinline__ void item_write(const LdsTab& tab, const uint32_t* lb
 }
 
 __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint32_t per_wg) {
+    // Persistent workgroups take blocks of per_wg consecutive passes from a device
+    // counter (ctick[0], zeroed by k_init), so slow passes (high symbols per unit) do
+    // not pile up on a few workgroups.
     extern __shared__ uint32_t lds[];
-    __shared__ uint32_t s_stream, s_have;
+    __shared__ uint32_t s_stream, s_have, s_next;
     __shared__ uint64_t s_off[kWG + 1];
-    const uint32_t p0 = blockIdx.x * per_wg;
-    const uint32_t p1 = min(n_passes, p0 + per_wg);
-    if (p0 >= p1) return;
-    if (threadIdx.x == 0) s_have = kNone;
+    if (threadIdx.x == 0) {
+        s_have = kNone;
+        s_next = atomicAdd(&a.w.ctick[0], per_wg);
+    }
+    __syncthreads();
+    uint32_t t = s_next;
+    if (t >= n_passes) return;
+    uint32_t t_end = min(n_passes, t + per_wg);
     // prefetch the first pass
-    if (threadIdx.x == 0) s_stream = stream_of_task<true>(a, p0);
+    if (threadIdx.x == 0) s_stream = stream_of_task<true>(a, t);
     __syncthreads();
     WPrefetch pf;
     {
         const StreamPlan sp = *plan_of(a, __builtin_amdgcn_readfirstlane(s_stream));
-        pass_load(sp.units, sp.n_units, (p0 - sp.task_item) * kWG * kWI, pf);
+        pass_load(sp.units, sp.n_units, (t - sp.task_item) * kWG * kWI, pf);
     }
-    for (uint32_t t = p0; t < p1; ++t) {
+    while (t < n_passes) {
         __syncthreads();
-        if (threadIdx.x == 0) s_stream = stream_of_task<true>(a, t);
+        if (threadIdx.x == 0) {
+            s_stream = stream_of_task<true>(a, t);
+            if (t + 1u == t_end) s_next = atomicAdd(&a.w.ctick[0], per_wg);
+        }
         __syncthreads();
         const uint32_t sid = __builtin_amdgcn_readfirstlane(s_stream);
         const StreamPlan sp = *plan_of(a, sid);
@@ -862,11 +874,12 @@ __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint3
             s_off[nitems] = qe < sp.n_ck ? checkpoint_offset(sp, a.w, qe, &e2) : sp.n_out;
         }
         __syncthreads();
-        if (t + 1u < p1) {
-            // the next pass may belong to the next stream
-            const uint32_t ns = stream_of_task<true>(a, t + 1u);
+        const uint32_t tn = t + 1u < t_end ? t + 1u : __builtin_amdgcn_readfirstlane(s_next);
+        if (tn < n_passes) {
+            // the next pass may belong to another stream
+            const uint32_t ns = stream_of_task<true>(a, tn);
             const StreamPlan np = *plan_of(a, __builtin_amdgcn_readfirstlane(ns));
-            pass_load(np.units, np.n_units, (t + 1u - np.task_item) * kWG * kWI, pf);
+            pass_load(np.units, np.n_units, (tn - np.task_item) * kWG * kWI, pf);
         }
         if (threadIdx.x < nitems) {
             uint64_t off = s_off[threadIdx.x], end = s_off[threadIdx.x + 1];
@@ -876,6 +889,8 @@ __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint3
                 item_write(tab, buf + threadIdx.x * kWL, entry, sp.out + off, (uint32_t)(end - off));
             }
         }
+        if (t + 1u == t_end) t_end = min(n_passes, tn + per_wg);
+        t = tn;
     }
 }
 
@@ -985,7 +1000,7 @@ Layout make_layout(uint64_t chunks, uint64_t cks, uint64_t n_plans) {
     l.o_bsum = take(8 * (chunks / 2048 + 2));
     l.o_fcnt = take(4 * (n_plans + 1));
     // plans, then the chunk-task order (at most chunks/256 + n_plans tasks)
-    l.o_plans = take(sizeof(StreamPlan) * n_plans + 4 * (chunks / 256 + n_plans + 1));
+    l.o_plans = take(sizeof(StreamPlan) * n_plans + 4 * (chunks / 256 + 2 * n_plans + 1));
     l.bytes = o;
     return l;
 }
@@ -1041,10 +1056,9 @@ static int launch_pipeline(const KArgs& a, uint32_t n_chunk_tasks, uint32_t n_it
     }
     mark(5);
     {
-        // persistent write pass: about 8 workgroups per CU, contiguous pass ranges
-        uint32_t nwg = std::min<uint32_t>(n_item_tasks, 2048u);
-        uint32_t per = (n_item_tasks + nwg - 1) / nwg;
-        nwg = (n_item_tasks + per - 1) / per;
+        // persistent write pass: workgroups take blocks of 4 passes from a device counter
+        const uint32_t per = 4u;
+        uint32_t nwg = std::min<uint32_t>((n_item_tasks + per - 1) / per, 2048u);
         (void)maxK;
         const size_t lds_w = lds + (size_t)kWG * kWL * 4u;
         hipLaunchKernelGGL(k_write, dim3(nwg), dim3(kWG), lds_w, st, a, n_item_tasks, per);
@@ -1240,10 +1254,22 @@ static std::vector<char> plan_image(const std::vector<StreamPlan>& plans, uint32
                      [](const std::pair<double, uint32_t>& x, const std::pair<double, uint32_t>& y) {
                          return x.first > y.first;
                      });
-    std::vector<char> img(sizeof(StreamPlan) * n + 4 * (size_t)n_tasks);
+    std::vector<char> img(sizeof(StreamPlan) * n + 4 * ((size_t)n_tasks + n));
     std::memcpy(img.data(), plans.data(), sizeof(StreamPlan) * n);
     uint32_t* ord = (uint32_t*)(img.data() + sizeof(StreamPlan) * n);
     for (uint32_t k = 0; k < n_tasks; ++k) ord[k] = cost[k].second;
+    // streams in the same order (for the fixup kernel)
+    std::vector<std::pair<double, uint32_t>> sc(n);
+    for (size_t i = 0; i < n; ++i) {
+        const StreamPlan& p = plans[i];
+        const double spu = p.n_units ? (double)p.n_out / (double)p.n_units : 0.0;
+        sc[i] = {((double)p.S + 2048.0) * spu, (uint32_t)i};
+    }
+    std::stable_sort(sc.begin(), sc.end(),
+                     [](const std::pair<double, uint32_t>& x, const std::pair<double, uint32_t>& y) {
+                         return x.first > y.first;
+                     });
+    for (size_t i = 0; i < n; ++i) ord[n_tasks + i] = sc[i].second;
     return img;
 }
 
@@ -1282,6 +1308,7 @@ extern "C" int mans_decode_batch(const mans_stream_desc* descs, size_t n,
     std::memset(&a, 0, sizeof(a));
     a.plans = dplans;
     a.order = (const uint32_t*)(dplans + n);
+    a.sorder = a.order + ct;
     a.n_streams = (uint32_t)n;
     a.n_tasks = ct;
     a.w = make_work(opts->workspace, l);
@@ -1327,6 +1354,7 @@ extern "C" int mans_batch_graph_create(const mans_stream_desc* descs, size_t n,
     std::memset(&a, 0, sizeof(a));
     a.plans = dplans;
     a.order = (const uint32_t*)(dplans + n);
+    a.sorder = a.order + ct;
     a.n_streams = (uint32_t)n;
     a.n_tasks = ct;
     a.w = make_work(opts->workspace, l);
