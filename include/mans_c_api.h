@@ -106,7 +106,7 @@ typedef struct mans_decode_args {
                                    `stream` around the kernels (timing), or NULL */
 } mans_decode_args;
 
-/* kernel_events[i] is recorded before kernel i (0 init, 1 sync, 2 seam, 3 fixup,
+/* kernel_events[i] is recorded before kernel i (0 init, 1 sync, 2 check, 3 fixup,
  * 4 scan, 5 write); kernel_events[6] after the last one. */
 #define MANS_N_KERNEL_EVENTS 7
 
@@ -186,6 +186,13 @@ typedef struct mans_geometry_rule {
     uint64_t units_max;
 } mans_geometry_rule;
 int mans_set_geometry_rules(const mans_geometry_rule* rules, size_t n);
+/* Polls of the sync kernel's wait for the previous range's last exit (a range of 256
+ * chunks starts its first merge walk there).  A wait that runs out leaves that chunk to
+ * the fixup rounds (the result is the same; status word 7 counts such waits).  Applies
+ * to decodes enqueued (or graphs captured) afterwards.  No counterpart in the reference
+ * (its phase-2 kernel re-launches from the host instead of waiting). */
+#define MANS_WAIT_POLLS_DEFAULT (1u << 20)
+int mans_set_wait_polls(uint32_t polls);
 /* The chunk length the decoder picks for a stream (rules, else built-in). */
 int mans_chunk_units(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t* chunk_units);
 

@@ -37,7 +37,7 @@ class DecodeArgs(C.Structure):
 
 
 N_KERNEL_EVENTS = 7
-KERNEL_NAMES = ["init", "sync", "seam", "fixup", "scan", "write"]
+KERNEL_NAMES = ["init", "sync", "check", "fixup", "scan", "write"]
 
 
 class StreamDesc(C.Structure):
@@ -75,7 +75,7 @@ EXPORTS = [
     "mans_batch_workspace_bytes", "mans_decode_batch", "mans_gen_symbols",
     "mans_normalize_counts", "mans_build_tables", "mans_max_encoded_units", "mans_encode",
     "mans_make_stream", "mans_batch_graph_create", "mans_graph_launch", "mans_graph_destroy",
-    "mans_set_geometry_rules", "mans_chunk_units", "mans_histogram",
+    "mans_set_geometry_rules", "mans_chunk_units", "mans_histogram", "mans_set_wait_polls",
 ]
 
 _lib = None
@@ -131,6 +131,7 @@ def lib():
         "mans_set_geometry_rules": ([C.POINTER(GeometryRule), C.c_size_t], i32),
         "mans_chunk_units": ([u64, u64, u32, C.POINTER(u32)], i32),
         "mans_histogram": ([vp, u64, vp, i32], i32),
+        "mans_set_wait_polls": ([u32], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(l, name)
@@ -238,6 +239,14 @@ def set_geometry_rules(rules):
         arr[i].L_max, arr[i].spu_max = int(L_max), float(spu_max)
         arr[i].units_max, arr[i].chunk_units = int(units_max), int(chunk)
     check(lib().mans_set_geometry_rules(arr, len(rules)))
+
+
+WAIT_POLLS_DEFAULT = 1 << 20
+
+
+def set_wait_polls(polls=WAIT_POLLS_DEFAULT):
+    """Polls of the sync kernel's wait for the previous range's exit (see mans_c_api.h)."""
+    check(lib().mans_set_wait_polls(int(polls)))
 
 
 def chunk_units(n_units, n_out, L):

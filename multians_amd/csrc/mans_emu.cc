@@ -72,27 +72,24 @@ int mans_emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint6
     } else {
         for (uint32_t c = 0; c < nc; ++c) lane_spec(sp, w, ht, c);
     }
-    // k_sync walks: in a range of R = 256 chunks, lane c walks from its exit into c+1 and
-    // on to the range end until it merges.  Lanes run in lockstep rounds on the GPU, so
-    // where walks overlap the one that started further back writes last; running the
-    // lanes from the back of the range to its front reproduces that.
-    // A walk leaves its first chunk only if the previous lane's walk merged inside its
-    // own chunk (the range's first walk always may).
+    // Walks (k_spec's for a stream of one range, k_walk's otherwise): in a range of R =
+    // 256 chunks, the walk into chunk c0 + t starts from the spec exit of its predecessor
+    // and goes on to the range end until it merges (chunk 0 is not walked).  Lanes run in
+    // lockstep rounds on the GPU, so where walks overlap the one that started further
+    // back writes last; running the lanes from the back of the range to its front
+    // reproduces that.  A walk leaves its first chunk only if the previous
+    // lane's walk merged inside its own chunk (the range's first walk always may, and
+    // chunk 0 counts as merged).  Ranges are independent: a walk never leaves its range.
     const uint32_t R = 256u;
     std::vector<uint8_t> own_merged(nc, 0);
+    own_merged[0] = 1;
     for (uint32_t t = 1; t < nc; ++t) own_merged[t] = (uint8_t)lane_merges_in_chunk(sp, w, ht, t, exitv[t - 1]);
     for (uint32_t c0 = 0; c0 < nc; c0 += R) {
         const uint32_t c_end = c0 + R < nc ? c0 + R : nc;
-        for (uint32_t t = c_end; t-- > c0 + 1u;) {
-            const uint32_t stop = (t >= c0 + 2u && !own_merged[t - 1]) ? 1u : 0u;
+        for (uint32_t t = c_end; t-- > (c0 > 1u ? c0 : 1u);) {
+            const uint32_t stop = (t > c0 && !own_merged[t - 1]) ? 1u : 0u;
             st_merges += lane_walk_chain(sp, w, ht, t, c_end, exitv[t - 1], stop);
         }
-    }
-    // k_seam: the walk into the first chunk of each later range, from the previous
-    // range's exit as the walks left it
-    for (uint32_t c0 = R; c0 < nc; c0 += R) {
-        const uint32_t c_end = c0 + R < nc ? c0 + R : nc;
-        st_merges += lane_walk_chain(sp, w, ht, c0, c_end, chunk_exit(sp, w, c0 - 1u));
     }
     // k_check: every chunk whose walk start differs from its predecessor's exit
     for (uint32_t c = 0; c < nc; ++c) {

@@ -4,15 +4,16 @@
 // Replaces cuhd::CUHDGPUDecoder::decode and its kernels
 // (src/multians_gpu_decoder.cu:195-687 of the reference):
 //   phase1_decode_subseq           -> k_sync   (speculative decode + checkpoints, then
-//                                     merge walks inside the workgroup's range)
-//   phase2_synchronise_blocks+loop -> k_seam + k_fixup (walks across workgroup ranges,
-//                                     fixup rounds; on device, no host round trip)
+//                                     merge walks from each chunk's predecessor exit)
+//   phase2_synchronise_blocks+loop -> k_check + k_fixup (consistency check, fixup
+//                                     rounds; on device, no host round trip)
 //   phase3 copies + thrust scan    -> k_scan_reduce/top/down (u64 exclusive scan)
 //   phase4_decode_write_output     -> k_write  (count-driven decode + store)
 // The per-lane logic lives in mans_core.h.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -39,6 +40,7 @@ struct KArgs {
     const uint32_t* sorder;         // batch: streams, longest expected first
     uint32_t n_streams;
     uint32_t n_tasks;
+    uint32_t wait_polls;             // k_sync: polls of a range's wait for its predecessor
     Work w;
 };
 
@@ -94,11 +96,19 @@ __device__ uint32_t stage_table(const StreamPlan& sp, uint32_t* lds, uint32_t* s
     return __builtin_amdgcn_readfirstlane(f);
 }
 
-__global__ void k_init(Work w, uint32_t n_status, uint32_t n_streams) {
+// Clears the status words, tickets and fixup counters, and marks the last exit of each
+// range that another range waits for (k_sync) as not yet published.
+__global__ void k_init(KArgs a, uint32_t n_status) {
+    const Work& w = a.w;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_status) w.status[i] = 0u;
     if (i < 4u) w.ctick[i] = 0u;
-    if (i < n_streams) w.fcnt[i] = 0u;
+    if (i < a.n_streams) w.fcnt[i] = 0u;
+    if (i < a.n_tasks) {
+        const StreamPlan& sp = *plan_of(a, stream_of_task<false>(a, i));
+        const uint32_t c0 = (i - sp.task_chunk) * kWG;
+        if (c0 > 0u && c0 < sp.n_chunks) w.exitv[sp.chunk_base + c0 - 1u] = kNone;
+    }
 }
 
 // ------------------------------------------------------------ round engine
@@ -434,15 +444,28 @@ __device__ void walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& ta
     }
 }
 
-// Spec pass for 256 consecutive chunks of one stream, then the walks into chunks
-// c0+1 .. c0+255 (their predecessors' exits are this workgroup's own results).
+// Spec pass, then merge walks, for a range of 256 consecutive chunks of one stream.
+// Workgroups take their task from a ticket counter in task order, in which the ranges of
+// a stream are ascending, so a range's predecessor has always started before it.
+// The walk into chunk c0 + l (lane l) starts from the spec exit of chunk c0 + l - 1 and
+// continues until it merges, at most to the end of the range; chunk 0 is the true start
+// and is not walked.  Lane 0's start is the previous range's last exit, published by
+// that range's workgroup as soon as its spec pass ends (k_init set it to kNone): the
+// workgroup waits for it, a bounded number of polls (a wait that runs out skips that
+// walk and marks the chunk's start kNone, so k_check lists it and k_fixup walks it).
 __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
     extern __shared__ uint32_t lds[];
-    __shared__ uint32_t s_stream;
+    __shared__ uint32_t s_task, s_stream, s_entry;
     __shared__ uint32_t s_rbase[kWG];
-    const uint32_t t = chunk_task(a, blockIdx.x);
-    if (threadIdx.x == 0) s_stream = stream_of_task<false>(a, t);
+    __shared__ uint32_t s_exit[kWG];
+    __shared__ uint32_t s_mrg[kWG];
+    if (threadIdx.x == 0) {
+        const uint32_t t = chunk_task(a, atomicAdd(&a.w.ctick[1], 1u));
+        s_task = t;
+        s_stream = stream_of_task<false>(a, t);
+    }
     __syncthreads();
+    const uint32_t t = s_task;
     const StreamPlan sp = *plan_of(a, __builtin_amdgcn_readfirstlane(s_stream));
     const uint32_t cyc = stage_table(sp, lds, a.w.status);
     LdsTab tab{lds};
@@ -462,6 +485,7 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
         return;
     }
     const bool valid = c < sp.n_chunks;
+    const uint32_t c_end = min(c0 + (uint32_t)kWG, sp.n_chunks);
     // ---- spec
     RLane L;
     L.active = valid ? 1u : 0u;
@@ -489,46 +513,39 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
     });
     const uint32_t my_exit = pack_entry(L.xi, L.o);
     if (valid) {
-        w.exitv[sp.chunk_base + c] = my_exit;
+        if (c + 1u == c_end && c_end < sp.n_chunks)    // the next range waits for this one
+            __hip_atomic_store(&w.exitv[sp.chunk_base + c], my_exit, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        else
+            w.exitv[sp.chunk_base + c] = my_exit;
         w.own_total[sp.chunk_base + c] = L.cnt;
+    }
+    s_exit[threadIdx.x] = my_exit;
+    s_mrg[threadIdx.x] = c == 0u ? 1u : 0u;   // the stream start counts as merged
+    if (threadIdx.x == 0) {
+        uint32_t e = 0u;
+        if (c0 > 0u) {
+            const uint32_t* p = &w.exitv[sp.chunk_base + c0 - 1u];
+            for (uint32_t i = 0;; ++i) {
+                e = ld_coherent(p);
+                if (e != kNone || i >= a.wait_polls) break;
+                __builtin_amdgcn_s_sleep(4);
+            }
+            if (e == kNone) atomicAdd(&w.status[7], 1u);    // waits run out (statistics)
+        }
+        s_entry = e;
     }
     __threadfence();
     __syncthreads();
-    // ---- walks from this lane's exit into c+1, continuing until they merge, at most
-    // to the end of this workgroup's range (consistency is checked by k_check)
-    const uint32_t c_end = min(c0 + (uint32_t)kWG, sp.n_chunks);
-    const uint32_t tgt = c + 1u;
-    const uint32_t want = tgt < c_end ? 1u : 0u;
-    const uint32_t nr = c0 + 1u < c_end ? ((c_end - c0 - 1u) * sp.S + kR - 1u) / kR : 0u;
-    __shared__ uint32_t s_mrg[kWG];
-    s_mrg[threadIdx.x] = 0u;               // visible after run_rounds' first barrier
-    walk_chain(sp, w, tab, io, want, tgt, c_end, my_exit, nr, s_mrg);
-}
-
-// Walks into the first chunk of each workgroup range (chunks c0 = 256k, k >= 1), from
-// the exit of the previous workgroup's last chunk.
-__global__ __launch_bounds__(kWG) void k_seam(KArgs a) {
-    extern __shared__ uint32_t lds[];
-    __shared__ uint32_t s_stream;
-    __shared__ uint32_t s_rbase[kWG];
-    const uint32_t t = chunk_task(a, blockIdx.x);
-    if (threadIdx.x == 0) s_stream = stream_of_task<false>(a, t);
-    __syncthreads();
-    const StreamPlan sp = *plan_of(a, __builtin_amdgcn_readfirstlane(s_stream));
-    // task t of this stream covers seams (t - task_chunk)*256 + lane, seam k -> chunk 256k
-    const uint32_t k = (t - sp.task_chunk) * kWG + threadIdx.x;
-    const uint32_t tgt = k * kWG;
-    const uint32_t want = (k >= 1u && tgt < sp.n_chunks) ? 1u : 0u;
-    if (!__syncthreads_or(want)) return;
-    if (stage_table(sp, lds, a.w.status)) return;
-    LdsTab tab{lds};
-    RoundIO io{lds + sp.L, s_rbase, sp.units, sp.n_units};
-    const Work& w = a.w;
-    // the previous range's exit as its walks left it (another seam walk may still change
-    // it; k_check finds any chunk whose walk started from a different exit)
-    const uint32_t entry = want ? chunk_exit(sp, w, tgt - 1u) : 0u;
-    const uint32_t c_end = want ? min(tgt + (uint32_t)kWG, sp.n_chunks) : tgt;
-    walk_chain(sp, w, tab, io, want, tgt, c_end, entry, (kWG * sp.S) / kR);
+    // ---- walks
+    uint32_t want = (c >= 1u && c < c_end) ? 1u : 0u;
+    uint32_t entry = 0u;
+    if (want) entry = threadIdx.x ? s_exit[threadIdx.x - 1u] : s_entry;
+    if (want && entry == kNone) {          // predecessor not published in time
+        w.wentry[sp.chunk_base + c] = kNone;
+        want = 0u;
+    }
+    walk_chain(sp, w, tab, io, want, c, c_end, entry, ((c_end - c0) * sp.S + kR - 1u) / kR, s_mrg);
 }
 
 // Consistency check over all chunks (grid as k_sync): a chunk whose recorded walk start
@@ -905,6 +922,7 @@ static uint32_t ilog2(uint32_t v) { uint32_t r = 0; while ((1u << (r + 1)) <= v)
 // the typical distance over which a guessed-state decode merges with the true
 // path (tens to hundreds of units at L = 4096), so the spec and walk passes stay
 // close to one decode of the stream.
+static std::atomic<uint32_t> g_wait_polls{MANS_WAIT_POLLS_DEFAULT};
 static std::mutex g_rules_mu;
 static std::vector<mans_geometry_rule> g_rules;
 
@@ -1030,19 +1048,19 @@ Work make_work(void* base, const Layout& l) {
     return w;
 }
 
-static int launch_pipeline(const KArgs& a, uint32_t n_chunk_tasks, uint32_t n_item_tasks,
+static int launch_pipeline(KArgs a, uint32_t n_chunk_tasks, uint32_t n_item_tasks,
                            uint32_t total_chunks, uint32_t maxL, uint32_t maxK, hipStream_t st,
                            void** ev = nullptr) {
     size_t lds = (size_t)maxL * 4u;
     size_t lds_s = lds + (size_t)kWG * kLS * 4u;   // table + round buffer
     auto mark = [&](int i) { if (ev) (void)hipEventRecord((hipEvent_t)ev[i], st); };
+    a.wait_polls = g_wait_polls.load();
     mark(0);
-    hipLaunchKernelGGL(k_init, dim3((std::max<uint32_t>(a.n_streams, 64u) + 255u) / 256u), dim3(256),
-                       0, st, a.w, 64u, a.n_streams);
+    hipLaunchKernelGGL(k_init, dim3((std::max({a.n_streams, a.n_tasks, 64u}) + 255u) / 256u),
+                       dim3(256), 0, st, a, 64u);
     mark(1);
     hipLaunchKernelGGL(k_sync, dim3(n_chunk_tasks), dim3(kWG), lds_s, st, a);
     mark(2);
-    hipLaunchKernelGGL(k_seam, dim3(n_chunk_tasks), dim3(kWG), lds_s, st, a);
     hipLaunchKernelGGL(k_check, dim3(n_chunk_tasks), dim3(kWG), 0, st, a);
     mark(3);
     hipLaunchKernelGGL(k_fixup, dim3(a.n_streams), dim3(kWG), lds_s, st, a);
@@ -1399,6 +1417,11 @@ extern "C" int mans_graph_destroy(void* graph) {
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     delete g;
+    return MANS_OK;
+}
+
+extern "C" int mans_set_wait_polls(uint32_t polls) {
+    g_wait_polls.store(polls);
     return MANS_OK;
 }
 

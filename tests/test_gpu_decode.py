@@ -69,3 +69,21 @@ def test_constant_stream(n):
     torch.cuda.synchronize()
     assert ds.status() == 0
     assert np.array_equal(out[:n].cpu().numpy(), sym)
+
+
+@pytest.mark.parametrize("polls", [0, 1])
+def test_range_wait_runs_out(polls):
+    """A range whose wait for its predecessor's exit runs out leaves its first chunk to
+    the fixup rounds; the result must not change.  Forced from the host by a tiny poll
+    limit (every wait still ends), on a stream of many 256-chunk ranges."""
+    from multians_amd import _lib
+    print(f"case polls={polls}", flush=True)
+    _lib.set_wait_polls(polls)
+    try:
+        got, ref, sym = _case(0.1, 4096, 2_000_000, chunk_units=64, checkpoint_units=8)
+        got2, ref2, _ = _case(1.0, 1024, 3_000_000, seed=5, chunk_units=128, checkpoint_units=16)
+    finally:
+        _lib.set_wait_polls()
+    assert np.array_equal(got, ref)
+    assert np.array_equal(got[::-1], sym)
+    assert np.array_equal(got2, ref2)
