@@ -28,9 +28,31 @@ namespace {
 
 constexpr int kWG = 256;           // lanes per workgroup (4 waves)
 
+// LDS words by byte address.  The staged table's words carry LDS dword indices (table
+// start + next-state base), so a decode step forms the address of the next table word
+// with one add and one shift.
+typedef const uint32_t __attribute__((address_space(3)))* lds_cptr;
+
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+    return (uint32_t)(size_t)(const __attribute__((address_space(3))) void*)p;
+}
+
+__device__ __forceinline__ uint32_t lds_word(uint32_t byte_addr) {
+    return *(lds_cptr)(size_t)byte_addr;
+}
+
+// The 32-bit window at LDS bit address pb: the two words from the one holding bit pb,
+// shifted by pb mod 32 (alignbit uses the low five bits of its shift).
+__device__ __forceinline__ void lds_window_load(uint32_t pb, uint32_t& lo, uint32_t& hi) {
+    const uint32_t a = (pb >> 3) & ~3u;
+    lo = lds_word(a);
+    hi = lds_word(a + 4u);
+}
+
+// A staged table: word i (state i) sits at LDS dword tb + i and holds nb (bits 0-4),
+// the symbol (bits 8-15) and tb + base (bits 16-31): state i steps to tb + base + bits.
 struct LdsTab {
-    const uint32_t* p;
-    __device__ __forceinline__ uint32_t operator[](uint32_t i) const { return p[i]; }
+    uint32_t tb;
 };
 
 struct KArgs {
@@ -84,9 +106,11 @@ __device__ uint32_t stage_table(const StreamPlan& sp, uint32_t* lds, uint32_t* s
     if (threadIdx.x == 0) s_flags = 0u;
     __syncthreads();
     int bad = 0, cyc = 0;
+    // tb + base < 2^16: tb <= 160 KiB / 4 and base < L <= 16384
+    const uint32_t tb = lds_addr(lds) >> 2;
     for (uint32_t i = threadIdx.x; i < sp.L; i += blockDim.x) {
         uint32_t e = load_unit(sp.table, sp.L, i);
-        lds[i] = derive_entry(e, sp.L, i, &bad, &cyc);
+        lds[i] = derive_entry(e, sp.L, i, &bad, &cyc) + (tb << 16);
     }
     if (bad) atomicOr(&status[0], 1u);
     if (cyc) atomicOr(&s_flags, 1u);
@@ -175,42 +199,45 @@ struct RLane {
 };
 
 // Decode the lane's steps of round r (those starting in units [rs, min(rs+kR, u_stop))).
-// Branch-free step: the unit window (lo, hi) and the next unit n1 shift on a crossing
-// by selects; n1 is re-read from LDS every step (off the dependency chain).
-// Software-pipelined: the table word of the next step is requested as soon as the next
-// state index exists, before the window bookkeeping of the current step, so the only
-// dependent chain per step is LDS read -> bfe -> add -> next LDS read.  The last
-// iteration requests one word it does not use (an in-range index).
+// The lane's position is a bit offset p into its staged segment; each iteration reads
+// the 32-bit window at p (two LDS words, one ds_read2) and decodes two steps from it:
+// a step takes at most 15 bits, so the second step's bits (offset nb1, width nb2) lie in
+// the same window.  The second step runs whether or not the first one ended the round;
+// if it did, the first step's state is kept (both table words and the window at the
+// next position are in range: p stays below (kR + 1) * 32).
 __device__ __forceinline__ void round_decode(const LdsTab& tab, const uint32_t* lbuf, RLane& L,
                                              uint32_t r) {
     if (!L.active) return;
     const uint32_t rs = L.rbase + r * kR;
     const uint32_t re = min(rs + kR, L.u_stop);
     if (L.u >= re) return;
-    uint32_t ui = L.u - rs;                  // unit index within the staged segment
-    const uint32_t ue = re - rs;
-    uint32_t o = L.o, xi = L.xi, cnt = L.cnt;
-    uint32_t lo = lbuf[ui], hi = lbuf[ui + 1u], n1 = lbuf[ui + 2u];
-    uint32_t w = tab[xi];
-    // opaque copy: keeps the compiler from merging this load with the in-loop one into
-    // a single load at the loop head, which would undo the pipelining
-    asm volatile("" : "+v"(w));
+    // p: LDS bit address (the segment start is a multiple of 32)
+    const uint32_t pb0 = lds_addr(lbuf) * 8u;
+    const uint32_t end = pb0 + (re - rs) * 32u;
+    uint32_t p = pb0 + (L.u - rs) * 32u + L.o;
+    uint32_t lo, hi;
+    lds_window_load(p, lo, hi);
+    uint32_t i = L.xi + tab.tb;
+    uint32_t w = lds_word(i << 2);
+    uint32_t cnt = L.cnt, i1, p1;
     do {
-        const uint32_t win = funnel_shr(hi, lo, o);
-        const uint32_t nb = w & 31u;          // w is dead once the next read is issued
-        xi = (w >> 16) + low_bits(win, w);
-        w = tab[xi];
-        __builtin_amdgcn_sched_barrier(0);   // issue the read before the bookkeeping
-        o += nb;
-        ++cnt;
-        const bool c = o >= 32u;
-        o &= 31u;
-        lo = c ? hi : lo;
-        hi = c ? n1 : hi;
-        ui += c ? 1u : 0u;
-        n1 = lbuf[ui + 2u];
-    } while (ui < ue);
-    L.u = rs + ui; L.o = o; L.xi = xi; L.cnt = cnt;
+        const uint32_t win = funnel_shr(hi, lo, p);
+        i1 = (w >> 16) + low_bits(win, w);
+        const uint32_t w1 = lds_word(i1 << 2);
+        p1 = p + (w & 0xFFu);
+        i = (w1 >> 16) + __builtin_amdgcn_ubfe(win, w, w1);   // offset nb1, width nb2
+        w = lds_word(i << 2);
+        p = p1 + (w1 & 0xFFu);
+        lds_window_load(p, lo, hi);
+        cnt += 2u;
+    } while (p < end);
+    if (p1 >= end) {                        // the round ended after the first step
+        i = i1;
+        p = p1;
+        --cnt;
+    }
+    p -= pb0;
+    L.u = rs + (p >> 5); L.o = p & 31u; L.xi = i - tab.tb; L.cnt = cnt;
 }
 
 // Run rounds (at most nr_max, uniform) until no lane is active.  After each round,
@@ -468,7 +495,7 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
     const uint32_t t = s_task;
     const StreamPlan sp = *plan_of(a, __builtin_amdgcn_readfirstlane(s_stream));
     const uint32_t cyc = stage_table(sp, lds, a.w.status);
-    LdsTab tab{lds};
+    LdsTab tab{lds_addr(lds) >> 2};
     RoundIO io{lds + sp.L, s_rbase, sp.units, sp.n_units};
     const Work& w = a.w;
     const uint32_t c0 = (t - sp.task_chunk) * kWG;
@@ -587,7 +614,7 @@ __global__ __launch_bounds__(kWG) void k_fixup(KArgs a) {
         if (s_n == 0u) continue;
         const StreamPlan sp = *plan_of(a, sid);
         if (stage_table(sp, lds, w.status)) continue;
-        LdsTab tab{lds};
+        LdsTab tab{lds_addr(lds) >> 2};
         RoundIO io{lds + sp.L, s_rbase, sp.units, sp.n_units};
         uint32_t* list = w.flist + sp.chunk_base;
         uint32_t* next = w.flist2 + sp.chunk_base;
@@ -773,44 +800,44 @@ __device__ __forceinline__ void item_write(const LdsTab& tab, const uint32_t* lb
     typedef uint8_t __attribute__((address_space(1)))* gptr_u8;
     typedef uint32_t __attribute__((address_space(1)))* gptr_w;
     gptr_u8 dst = (gptr_u8)dst_generic;
-    uint32_t ui = entry_off(entry) >> 5;
-    uint32_t o = entry_off(entry) & 31u;
-    uint32_t lo = lbuf[ui], hi = lbuf[ui + 1u], n1 = lbuf[ui + 2u];
-    uint32_t xi = entry_state(entry);
-    uint32_t wcur = tab[xi];
-    asm volatile("" : "+v"(wcur));     // as in round_decode
-    // software-pipelined like round_decode: the next table word is requested first
-    auto next_sym = [&]() -> uint32_t {
-        const uint32_t w = wcur;
-        const uint32_t win = funnel_shr(hi, lo, o);
-        const uint32_t nb = w & 31u;
-        const uint32_t sym = w >> 8;
-        xi = (w >> 16) + low_bits(win, w);
-        wcur = tab[xi];
-        __builtin_amdgcn_sched_barrier(0);
-        o += nb;
-        const bool c = o >= 32u;
-        o &= 31u;
-        lo = c ? hi : lo;
-        hi = c ? n1 : hi;
-        ui += c ? 1u : 0u;
-        n1 = lbuf[min(ui + 2u, kWL - 1u)];
-        __builtin_amdgcn_sched_barrier(0);   // keep the unit read in this step
-        return sym;                           // symbol in bits 0-7
+    // p: LDS bit address in the item's staged units (at most kWI * 32 + 15 past their
+    // start after the last step, so the window read stays inside the kWI + 2 units)
+    uint32_t p = lds_addr(lbuf) * 8u + entry_off(entry);
+    uint32_t lo, hi;
+    lds_window_load(p, lo, hi);
+    uint32_t w = lds_word((entry_state(entry) + tab.tb) << 2);
+    // one step; the symbol is byte 1 of the result
+    auto step1 = [&]() -> uint32_t {
+        const uint32_t cur = w;
+        const uint32_t win = funnel_shr(hi, lo, p);
+        const uint32_t i1 = (w >> 16) + low_bits(win, w);
+        p += w & 0xFFu;
+        w = lds_word(i1 << 2);
+        lds_window_load(p, lo, hi);
+        return cur;
+    };
+    auto next_sym = [&]() -> uint32_t { return (step1() >> 8) & 0xFFu; };
+    // two steps from one window (as in round_decode); symbols in bytes 0 and 1
+    auto step2 = [&]() -> uint32_t {
+        const uint32_t win = funnel_shr(hi, lo, p);
+        const uint32_t i1 = (w >> 16) + low_bits(win, w);
+        const uint32_t w1 = lds_word(i1 << 2);
+        const uint32_t p1 = p + (w & 0xFFu);
+        const uint32_t i2 = (w1 >> 16) + __builtin_amdgcn_ubfe(win, w, w1);
+        const uint32_t two = __builtin_amdgcn_perm(w1, w, 0x0c0c0501u);
+        w = lds_word(i2 << 2);
+        p = p1 + (w1 & 0xFFu);
+        lds_window_load(p, lo, hi);
+        return two;
     };
     // Symbols are packed four to a dword and stored 16 bytes at a time: the lanes of a
     // wave write 64 separate regions, so every store instruction costs one cache-line
     // request per lane in the address unit, whatever its width.  Bytes, then dwords up
     // to 16-byte alignment, then 16-byte stores, then the tail.
     auto next_word = [&]() -> uint32_t {
-        uint32_t acc = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t sym = next_sym();
-            // acc = (acc >> 8) | (symbol << 24): byte 0 of sym goes on top
-            acc = __builtin_amdgcn_perm(sym, acc, 0x04030201u);
-        }
-        return acc;
+        const uint32_t a = step2();
+        const uint32_t b = step2();
+        return __builtin_amdgcn_perm(b, a, 0x05040100u);
     };
     uint32_t i = 0;
     const uint32_t head = (uint32_t)((4u - ((uintptr_t)dst & 3u)) & 3u);
@@ -902,7 +929,7 @@ __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint3
             uint64_t off = s_off[threadIdx.x], end = s_off[threadIdx.x + 1];
             if (end > sp.n_out) end = sp.n_out;
             if (off < end) {
-                LdsTab tab{lds};
+                LdsTab tab{lds_addr(lds) >> 2};
                 item_write(tab, buf + threadIdx.x * kWL, entry, sp.out + off, (uint32_t)(end - off));
             }
         }
