@@ -90,6 +90,19 @@ int mans_emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint6
             const uint32_t stop = (t > c0 && !own_merged[t - 1]) ? 1u : 0u;
             st_merges += lane_walk_chain(sp, w, ht, t, c_end, exitv[t - 1], stop);
         }
+        // repair passes (as many as the kernel's): chunks after c0 whose start is not
+        // their predecessor's exit, walked again from it (entries captured first; no stop
+        // rule; back to front as above)
+        for (uint32_t pass = 0; pass < 4u; ++pass) {
+            std::vector<uint32_t> again, from;
+            for (uint32_t t = c0 + 1u; t < c_end; ++t) {
+                const uint32_t e = chunk_exit(sp, w, t - 1u);
+                if (went[t] != e) { again.push_back(t); from.push_back(e); }
+            }
+            if (again.empty()) break;
+            for (size_t k = again.size(); k-- > 0;)
+                st_merges += lane_walk_chain(sp, w, ht, again[k], c_end, from[k]);
+        }
     }
     // k_check: every chunk whose walk start differs from its predecessor's exit
     for (uint32_t c = 0; c < nc; ++c) {

@@ -27,6 +27,7 @@ using namespace mans;
 namespace {
 
 constexpr int kWG = 256;           // lanes per workgroup (4 waves)
+constexpr uint32_t kRepairPasses = 4;   // in-range repair passes after the walks (k_sync)
 
 // LDS words by byte address.  The staged table's words carry LDS dword indices (table
 // start + next-state base), so a decode step forms the address of the next table word
@@ -277,6 +278,12 @@ __device__ void run_rounds(const LdsTab& tab, RoundIO& io, RLane& L, uint32_t nr
 
 __device__ __forceinline__ uint32_t ld_coherent(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// mans::chunk_exit for records other lanes of this kernel have just written.
+__device__ __forceinline__ uint32_t chunk_exit_now(const StreamPlan& sp, const Work& w, uint32_t c) {
+    const uint32_t gc = sp.chunk_base + c;
+    return ld_coherent(&w.mq[gc]) == kNone ? ld_coherent(&w.wexit[gc]) : ld_coherent(&w.exitv[gc]);
 }
 
 // Walk into chunk c (c >= 1) of sp from `entry` for the lanes with want != 0, in rounds.
@@ -572,7 +579,25 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
         w.wentry[sp.chunk_base + c] = kNone;
         want = 0u;
     }
-    walk_chain(sp, w, tab, io, want, c, c_end, entry, ((c_end - c0) * sp.S + kR - 1u) / kR, s_mrg);
+    const uint32_t nr = ((c_end - c0) * sp.S + kR - 1u) / kR;
+    walk_chain(sp, w, tab, io, want, c, c_end, entry, nr, s_mrg);
+    // ---- repair passes: a chunk whose recorded start is not its predecessor's exit (a
+    // walk that stopped at its chunk's end although its start was right, found right by a
+    // longer walk) is walked again from that exit.  The walks of a pass start together and
+    // continue until they merge (entries are captured before any of them writes); a pass
+    // can change an exit that a later chunk started from, so passes repeat, at most
+    // kRepairPasses.  What remains (and chunk c0's entry) is left to k_check / k_fixup.
+    for (uint32_t pass = 0; pass < kRepairPasses; ++pass) {
+        __threadfence();
+        __syncthreads();
+        uint32_t want2 = 0u, entry2 = 0u;
+        if (c > c0 && c < c_end) {
+            entry2 = chunk_exit_now(sp, w, c - 1u);
+            want2 = ld_coherent(&w.wentry[sp.chunk_base + c]) != entry2 ? 1u : 0u;
+        }
+        if (!__syncthreads_or(want2)) break;
+        walk_chain(sp, w, tab, io, want2, c, c_end, entry2, nr);
+    }
 }
 
 // Consistency check over all chunks (grid as k_sync): a chunk whose recorded walk start
