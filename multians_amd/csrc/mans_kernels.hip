@@ -56,6 +56,12 @@ struct LdsTab {
     uint32_t tb;
 };
 
+// First LDS word after a staged table of L words, 16-byte aligned (the unit buffers
+// after it take 16- and 8-byte stores).
+__host__ __device__ __forceinline__ uint32_t table_words(uint32_t L) {
+    return (L + 3u) & ~3u;
+}
+
 struct KArgs {
     StreamPlan one;                 // used when plans == nullptr
     const StreamPlan* plans;        // batch
@@ -503,7 +509,7 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
     const StreamPlan sp = *plan_of(a, __builtin_amdgcn_readfirstlane(s_stream));
     const uint32_t cyc = stage_table(sp, lds, a.w.status);
     LdsTab tab{lds_addr(lds) >> 2};
-    RoundIO io{lds + sp.L, s_rbase, sp.units, sp.n_units};
+    RoundIO io{lds + table_words(sp.L), s_rbase, sp.units, sp.n_units};
     const Work& w = a.w;
     const uint32_t c0 = (t - sp.task_chunk) * kWG;
     const uint32_t c = c0 + threadIdx.x;
@@ -640,7 +646,7 @@ __global__ __launch_bounds__(kWG) void k_fixup(KArgs a) {
         const StreamPlan sp = *plan_of(a, sid);
         if (stage_table(sp, lds, w.status)) continue;
         LdsTab tab{lds_addr(lds) >> 2};
-        RoundIO io{lds + sp.L, s_rbase, sp.units, sp.n_units};
+        RoundIO io{lds + table_words(sp.L), s_rbase, sp.units, sp.n_units};
         uint32_t* list = w.flist + sp.chunk_base;
         uint32_t* next = w.flist2 + sp.chunk_base;
         if (threadIdx.x == 0) {                                    // statistics
@@ -925,7 +931,7 @@ __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint3
         __syncthreads();
         const uint32_t sid = __builtin_amdgcn_readfirstlane(s_stream);
         const StreamPlan sp = *plan_of(a, sid);
-        uint32_t* buf = lds + sp.L;
+        uint32_t* buf = lds + table_words(sp.L);
         if (s_have != sid) {
             (void)stage_table(sp, lds, a.w.status);    // ends with a barrier
             if (threadIdx.x == 0) s_have = sid;
@@ -1103,7 +1109,7 @@ Work make_work(void* base, const Layout& l) {
 static int launch_pipeline(KArgs a, uint32_t n_chunk_tasks, uint32_t n_item_tasks,
                            uint32_t total_chunks, uint32_t maxL, uint32_t maxK, hipStream_t st,
                            void** ev = nullptr) {
-    size_t lds = (size_t)maxL * 4u;
+    size_t lds = (size_t)table_words(maxL) * 4u;
     size_t lds_s = lds + (size_t)kWG * kLS * 4u;   // table + round buffer
     auto mark = [&](int i) { if (ev) (void)hipEventRecord((hipEvent_t)ev[i], st); };
     a.wait_polls = g_wait_polls.load();
@@ -1143,7 +1149,8 @@ static int check_stream_args(const uint32_t* units, uint64_t n_units, const uint
                              uint64_t n_out, const uint32_t* table, uint32_t L,
                              uint32_t first_state, uint32_t first_bit) {
     if (!table) return set_error(MANS_EINVAL, "table is NULL");
-    if (L < 2u || L > 16384u) return set_error(MANS_EINVAL, "L=%u outside [2, 16384]", L);
+    if (L < 2u || L > 16384u || (L & (L - 1u)))
+        return set_error(MANS_EINVAL, "L=%u is not a power of two in [2, 16384]", L);
     if (n_out && (!units || !out)) return set_error(MANS_EINVAL, "units/out is NULL");
     if (n_out && n_units == 0) return set_error(MANS_EINVAL, "n_units is 0 but n_out > 0");
     if (((uintptr_t)units & 15u) != 0u)

@@ -43,6 +43,19 @@ def test_c_abi_errors_without_gpu():
     assert L.mans_decode(None) == _lib.MANS_EINVAL
     assert b"NULL" in L.mans_last_error()
     assert L.mans_error_string(_lib.MANS_ETABLE) == b"decoder table rejected"
+    # the state count must be a power of two (argument checks come before any GPU call)
+    units = np.zeros(16, np.uint32)
+    table = np.zeros(1000, np.uint32)
+    out = np.zeros(16, np.uint8)
+    args = _lib.DecodeArgs()
+    args.units = units.ctypes.data
+    args.n_units = 16
+    args.out = out.ctypes.data
+    args.n_out = 16
+    args.table = table.ctypes.data
+    args.L = 1000
+    assert L.mans_decode(C.byref(args)) == _lib.MANS_EINVAL
+    assert b"power of two" in L.mans_last_error()
 
 
 def _fnv(b):
