@@ -16,6 +16,7 @@
 #include <atomic>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -106,8 +107,10 @@ __device__ uint32_t stream_of_chunk(const KArgs& a, uint32_t gc) {
     return lo;
 }
 
-// Stage the derived table of stream sp into LDS.  Returns (uniform) 0, or 1 when the
-// table has zero-bit cycles (see derive_entry); invalid tables set status[0] bit 0.
+// Stage the derived table of stream sp into LDS.  Returns (uniform) flags: bit 0 when
+// the table has zero-bit cycles (see derive_entry), bit 1 when it has rejected entries
+// (they stage as zero, a zero-bit self-loop a round would never leave, so nothing of
+// that stream is decoded); rejected tables set status[0] bit 0.
 __device__ uint32_t stage_table(const StreamPlan& sp, uint32_t* lds, uint32_t* status) {
     __shared__ uint32_t s_flags;
     if (threadIdx.x == 0) s_flags = 0u;
@@ -119,11 +122,14 @@ __device__ uint32_t stage_table(const StreamPlan& sp, uint32_t* lds, uint32_t* s
         uint32_t e = load_unit(sp.table, sp.L, i);
         lds[i] = derive_entry(e, sp.L, i, &bad, &cyc) + (tb << 16);
     }
-    if (bad) atomicOr(&status[0], 1u);
+    if (bad) {
+        atomicOr(&status[0], 1u);
+        atomicOr(&s_flags, 2u);
+    }
     if (cyc) atomicOr(&s_flags, 1u);
     __syncthreads();
     const uint32_t f = s_flags;
-    if (f && sp.n_ck > 1u && threadIdx.x == 0) atomicOr(&status[0], 1u);
+    if ((f & 1u) && sp.n_ck > 1u && threadIdx.x == 0) atomicOr(&status[0], 1u);
     return __builtin_amdgcn_readfirstlane(f);
 }
 
@@ -507,12 +513,27 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
     __syncthreads();
     const uint32_t t = s_task;
     const StreamPlan sp = *plan_of(a, __builtin_amdgcn_readfirstlane(s_stream));
-    const uint32_t cyc = stage_table(sp, lds, a.w.status);
+    const uint32_t sf = stage_table(sp, lds, a.w.status);
+    const uint32_t cyc = sf & 1u;
     LdsTab tab{lds_addr(lds) >> 2};
     RoundIO io{lds + table_words(sp.L), s_rbase, sp.units, sp.n_units};
     const Work& w = a.w;
     const uint32_t c0 = (t - sp.task_chunk) * kWG;
     const uint32_t c = c0 + threadIdx.x;
+    if (sf & 2u) {
+        // rejected table (uniform): empty, consistent records, so that the later
+        // kernels read in-range values (the stream's output is not written)
+        if (c < sp.n_chunks) {
+            const uint32_t gc = sp.chunk_base + c;
+            w.mq[gc] = kNone;
+            w.wentry[gc] = 0u;
+            w.wexit[gc] = 0u;
+            w.wtotal[gc] = 0u;
+            w.exitv[gc] = 0u;
+            w.own_total[gc] = 0u;
+        }
+        return;
+    }
     if (cyc) {
         // zero-bit cycles: only a single checkpoint interval is decodable (count-driven)
         if (c == 0u && sp.n_chunks) {
@@ -904,7 +925,7 @@ __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint3
     // counter (ctick[0], zeroed by k_init), so slow passes (high symbols per unit) do
     // not pile up on a few workgroups.
     extern __shared__ uint32_t lds[];
-    __shared__ uint32_t s_stream, s_have, s_next;
+    __shared__ uint32_t s_stream, s_have, s_next, s_skip;
     __shared__ uint64_t s_off[kWG + 1];
     if (threadIdx.x == 0) {
         s_have = kNone;
@@ -933,8 +954,11 @@ __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint3
         const StreamPlan sp = *plan_of(a, sid);
         uint32_t* buf = lds + table_words(sp.L);
         if (s_have != sid) {
-            (void)stage_table(sp, lds, a.w.status);    // ends with a barrier
-            if (threadIdx.x == 0) s_have = sid;
+            const uint32_t sf = stage_table(sp, lds, a.w.status);    // ends with a barrier
+            if (threadIdx.x == 0) {
+                s_have = sid;
+                s_skip = sf & 2u;                  // rejected table: nothing to write
+            }
         }
         // items i0 .. i0+nitems-1 of the stream; item i starts at checkpoint i*kWF
         const uint32_t i0 = (t - sp.task_item) * kWG;
@@ -956,7 +980,7 @@ __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint3
             const StreamPlan np = *plan_of(a, __builtin_amdgcn_readfirstlane(ns));
             pass_load(np.units, np.n_units, (tn - np.task_item) * kWG * kWI, pf);
         }
-        if (threadIdx.x < nitems) {
+        if (threadIdx.x < nitems && !s_skip) {
             uint64_t off = s_off[threadIdx.x], end = s_off[threadIdx.x + 1];
             if (end > sp.n_out) end = sp.n_out;
             if (off < end) {
@@ -1352,51 +1376,251 @@ static std::vector<char> plan_image(const std::vector<StreamPlan>& plans, uint32
 
 }  // namespace mans
 
+// ------------------------------------------------------------ batch parts
+// A batch of many streams is decoded as up to kMaxParts independent pipelines on
+// concurrent HIP streams: each part (a share of the streams, balanced by symbol count)
+// has its own plans, tickets and workspace region.  While one part's sync pass drains
+// its last long workgroups, another part's write pass uses the free CUs.  A part's
+// status words are folded into the first part's (read by mans_decode_status).
+constexpr size_t kMaxParts = 3;
+constexpr size_t kStreamsPerPart = 256;   // fewer streams per part: one pipeline
+
+struct BatchPart {
+    std::vector<mans_stream_desc> descs;
+    std::vector<StreamPlan> plans;
+    uint64_t chunks = 0, cks = 0, units = 0, syms = 0, off = 0;
+    uint32_t ct = 0, it = 0, ml = 0, mk = 0;
+    Layout l;
+};
+
+static int plan_parts(const mans_stream_desc* d, size_t n, size_t k, std::vector<BatchPart>& parts,
+                      uint64_t* bytes) {
+    parts.assign(k, BatchPart());
+    if (k == 1) {
+        parts[0].descs.assign(d, d + n);
+    } else {
+        // longest processing time first: each stream to the part with the fewest symbols
+        std::vector<size_t> idx(n);
+        for (size_t i = 0; i < n; ++i) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](size_t x, size_t y) { return d[x].n_out > d[y].n_out; });
+        std::vector<uint64_t> load(k, 0);
+        std::vector<std::vector<size_t>> mine(k);
+        for (size_t i : idx) {
+            size_t p = 0;
+            for (size_t q = 1; q < k; ++q)
+                if (load[q] < load[p]) p = q;
+            load[p] += d[i].n_out + 1u;
+            mine[p].push_back(i);
+        }
+        for (size_t p = 0; p < k; ++p) {
+            std::sort(mine[p].begin(), mine[p].end());
+            for (size_t i : mine[p]) parts[p].descs.push_back(d[i]);
+        }
+    }
+    uint64_t o = 0;
+    for (BatchPart& bp : parts) {
+        int rc = build_plans(bp.descs.data(), bp.descs.size(), bp.plans, &bp.chunks, &bp.cks,
+                             &bp.ct, &bp.it, &bp.ml, &bp.mk, &bp.units, &bp.syms);
+        if (rc) return rc;
+        bp.l = make_layout(bp.chunks, bp.cks, bp.descs.size());
+        bp.off = o;
+        o = align256(o + bp.l.bytes);
+    }
+    *bytes = o;
+    return MANS_OK;
+}
+
+static size_t n_parts(size_t n) {
+    return std::max<size_t>(1, std::min(kMaxParts, n / kStreamsPerPart));
+}
+
+// Fold a later part's status words into the first part's.
+__global__ void k_status_merge(uint32_t* dst, const uint32_t* src) {
+    if (threadIdx.x == 0) {
+        dst[0] |= src[0];
+        dst[3] += src[3];
+        dst[4] = max(dst[4], src[4]);
+        dst[5] += src[5];
+        dst[6] = max(dst[6], src[6]);
+        dst[7] += src[7];
+    }
+}
+
+// Internal streams (default priority) and fork/join events for the parts of batches
+// enqueued on one caller stream, and a ring of two pinned staging buffers for their plan
+// uploads, created once per (caller stream, device).  With pinned staging the upload is
+// an ordinary stream-ordered copy: the host does not wait for the previous decode.
+struct PartStreams {
+    hipStream_t caller = nullptr;
+    int dev = 0;
+    std::mutex mu;                        // one enqueue at a time per caller stream
+    hipStream_t s[kMaxParts] = {};
+    hipEvent_t fork = nullptr, join[kMaxParts] = {};
+    char* pin[2] = {};
+    size_t cap[2] = {};
+    hipEvent_t copied[2] = {};
+    int next = 0;
+};
+static std::mutex g_parts_mu;
+static std::vector<std::unique_ptr<PartStreams>> g_parts;
+
+static int part_streams(hipStream_t caller, PartStreams** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return set_error(MANS_EHIP, "get device: %s", hipGetErrorString(e));
+    std::lock_guard<std::mutex> lk(g_parts_mu);
+    for (auto& ps : g_parts)
+        if (ps->caller == caller && ps->dev == dev) {
+            *out = ps.get();
+            return MANS_OK;
+        }
+    std::unique_ptr<PartStreams> ps(new PartStreams());
+    ps->caller = caller;
+    ps->dev = dev;
+    e = hipEventCreateWithFlags(&ps->fork, hipEventDisableTiming);
+    for (size_t p = 0; p < kMaxParts && e == hipSuccess; ++p) {
+        e = hipStreamCreateWithFlags(&ps->s[p], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ps->join[p], hipEventDisableTiming);
+    }
+    for (int b = 0; b < 2 && e == hipSuccess; ++b)
+        e = hipEventCreateWithFlags(&ps->copied[b], hipEventDisableTiming);
+    if (e != hipSuccess) return set_error(MANS_EHIP, "part streams: %s", hipGetErrorString(e));
+    *out = ps.get();
+    g_parts.push_back(std::move(ps));
+    return MANS_OK;
+}
+
+static KArgs part_args(const BatchPart& bp, void* workspace) {
+    char* base = (char*)workspace + bp.off;
+    StreamPlan* dplans = (StreamPlan*)(base + bp.l.o_plans);
+    KArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.plans = dplans;
+    a.order = (const uint32_t*)(dplans + bp.descs.size());
+    a.sorder = a.order + bp.ct;
+    a.n_streams = (uint32_t)bp.descs.size();
+    a.n_tasks = bp.ct;
+    a.w = make_work(base, bp.l);
+    return a;
+}
+
+// Enqueue the parts' pipelines: forked from `st` onto `ps`'s streams and joined back,
+// then the status merge on `st`.
+static int launch_parts(const std::vector<BatchPart>& parts, void* workspace, hipStream_t st,
+                        const PartStreams& ps) {
+    hipError_t e = hipEventRecord(ps.fork, st);
+    for (size_t p = 0; p < parts.size() && e == hipSuccess; ++p) {
+        const BatchPart& bp = parts[p];
+        e = hipStreamWaitEvent(ps.s[p], ps.fork, 0);
+        if (e != hipSuccess) break;
+        int rc = launch_pipeline(part_args(bp, workspace), bp.ct, bp.it, (uint32_t)bp.chunks, bp.ml,
+                                 bp.mk, ps.s[p], nullptr);
+        if (rc) return rc;
+        e = hipEventRecord(ps.join[p], ps.s[p]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, ps.join[p], 0);
+    }
+    uint32_t* s0 = (uint32_t*)((char*)workspace + parts[0].off + parts[0].l.o_status);
+    for (size_t p = 1; p < parts.size() && e == hipSuccess; ++p) {
+        const uint32_t* sp = (const uint32_t*)((char*)workspace + parts[p].off + parts[p].l.o_status);
+        hipLaunchKernelGGL(k_status_merge, dim3(1), dim3(64), 0, st, s0, sp);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) return set_error(MANS_EHIP, "part launch: %s", hipGetErrorString(e));
+    return MANS_OK;
+}
+
+// Plan images of all parts, uploaded into their workspace regions: synchronously (ps ==
+// nullptr) or as copies on st from the next pinned staging buffer of ps (reused once the
+// copies from its previous use have run).
+static int upload_parts(const std::vector<BatchPart>& parts, void* workspace, hipStream_t st,
+                        PartStreams* ps) {
+    std::vector<std::vector<char>> imgs;
+    size_t total = 0;
+    for (const BatchPart& bp : parts) {
+        imgs.push_back(plan_image(bp.plans, bp.ct));
+        total += (imgs.back().size() + 255u) & ~(size_t)255u;
+    }
+    hipError_t e = hipSuccess;
+    if (!ps) {
+        for (size_t p = 0; p < parts.size() && e == hipSuccess; ++p)
+            e = hipMemcpy((char*)workspace + parts[p].off + parts[p].l.o_plans, imgs[p].data(),
+                          imgs[p].size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return set_error(MANS_EHIP, "plan upload: %s", hipGetErrorString(e));
+        return MANS_OK;
+    }
+    const int b = ps->next;
+    ps->next ^= 1;
+    e = hipEventSynchronize(ps->copied[b]);          // previous copies from this buffer done
+    if (e == hipSuccess && ps->cap[b] < total) {
+        if (ps->pin[b]) e = hipHostFree(ps->pin[b]);
+        ps->pin[b] = nullptr;
+        ps->cap[b] = 0;
+        if (e == hipSuccess) e = hipHostMalloc((void**)&ps->pin[b], total, hipHostMallocDefault);
+        if (e == hipSuccess) ps->cap[b] = total;
+    }
+    size_t o = 0;
+    for (size_t p = 0; p < parts.size() && e == hipSuccess; ++p) {
+        std::memcpy(ps->pin[b] + o, imgs[p].data(), imgs[p].size());
+        e = hipMemcpyAsync((char*)workspace + parts[p].off + parts[p].l.o_plans, ps->pin[b] + o,
+                           imgs[p].size(), hipMemcpyHostToDevice, st);
+        o += (imgs[p].size() + 255u) & ~(size_t)255u;
+    }
+    if (e == hipSuccess) e = hipEventRecord(ps->copied[b], st);
+    if (e != hipSuccess) return set_error(MANS_EHIP, "plan upload: %s", hipGetErrorString(e));
+    return MANS_OK;
+}
+
 extern "C" int mans_batch_workspace_bytes(const mans_stream_desc* descs, size_t n, uint64_t* bytes) {
     if (!descs || !n || !bytes) return set_error(MANS_EINVAL, "bad batch");
-    std::vector<StreamPlan> plans;
-    uint64_t chunks, cks, u, s;
-    uint32_t ct, it, ml, mk;
-    int rc = build_plans(descs, n, plans, &chunks, &cks, &ct, &it, &ml, &mk, &u, &s);
+    // enough for the split decode and for the single pipeline (kernel timing)
+    std::vector<BatchPart> parts;
+    uint64_t b1 = 0, bk = 0;
+    int rc = plan_parts(descs, n, 1, parts, &b1);
+    if (!rc) rc = plan_parts(descs, n, n_parts(n), parts, &bk);
     if (rc) return rc;
-    *bytes = make_layout(chunks, cks, n).bytes;
+    *bytes = std::max(b1, bk);
     return MANS_OK;
 }
 
 extern "C" int mans_decode_batch(const mans_stream_desc* descs, size_t n,
                                  const mans_batch_opts* opts, mans_batch_stats* stats) {
     if (!descs || !n || !opts) return set_error(MANS_EINVAL, "bad batch");
-    std::vector<StreamPlan> plans;
-    uint64_t chunks, cks, units, syms;
-    uint32_t ct, it, ml, mk;
-    int rc = build_plans(descs, n, plans, &chunks, &cks, &ct, &it, &ml, &mk, &units, &syms);
+    // per-kernel timing events describe one pipeline, so a timed decode is not split
+    const size_t k = opts->kernel_events ? 1 : n_parts(n);
+    std::vector<BatchPart> parts;
+    uint64_t need = 0;
+    int rc = plan_parts(descs, n, k, parts, &need);
     if (rc) return rc;
-    Layout l = make_layout(chunks, cks, n);
-    if (!opts->workspace || opts->workspace_bytes < l.bytes)
+    if (!opts->workspace || opts->workspace_bytes < need)
         return set_error(MANS_ENOMEM, "batch workspace of %llu bytes < %llu needed",
-                         (unsigned long long)opts->workspace_bytes, (unsigned long long)l.bytes);
+                         (unsigned long long)opts->workspace_bytes, (unsigned long long)need);
     hipStream_t st = (hipStream_t)opts->stream;
-    StreamPlan* dplans = (StreamPlan*)((char*)opts->workspace + l.o_plans);
-    const std::vector<char> img = plan_image(plans, ct);
-    hipError_t e = hipMemcpyAsync(dplans, img.data(), img.size(), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return set_error(MANS_EHIP, "plan upload: %s", hipGetErrorString(e));
-    KArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.plans = dplans;
-    a.order = (const uint32_t*)(dplans + n);
-    a.sorder = a.order + ct;
-    a.n_streams = (uint32_t)n;
-    a.n_tasks = ct;
-    a.w = make_work(opts->workspace, l);
-    rc = launch_pipeline(a, ct, it, (uint32_t)chunks, ml, mk, st, opts->kernel_events);
+    PartStreams* ps = nullptr;
+    rc = part_streams(st, &ps);
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> lk(ps->mu);
+        rc = upload_parts(parts, opts->workspace, st, ps);
+        if (rc) return rc;
+        if (k == 1) {
+            const BatchPart& bp = parts[0];
+            rc = launch_pipeline(part_args(bp, opts->workspace), bp.ct, bp.it, (uint32_t)bp.chunks,
+                                 bp.ml, bp.mk, st, opts->kernel_events);
+        } else {
+            rc = launch_parts(parts, opts->workspace, st, *ps);
+        }
+    }
     if (rc) return rc;
     if (stats) {
-        stats->total_units = units;
-        stats->total_symbols = syms;
+        std::memset(stats, 0, sizeof(*stats));
+        for (const BatchPart& bp : parts) {
+            stats->total_units += bp.units;
+            stats->total_symbols += bp.syms;
+            stats->n_chunks += (uint32_t)bp.chunks;
+            stats->n_items += (uint32_t)bp.cks;
+        }
         stats->n_streams = (uint32_t)n;
-        stats->n_chunks = (uint32_t)chunks;
-        stats->n_items = (uint32_t)cks;
         stats->status = 0;
     }
     return MANS_OK;
@@ -1414,44 +1638,54 @@ extern "C" int mans_batch_graph_create(const mans_stream_desc* descs, size_t n,
                                        const mans_batch_opts* opts, void** graph) {
     if (!descs || !n || !opts || !graph) return set_error(MANS_EINVAL, "bad batch");
     *graph = nullptr;
-    std::vector<StreamPlan> plans;
-    uint64_t chunks, cks, units, syms;
-    uint32_t ct, it, ml, mk;
-    int rc = build_plans(descs, n, plans, &chunks, &cks, &ct, &it, &ml, &mk, &units, &syms);
+    std::vector<BatchPart> parts;
+    uint64_t need = 0;
+    int rc = plan_parts(descs, n, n_parts(n), parts, &need);
     if (rc) return rc;
-    Layout l = make_layout(chunks, cks, n);
-    if (!opts->workspace || opts->workspace_bytes < l.bytes)
+    if (!opts->workspace || opts->workspace_bytes < need)
         return set_error(MANS_ENOMEM, "batch workspace of %llu bytes < %llu needed",
-                         (unsigned long long)opts->workspace_bytes, (unsigned long long)l.bytes);
-    StreamPlan* dplans = (StreamPlan*)((char*)opts->workspace + l.o_plans);
-    const std::vector<char> img = plan_image(plans, ct);
-    hipError_t e = hipMemcpy(dplans, img.data(), img.size(), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return set_error(MANS_EHIP, "plan upload: %s", hipGetErrorString(e));
-    KArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.plans = dplans;
-    a.order = (const uint32_t*)(dplans + n);
-    a.sorder = a.order + ct;
-    a.n_streams = (uint32_t)n;
-    a.n_tasks = ct;
-    a.w = make_work(opts->workspace, l);
+                         (unsigned long long)opts->workspace_bytes, (unsigned long long)need);
+    rc = upload_parts(parts, opts->workspace, nullptr, nullptr);
+    if (rc) return rc;
+    // capture on a private stream; the parts fork onto private streams inside the capture
     hipStream_t cs = nullptr;
-    e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
-    if (e != hipSuccess) return set_error(MANS_EHIP, "capture stream: %s", hipGetErrorString(e));
+    PartStreams ps;
+    hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ps.fork, hipEventDisableTiming);
+    for (size_t p = 0; p < parts.size() && e == hipSuccess; ++p) {
+        e = hipStreamCreateWithFlags(&ps.s[p], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ps.join[p], hipEventDisableTiming);
+    }
+    // (the staging ring and mutex of ps are unused here)
     GraphHandle* g = new GraphHandle();
-    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) {
-        rc = launch_pipeline(a, ct, it, (uint32_t)chunks, ml, mk, cs, nullptr);
-        hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
-        if (!rc && e2 != hipSuccess) rc = set_error(MANS_EHIP, "end capture: %s", hipGetErrorString(e2));
+    if (e != hipSuccess) {
+        rc = set_error(MANS_EHIP, "capture streams: %s", hipGetErrorString(e));
     } else {
-        rc = set_error(MANS_EHIP, "begin capture: %s", hipGetErrorString(e));
+        e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            if (parts.size() == 1) {
+                const BatchPart& bp = parts[0];
+                rc = launch_pipeline(part_args(bp, opts->workspace), bp.ct, bp.it,
+                                     (uint32_t)bp.chunks, bp.ml, bp.mk, cs, nullptr);
+            } else {
+                rc = launch_parts(parts, opts->workspace, cs, ps);
+            }
+            hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
+            if (!rc && e2 != hipSuccess) rc = set_error(MANS_EHIP, "end capture: %s", hipGetErrorString(e2));
+        } else {
+            rc = set_error(MANS_EHIP, "begin capture: %s", hipGetErrorString(e));
+        }
     }
     if (!rc) {
         e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
         if (e != hipSuccess) rc = set_error(MANS_EHIP, "instantiate: %s", hipGetErrorString(e));
     }
-    (void)hipStreamDestroy(cs);
+    for (size_t p = 0; p < kMaxParts; ++p) {
+        if (ps.s[p]) (void)hipStreamDestroy(ps.s[p]);
+        if (ps.join[p]) (void)hipEventDestroy(ps.join[p]);
+    }
+    if (ps.fork) (void)hipEventDestroy(ps.fork);
+    if (cs) (void)hipStreamDestroy(cs);
     if (rc) {
         if (g->exec) (void)hipGraphExecDestroy(g->exec);
         if (g->graph) (void)hipGraphDestroy(g->graph);

@@ -84,3 +84,73 @@ def test_batch_graph_replays_match_eager():
     for i, (es, sym) in enumerate(items):
         assert np.array_equal(b.output(i).cpu().numpy()[::-1], sym)
     g.close()
+
+
+def test_large_batch_runs_as_parts():
+    """A batch of 800 streams decodes as concurrent parts (one pipeline per share of
+    the streams); every stream still equals its symbols, and a timed decode (one
+    pipeline, kernel events) gives the same bytes."""
+    import torch
+    from multians_amd import _lib
+    from multians_amd.decoder import DeviceBatch
+    print("case parts n=800", flush=True)
+    items = _mixed(800, 31, max_log2=17)
+    b = DeviceBatch([es for es, _ in items])
+    b.decode()
+    torch.cuda.synchronize()
+    assert b.status() == 0
+    split = b.out.clone()
+    for i, (es, sym) in enumerate(items):
+        assert np.array_equal(b.output(i).cpu().numpy()[::-1], sym), "stream %d" % i
+    b.out.zero_()
+    kt = _lib.KernelTimer()
+    b.decode(kernel_events=kt.ptr)
+    torch.cuda.synchronize()
+    kt.close()
+    assert b.status() == 0
+    assert torch.equal(b.out, split)
+
+
+def test_parts_report_table_errors():
+    """A rejected table in any part of a split batch reaches mans_decode_status: the
+    second and third largest streams go to the second and third parts."""
+    import torch
+    from multians_amd.decoder import DeviceBatch
+    print("case parts status", flush=True)
+    items = _mixed(800, 32, max_log2=16)
+    order = np.argsort([-es.n_symbols for es, _ in items], kind="stable")
+    streams = [es for es, _ in items]
+    for j in (order[1], order[2]):
+        es = streams[j]
+        bad = es.table.copy()
+        bad[0] = (bad[0] & 0x00FFFFFF) | (20 << 24)       # nbits 20: rejected on device
+        streams[j] = type(es)(es.units, es.first_bit, es.first_state, bad, es.L, es.n_symbols,
+                              es.counts)
+    b = DeviceBatch(streams)
+    b.decode()
+    torch.cuda.synchronize()
+    assert b.status() != 0
+    good = [i for i in range(len(items)) if i not in (order[1], order[2])]
+    for i in good[::40]:
+        assert np.array_equal(b.output(i).cpu().numpy()[::-1], items[i][1]), "stream %d" % i
+
+
+def test_parts_graph_replays_match_eager():
+    """The graph of a split batch (parts forked onto private streams inside the
+    capture) replays to the eager result."""
+    import torch
+    from multians_amd.decoder import DeviceBatch
+    print("case parts graph", flush=True)
+    items = _mixed(600, 33, max_log2=15)
+    b = DeviceBatch([es for es, _ in items])
+    b.decode()
+    torch.cuda.synchronize()
+    eager = b.out.clone()
+    g = b.graph()
+    for rep in range(3):
+        b.out.zero_()
+        g.launch()
+        torch.cuda.synchronize()
+        assert b.status() == 0
+        assert torch.equal(b.out, eager), "replay %d" % rep
+    g.close()
