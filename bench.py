@@ -178,6 +178,45 @@ def bench_single(args, torch):
     return per, items
 
 
+def bench_tables(args, torch):
+    """Decoder tables built on the device for the first --tables streams of the batched
+    configuration (one mans_build_tables_device call, symbols resident), against the host
+    codec (threaded histogram, normalisation, spread) on the same streams; every device
+    table is checked equal to the host one."""
+    from multians_amd import _lib
+    specs = batch_specs(args.streams)[:args.tables]
+    syms = [_lib.gen_symbols(seed, lam, n, threads=THREADS) for (seed, lam, n, L) in specs]
+    jobs = [(torch.from_numpy(s).cuda(), L, torch.zeros(L, dtype=torch.int32, device="cuda"), None)
+            for s, (_, _, _, L) in zip(syms, specs)]
+    status = _lib.build_tables_device(jobs)                   # warm-up + check below
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    ts = []
+    for _ in range(3):
+        a.record()
+        _lib.build_tables_device(jobs)
+        b.record()
+        torch.cuda.synchronize()
+        ts.append(a.elapsed_time(b))
+    t0 = time.time()
+    host = []
+    for s, (_, _, _, L) in zip(syms, specs):
+        c = _lib.normalize_counts(_lib.histogram(s, threads=THREADS), L)
+        host.append(_lib.build_tables(c, L, want_enc=False)[1])
+    host_ms = (time.time() - t0) * 1e3
+    same = all(st == 0 and np.array_equal(j[2].cpu().numpy().view(np.uint32), h)
+               for st, j, h in zip(status, jobs, host))
+    if not same:
+        raise SystemExit("device tables differ from the host codec's")
+    out = {"tables": len(jobs), "symbols": int(sum(len(s) for s in syms)), "L": 4096,
+           "device_ms": round(min(ts), 3), "host_ms": round(host_ms, 1),
+           "host_threads": THREADS, "identical": True,
+           "timing": "device: HIP events around one mans_build_tables_device call (job "
+                     "upload included); host: wall clock of histogram + normalise + spread"}
+    log("tables", json.dumps(out))
+    return out
+
+
 def bench_stress(args, torch):
     """Stress configuration: one stream of 2^32 uint8 symbols (4 GiB, 64-bit offsets) for
     lambda = 0.002 (symbol 255 holds 60% of the mass) and lambda = 3.0 (ratio > 7), 4096
@@ -326,6 +365,8 @@ def main():
                     help="duration of the served run (0 = skip)")
     ap.add_argument("--serve-pool", type=int, default=1000,
                     help="distinct streams the served requests draw from")
+    ap.add_argument("--tables", type=int, default=0, metavar="N",
+                    help="also build decoder tables on the device for the first N streams")
     ap.add_argument("--stress", action="store_true",
                     help="also decode two 2^32-symbol streams (lambda 0.002 and 3.0)")
     ap.add_argument("--no-plain", action="store_true",
@@ -383,6 +424,8 @@ def main():
     if not args.no_single:
         per, _ = bench_single(args, torch)
         res["single_stream"] = per
+    if args.tables:
+        res["device_tables"] = bench_tables(args, torch)
     if args.stress:
         res["stress"] = bench_stress(args, torch)
     if args.serve_seconds > 0:

@@ -221,6 +221,30 @@ int mans_make_stream(uint64_t seed, double lambda, uint64_t n, uint32_t L, uint8
                      uint32_t* units_out, uint64_t units_cap, uint64_t* n_units,
                      uint32_t* first_bit, uint32_t* first_state, uint32_t* dec_out);
 
+/* ------------------------------------------------ decoder tables on the device */
+/* Builds decoder tables where the symbols already are: histogram of each job's symbols
+ * -> counts normalised to L -> spread -> decoder table (L u32, CUHDCodetableItem
+ * layout), the same bytes as mans_histogram + mans_normalize_counts +
+ * mans_build_tables.  Replaces the host-side ANSTableGenerator path
+ * (src/ans_table_generator.cc:61-95, :164-288); the reference has no device form.
+ * Enqueued on `stream`; nothing is allocated.  A job that cannot be normalised (no
+ * symbols, more distinct symbols than states) leaves its table unwritten and a nonzero
+ * code for mans_table_status. */
+typedef struct mans_table_job {
+    const void* symbols;    /* device, u8[n] */
+    uint64_t n;             /* 1 <= n < 2^50 */
+    uint32_t L;             /* power of two in [2, 16384] */
+    uint32_t reserved;
+    void* table;            /* device, u32[L]: decoder table */
+    void* counts;           /* device, u32[256] normalised counts, or NULL */
+} mans_table_job;
+int mans_table_workspace_bytes(size_t n_jobs, uint64_t* bytes);
+int mans_build_tables_device(const mans_table_job* jobs, size_t n_jobs, void* workspace,
+                             uint64_t workspace_bytes, void* stream);
+/* status[j] = 0 (built), 1 (no symbols), 2 (more distinct symbols than L), 3 (failed);
+ * waits for the stream. */
+int mans_table_status(const void* workspace, size_t n_jobs, void* stream, uint32_t* status);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,11 @@ class BatchOpts(C.Structure):
                 ("kernel_events", C.c_void_p)]
 
 
+class TableJob(C.Structure):
+    _fields_ = [("symbols", C.c_void_p), ("n", C.c_uint64), ("L", C.c_uint32),
+                ("reserved", C.c_uint32), ("table", C.c_void_p), ("counts", C.c_void_p)]
+
+
 class GeometryRule(C.Structure):
     _fields_ = [("L_max", C.c_uint32), ("chunk_units", C.c_uint32), ("spu_max", C.c_double),
                 ("units_max", C.c_uint64)]
@@ -76,6 +81,7 @@ EXPORTS = [
     "mans_normalize_counts", "mans_build_tables", "mans_max_encoded_units", "mans_encode",
     "mans_make_stream", "mans_batch_graph_create", "mans_graph_launch", "mans_graph_destroy",
     "mans_set_geometry_rules", "mans_chunk_units", "mans_histogram", "mans_set_wait_polls",
+    "mans_table_workspace_bytes", "mans_build_tables_device", "mans_table_status",
 ]
 
 _lib = None
@@ -132,6 +138,9 @@ def lib():
         "mans_chunk_units": ([u64, u64, u32, C.POINTER(u32)], i32),
         "mans_histogram": ([vp, u64, vp, i32], i32),
         "mans_set_wait_polls": ([u32], i32),
+        "mans_table_workspace_bytes": ([C.c_size_t, C.POINTER(u64)], i32),
+        "mans_build_tables_device": ([C.POINTER(TableJob), C.c_size_t, vp, u64, vp], i32),
+        "mans_table_status": ([vp, C.c_size_t, vp, C.POINTER(u32)], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(l, name)
@@ -239,6 +248,27 @@ def set_geometry_rules(rules):
         arr[i].L_max, arr[i].spu_max = int(L_max), float(spu_max)
         arr[i].units_max, arr[i].chunk_units = int(units_max), int(chunk)
     check(lib().mans_set_geometry_rules(arr, len(rules)))
+
+
+def build_tables_device(jobs, stream=None):
+    """Decoder tables on the device: jobs = [(symbols u8 tensor, L, table u32 tensor,
+    counts u32 tensor or None)], all on one device.  Returns the per-job status list
+    (0 = built); waits for the stream."""
+    import torch
+    st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    arr = (TableJob * len(jobs))()
+    for i, (sym, L, table, counts) in enumerate(jobs):
+        arr[i].symbols, arr[i].n, arr[i].L = sym.data_ptr(), sym.numel(), int(L)
+        arr[i].table = table.data_ptr()
+        arr[i].counts = counts.data_ptr() if counts is not None else None
+    b = C.c_uint64()
+    check(lib().mans_table_workspace_bytes(len(jobs), C.byref(b)))
+    ws = torch.empty(b.value, dtype=torch.uint8, device=jobs[0][0].device)
+    check(lib().mans_build_tables_device(arr, len(jobs), C.c_void_p(ws.data_ptr()), b.value,
+                                         C.c_void_p(st)))
+    status = (C.c_uint32 * len(jobs))()
+    check(lib().mans_table_status(C.c_void_p(ws.data_ptr()), len(jobs), C.c_void_p(st), status))
+    return list(status)
 
 
 WAIT_POLLS_DEFAULT = 1 << 20
