@@ -139,11 +139,15 @@ typedef struct mans_stream_desc {
     uint32_t reserved;
 } mans_stream_desc;
 
+/* mans_batch_opts.flags: decode as one pipeline even when the batch is large enough to
+ * be split into concurrent parts (a caller that keeps several batches in flight itself). */
+#define MANS_BATCH_ONE_PIPELINE 1u
+
 typedef struct mans_batch_opts {
     void* workspace;            /* device, >= mans_batch_workspace_bytes(...) */
     uint64_t workspace_bytes;
     void* stream;
-    uint32_t flags;
+    uint32_t flags;             /* MANS_BATCH_* */
     uint32_t reserved;
     void** kernel_events;       /* optional, as in mans_decode_args */
 } mans_batch_opts;

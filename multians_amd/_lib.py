@@ -53,6 +53,9 @@ class BatchOpts(C.Structure):
                 ("kernel_events", C.c_void_p)]
 
 
+MANS_BATCH_ONE_PIPELINE = 1
+
+
 class TableJob(C.Structure):
     _fields_ = [("symbols", C.c_void_p), ("n", C.c_uint64), ("L", C.c_uint32),
                 ("reserved", C.c_uint32), ("table", C.c_void_p), ("counts", C.c_void_p)]

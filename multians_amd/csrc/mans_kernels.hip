@@ -1465,6 +1465,19 @@ struct PartStreams {
 static std::mutex g_parts_mu;
 static std::vector<std::unique_ptr<PartStreams>> g_parts;
 
+// Creates the part streams of ps on first use: a decode that runs as one pipeline never
+// needs them, and every stream takes a share of the process's hardware queues.
+static int part_streams_ready(PartStreams* ps) {
+    if (ps->s[0]) return MANS_OK;
+    hipError_t e = hipSuccess;
+    for (size_t p = 0; p < kMaxParts && e == hipSuccess; ++p) {
+        e = hipStreamCreateWithFlags(&ps->s[p], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ps->join[p], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) return set_error(MANS_EHIP, "part streams: %s", hipGetErrorString(e));
+    return MANS_OK;
+}
+
 static int part_streams(hipStream_t caller, PartStreams** out) {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
@@ -1479,10 +1492,6 @@ static int part_streams(hipStream_t caller, PartStreams** out) {
     ps->caller = caller;
     ps->dev = dev;
     e = hipEventCreateWithFlags(&ps->fork, hipEventDisableTiming);
-    for (size_t p = 0; p < kMaxParts && e == hipSuccess; ++p) {
-        e = hipStreamCreateWithFlags(&ps->s[p], hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&ps->join[p], hipEventDisableTiming);
-    }
     for (int b = 0; b < 2 && e == hipSuccess; ++b)
         e = hipEventCreateWithFlags(&ps->copied[b], hipEventDisableTiming);
     if (e != hipSuccess) return set_error(MANS_EHIP, "part streams: %s", hipGetErrorString(e));
@@ -1553,11 +1562,14 @@ static int upload_parts(const std::vector<BatchPart>& parts, void* workspace, hi
     ps->next ^= 1;
     e = hipEventSynchronize(ps->copied[b]);          // previous copies from this buffer done
     if (e == hipSuccess && ps->cap[b] < total) {
+        // grow geometrically: freeing pinned memory synchronises the device, so it must
+        // stay rare on a stream that serves many batches
+        const size_t cap = std::max<size_t>({total, 2 * ps->cap[b], (size_t)1 << 20});
         if (ps->pin[b]) e = hipHostFree(ps->pin[b]);
         ps->pin[b] = nullptr;
         ps->cap[b] = 0;
-        if (e == hipSuccess) e = hipHostMalloc((void**)&ps->pin[b], total, hipHostMallocDefault);
-        if (e == hipSuccess) ps->cap[b] = total;
+        if (e == hipSuccess) e = hipHostMalloc((void**)&ps->pin[b], cap, hipHostMallocDefault);
+        if (e == hipSuccess) ps->cap[b] = cap;
     }
     size_t o = 0;
     for (size_t p = 0; p < parts.size() && e == hipSuccess; ++p) {
@@ -1587,7 +1599,7 @@ extern "C" int mans_decode_batch(const mans_stream_desc* descs, size_t n,
                                  const mans_batch_opts* opts, mans_batch_stats* stats) {
     if (!descs || !n || !opts) return set_error(MANS_EINVAL, "bad batch");
     // per-kernel timing events describe one pipeline, so a timed decode is not split
-    const size_t k = opts->kernel_events ? 1 : n_parts(n);
+    const size_t k = (opts->kernel_events || (opts->flags & MANS_BATCH_ONE_PIPELINE)) ? 1 : n_parts(n);
     std::vector<BatchPart> parts;
     uint64_t need = 0;
     int rc = plan_parts(descs, n, k, parts, &need);
@@ -1608,7 +1620,8 @@ extern "C" int mans_decode_batch(const mans_stream_desc* descs, size_t n,
             rc = launch_pipeline(part_args(bp, opts->workspace), bp.ct, bp.it, (uint32_t)bp.chunks,
                                  bp.ml, bp.mk, st, opts->kernel_events);
         } else {
-            rc = launch_parts(parts, opts->workspace, st, *ps);
+            rc = part_streams_ready(ps);
+            if (!rc) rc = launch_parts(parts, opts->workspace, st, *ps);
         }
     }
     if (rc) return rc;

@@ -219,6 +219,9 @@ class Server:
         o.workspace = base + out_bytes
         o.workspace_bytes = ws_bytes
         o.stream = hs.cuda_stream
+        # one pipeline per batch: the server already keeps max_inflight batches in
+        # flight on its own streams
+        o.flags = _lib.MANS_BATCH_ONE_PIPELINE
         _lib.check(self.lib.mans_decode_batch(descs, len(reqs), C.byref(o), None))
         check = None
         if check_every and self.pool.sym is not None:
