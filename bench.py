@@ -100,7 +100,8 @@ def bench_batched(args, torch):
     log("batch: %d streams, %.2f GB symbols, %.2f GB compressed, generated in %.1fs" %
         (len(items), total_sym / 1e9, 4.0 * total_units / 1e9, time.time() - t0))
     b = DeviceBatch([es for es, _ in items])
-    b.decode()
+    dec = (lambda: b.decode(one_pipeline=True)) if args.one_pipeline else b.decode
+    dec()
     torch.cuda.synchronize()
     st = b.status()
     bad = verify_batch(b, items, torch)
@@ -114,8 +115,8 @@ def bench_batched(args, torch):
     kms = kt.times_ms()
     kt.close()
     for _ in range(args.warmup):
-        b.decode()
-    ms_batch = time_events(torch, b.decode, args.steps)
+        dec()
+    ms_batch = time_events(torch, dec, args.steps)
     res = {"ms_batch": ms_batch, "ms_plain": None, "kernels_ms": kms,
            "total_symbols": total_sym, "total_units": total_units, "n_streams": len(items)}
     if args.no_plain:
@@ -365,6 +366,9 @@ def main():
                     help="duration of the served run (0 = skip)")
     ap.add_argument("--serve-pool", type=int, default=1000,
                     help="distinct streams the served requests draw from")
+    ap.add_argument("--one-pipeline", action="store_true",
+                    help="decode the batch as one pipeline (no concurrent parts), as the "
+                         "per-kernel timing decode does; for per-dispatch profiles")
     ap.add_argument("--tables", type=int, default=0, metavar="N",
                     help="also build decoder tables on the device for the first N streams")
     ap.add_argument("--stress", action="store_true",
@@ -405,7 +409,10 @@ def main():
            "data": "synthetic (seeded Exp(lambda) symbols, tables and encoding on the host)",
            "config": {"workload": "batched: %d tANS streams, uint8 sizes log-uniform 4 KiB-16 MiB, "
                                   "lambda ~ LogU(0.005, 2.0), own 4096-state table each, one "
-                                  "batched launch sequence" % bres["n_streams"],
+                                  "mans_decode_batch call per step (%s; kernels_ms and the "
+                                  "roofline are one pipeline's kernels)" %
+                                  (bres["n_streams"], "one pipeline" if args.one_pipeline else
+                                   "up to three concurrent pipelines"),
                       "streams": bres["n_streams"], "total_symbols": bres["total_symbols"],
                       "compressed_bytes": int(units4), "L": 4096},
            "batched_GBps": round(value, 3),

@@ -144,11 +144,13 @@ class DeviceBatch:
         _lib.check(_lib.lib().mans_batch_workspace_bytes(self.descs, n, C.byref(b)))
         self.ws = torch.zeros(b.value, dtype=torch.uint8, device=device)
 
-    def decode(self, stream=None, kernel_events=None):
+    def decode(self, stream=None, kernel_events=None, one_pipeline=False):
+        """one_pipeline: no split into concurrent parts (MANS_BATCH_ONE_PIPELINE)."""
         torch = _torch()
         st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         o = _lib.BatchOpts()
         o.kernel_events = kernel_events
+        o.flags = _lib.MANS_BATCH_ONE_PIPELINE if one_pipeline else 0
         o.workspace = self.ws.data_ptr()
         o.workspace_bytes = self.ws.numel()
         o.stream = st
