@@ -39,18 +39,20 @@ __global__ void k_tab_clear(JobWork* jw, uint32_t n_jobs) {
     if (threadIdx.x == 0) jw[j].status = 0u;
 }
 
-// Histogram of job blockIdx.y's symbols, slice blockIdx.x (kHistSlice symbols): four
-// per-wave LDS histograms (less atomic contention on skewed data), folded into the job's
-// u64 histogram.
+// Histogram of job blockIdx.y's symbols, slice blockIdx.x (kHistSlice symbols): sixteen
+// LDS histograms, one per lane residue mod 16, so that on skewed data (one symbol holding
+// most of the mass) at most four lanes of a wave add to the same counter at once; folded
+// into the job's u64 histogram.
+constexpr uint32_t kHistCopies = 16;
 __global__ __launch_bounds__(kHistWG) void k_tab_hist(const mans_table_job* jobs, JobWork* jw) {
-    __shared__ uint32_t h[4][256];
+    __shared__ uint32_t h[kHistCopies][256];
     const mans_table_job jb = jobs[blockIdx.y];
     const uint64_t b = (uint64_t)blockIdx.x * kHistSlice;
     if (b >= jb.n) return;                       // uniform
     const uint64_t e = min(jb.n, b + kHistSlice);
-    for (uint32_t i = threadIdx.x; i < 4u * 256u; i += kHistWG) (&h[0][0])[i] = 0u;
+    for (uint32_t i = threadIdx.x; i < kHistCopies * 256u; i += kHistWG) (&h[0][0])[i] = 0u;
     __syncthreads();
-    uint32_t* mine = h[threadIdx.x >> 6];
+    uint32_t* mine = h[threadIdx.x % kHistCopies];
     const uint8_t* s = (const uint8_t*)jb.symbols;
     // 16-byte blocks of the middle (16-byte aligned addresses), single bytes at the edges
     const uint64_t mis = (uint64_t)((uintptr_t)(s + b) & 15u);
@@ -69,7 +71,9 @@ __global__ __launch_bounds__(kHistWG) void k_tab_hist(const mans_table_job* jobs
     }
     for (uint64_t i = a1 + threadIdx.x; i < e; i += kHistWG) atomicAdd(&mine[s[i]], 1u);
     __syncthreads();
-    const uint32_t c = h[0][threadIdx.x] + h[1][threadIdx.x] + h[2][threadIdx.x] + h[3][threadIdx.x];
+    uint32_t c = 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < kHistCopies; ++k) c += h[k][threadIdx.x];
     if (c) atomicAdd((unsigned long long*)&jw[blockIdx.y].hist[threadIdx.x], (unsigned long long)c);
 }
 
