@@ -161,7 +161,16 @@ typedef struct mans_batch_stats {
     uint32_t status;
 } mans_batch_stats;
 
+/* Workspace for mans_decode_batch / mans_batch_graph_create over these streams (split or
+ * one pipeline). */
 int mans_batch_workspace_bytes(const mans_stream_desc* descs, size_t n, uint64_t* bytes);
+/* Decodes every stream of the batch, each with its own table, enqueued on opts->stream.
+ * Batches of 512 streams or more run as two or three concurrent pipelines on private
+ * streams, joined back into opts->stream (unless MANS_BATCH_ONE_PIPELINE or
+ * kernel_events).  The plans are copied from pinned host buffers owned by the library
+ * (two per caller stream), so this call must not be captured into a graph; use
+ * mans_batch_graph_create for that.  Returns after enqueueing; stats (optional) are
+ * host-side totals. */
 int mans_decode_batch(const mans_stream_desc* descs, size_t n, const mans_batch_opts* opts,
                       mans_batch_stats* stats);
 
