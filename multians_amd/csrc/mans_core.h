@@ -261,6 +261,10 @@ struct StreamPlan {
     uint64_t ck_base;    // global index of checkpoint 0
     uint32_t task_chunk; // first spec/walk task
     uint32_t task_item;  // first write task
+    uint32_t warm;       // spec warm-up: units decoded before each chunk c >= 1 (multiple
+                         // of K), so the spec path is likely on the true one at the chunk
+                         // start; 0 = none
+    uint32_t pad;
 };
 
 // Workspace arrays (global indices).
@@ -326,7 +330,15 @@ MANS_HD void lane_spec(const StreamPlan& sp, const Work& w, const Tab& tab, uint
         off0 = 0u;
         xi = sp.first_state - L;   // guessed state (the reference guesses the same)
     }
-    bcursor_init(cur, sp.units, sp.n_units, u0, off0);
+    // warm-up: a chunk c >= 1 starts its guessed decode warm units earlier; the first
+    // step starting in unit u0 gives the chunk's first checkpoint
+    const uint32_t wu = c == 0u ? 0u : (sp.warm < u0 ? sp.warm : u0);
+    bcursor_init(cur, sp.units, sp.n_units, u0 - wu, off0);
+    while (cur.u < u0) {
+        uint32_t crossed;
+        bstep(tab, cur, xi, crossed);
+    }
+    off0 = cur.o + (cur.u - u0) * 32u;     // = cur.o: a step crosses one unit at most
     const uint32_t gc = sp.chunk_base + c;
     const uint32_t kmask = sp.K - 1u;
     w.ck[ck_slot(sp, u0 >> sp.log2K)] = pack_entry(xi, off0);

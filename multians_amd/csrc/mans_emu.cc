@@ -22,14 +22,12 @@ uint32_t ilog2(uint32_t v) { uint32_t r = 0; while ((1u << (r + 1)) <= v) ++r; r
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-// Returns 0, -4 for a rejected table.  stats (optional, 8 u64): n_chunks, n_ck,
-// walk failures (round 1), fixup rounds, total re-walks, merges, 0, 0.
-int mans_emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint64_t n_out,
-                    const uint32_t* table, uint32_t L, uint32_t first_state, uint32_t first_bit,
-                    uint32_t S, uint32_t K, uint64_t* stats, uint32_t* ck_entries_out,
-                    uint64_t* ck_offsets_out) {
+int emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint64_t n_out,
+               const uint32_t* table, uint32_t L, uint32_t first_state, uint32_t first_bit,
+               uint32_t S, uint32_t K, uint32_t warm, uint64_t* stats, uint32_t* ck_entries_out,
+               uint64_t* ck_offsets_out) {
     if (!n_out) return 0;
     std::vector<uint32_t> tab(L);
     int bad = 0, cyc = 0;
@@ -42,6 +40,7 @@ int mans_emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint6
     sp.table = table;
     sp.n_out = n_out;
     sp.n_units = (uint32_t)n_units;
+    sp.warm = warm;
     sp.L = L;
     sp.first_state = first_state;
     sp.first_bit = first_bit;
@@ -166,6 +165,29 @@ int mans_emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint6
         stats[4] = st_rewalks; stats[5] = st_merges; stats[6] = 0; stats[7] = 0;
     }
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Returns 0, -4 for a rejected table.  stats (optional, 8 u64): n_chunks, n_ck,
+// walk failures (round 1), fixup rounds, total re-walks, merges, 0, 0.
+int mans_emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint64_t n_out,
+                    const uint32_t* table, uint32_t L, uint32_t first_state, uint32_t first_bit,
+                    uint32_t S, uint32_t K, uint64_t* stats, uint32_t* ck_entries_out,
+                    uint64_t* ck_offsets_out) {
+    return emu_decode(units, n_units, out, n_out, table, L, first_state, first_bit, S, K, 0u,
+                      stats, ck_entries_out, ck_offsets_out);
+}
+
+// The same with a spec warm-up of `warm` units (a multiple of K) before each chunk.
+int mans_emu_decode_warm(const uint32_t* units, uint64_t n_units, uint8_t* out, uint64_t n_out,
+                         const uint32_t* table, uint32_t L, uint32_t first_state,
+                         uint32_t first_bit, uint32_t S, uint32_t K, uint32_t warm,
+                         uint64_t* stats) {
+    return emu_decode(units, n_units, out, n_out, table, L, first_state, first_bit, S, K, warm,
+                      stats, nullptr, nullptr);
 }
 
 }  // extern "C"

@@ -112,3 +112,21 @@ def test_chained_walks_many_ranges(emu, lam, S, K):
     _, entries, counts, _, _ = ol.decode_seq(st, S=K)
     assert np.array_equal(ent, entries)
     assert np.array_equal(off, np.concatenate([[0], np.cumsum(counts.astype(np.uint64))[:-1]]))
+
+
+@pytest.mark.parametrize("warm", [16, 256, 2048])
+@pytest.mark.parametrize("lam,S", [(0.01, 256), (1.0, 512), (0.1, 64)])
+def test_spec_warmup(emu, lam, S, warm):
+    """A spec that starts `warm` units before each chunk (its first checkpoint is then the
+    warm-up path's state) gives the same output."""
+    emu.mans_emu_decode_warm.argtypes = [_u32p, C.c_uint64, _u8p, C.c_uint64, _u32p, C.c_uint32,
+                                         C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                         C.c_uint32, _u64p]
+    sym = ol.gen_symbols(44, lam, 600_000)
+    st, _ = ol.encode(sym, L=4096)
+    out = np.zeros(len(sym), np.uint8)
+    stats = np.zeros(8, np.uint64)
+    rc = emu.mans_emu_decode_warm(st.units, len(st.units), out, len(sym), st.table, st.L,
+                                  st.first_state, st.first_bit, S, 16, warm, stats)
+    assert rc == 0
+    assert np.array_equal(out[::-1], sym)

@@ -87,3 +87,30 @@ def test_range_wait_runs_out(polls):
     assert np.array_equal(got, ref)
     assert np.array_equal(got[::-1], sym)
     assert np.array_equal(got2, ref2)
+
+
+@pytest.mark.parametrize("spec", [(2026181085674, 0.25656187489628735, 1_105_978),
+                                  (2026181087171, 0.17296609080379266, 1_971_802)])
+def test_walk_enters_chunk_as_another_leaves(spec):
+    """Streams of one 256-chunk range whose spec warm-up puts most chunk starts on the
+    true path: a walk that did not merge in its own chunk enters the next one and merges
+    at its first checkpoint in the same round in which that chunk's own walk closes it.
+    Both write the chunk's record; the closing one must land first.  Decoded four times
+    (the order of the two writes is timing-dependent if it is not enforced)."""
+    import torch
+    from multians_amd import _lib
+    from multians_amd.decoder import DeviceStream, EncodedStream
+    seed, lam, n = spec
+    print(f"case walk-enter seed={seed}", flush=True)
+    units, fb, fs, dec, sym = _lib.make_stream(seed, lam, n, 4096, True)
+    ds = DeviceStream(EncodedStream(units, fb, fs, dec, 4096, n))
+    st = ol.Stream(units, fb, fs, dec, 4096, n)
+    ref, _, _, _, _ = ol.decode_seq(st)
+    for _ in range(4):
+        ds.ws.zero_()
+        out = ds.decode()
+        torch.cuda.synchronize()
+        assert ds.status() == 0
+        got = out[:n].cpu().numpy()
+        assert np.array_equal(got, ref)
+        assert np.array_equal(got[::-1], sym)
