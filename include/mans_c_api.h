@@ -87,6 +87,22 @@ int mans_event_elapsed_ms(void* start, void* stop, float* ms); /* synchronises s
  * Output: out[0..n_out) symbols in decode order (reverse of the encoder input), the
  *   first n_out steps of the decoder started at (first_bit, first_state).
  */
+/* Sync point of a stream at the start of sync chunk c (units [c*chunk_units, ...)): the
+ * decoder state (x - L) and the bit offset, within unit c*chunk_units, of the first step
+ * that starts in that unit, and the number of symbols decoded before that step.  One
+ * record per chunk, ceil(n_units / chunk_units) records (mans_sync_point_count).  A
+ * decode writes them (index_out) once its sync phase has found the true path; a later
+ * decode of the same stream with the same chunk_units starts its speculative chunk
+ * decodes there (index_in) instead of at a guessed state, and a window of the output can
+ * be decoded alone from the chunk that holds its first symbol (mans_stream_range).
+ * No counterpart in the reference, whose sync points (CUHDSubsequenceSyncPoint,
+ * include/cuhd_subsequence_sync_point.h:20-25) live in decoder memory for one call. */
+typedef struct mans_sync_point {
+    uint32_t state;             /* x - L, in [0, L) */
+    uint32_t bit;               /* in [0, 31]; chunk 0: 32 - first_bit (0..32) */
+    uint64_t symbols_before;
+} mans_sync_point;
+
 typedef struct mans_decode_args {
     const uint32_t* units;      /* device */
     uint64_t n_units;
@@ -104,6 +120,13 @@ typedef struct mans_decode_args {
     void* stream;               /* hipStream_t or NULL */
     void** kernel_events;       /* optional: MANS_N_KERNEL_EVENTS hipEvent_t recorded on
                                    `stream` around the kernels (timing), or NULL */
+    /* Optional sync points (device; both need chunk_units, a power of two in
+     * [16, 2^24], used as the sync-chunk length as given).  index_in: start the
+     * speculative decodes there; the decode still proves every chunk's start, so wrong
+     * records cost time, not correctness.  index_out: written with the stream's true
+     * sync points (may equal index_in). */
+    const mans_sync_point* index_in;
+    mans_sync_point* index_out;
 } mans_decode_args;
 
 /* kernel_events[i] is recorded before kernel i (0 init, 1 sync, 2 check, 3 fixup,
@@ -136,7 +159,9 @@ typedef struct mans_stream_desc {
     uint32_t L;
     uint32_t first_state;
     uint32_t first_bit;
-    uint32_t reserved;
+    uint32_t chunk_units;       /* sync-chunk length; 0 = tuned (then no index_in/out) */
+    const mans_sync_point* index_in;   /* optional, as in mans_decode_args */
+    mans_sync_point* index_out;        /* optional, as in mans_decode_args */
 } mans_stream_desc;
 
 /* mans_batch_opts.flags: decode as one pipeline even when the batch is large enough to
@@ -208,6 +233,27 @@ int mans_set_geometry_rules(const mans_geometry_rule* rules, size_t n);
 int mans_set_wait_polls(uint32_t polls);
 /* The chunk length the decoder picks for a stream (rules, else built-in). */
 int mans_chunk_units(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t* chunk_units);
+/* Sync points of a stream of n_units units with this chunk length (a power of two in
+ * [16, 2^24]): ceil(n_units / chunk_units). */
+int mans_sync_point_count(uint64_t n_units, uint32_t chunk_units, uint64_t* count);
+/* Checks `count` sync points (host copy) of a stream with L states: record 0 at symbol
+ * 0, states below L, bits at most 31 (32 in record 0), symbols_before non-decreasing.
+ * Decodes never need it (index_in records are masked into range and proven); a caller
+ * that takes sync points from storage runs it once before mans_stream_range. */
+int mans_sync_points_check(const mans_sync_point* index_host, uint64_t count, uint32_t L);
+/* Partial decode.  For the output symbols [first, last) (decode order) of a stream
+ * (`full`: units, n_units, n_out, table, L, first_state, first_bit, chunk_units) with
+ * sync points `index_host` (host copy, mans_sync_point_count records) and the same
+ * records on the device at `index_dev`: fills *sub with the sub-stream that starts at
+ * the sync point of the chunk holding symbol `first` and ends at the chunk holding
+ * symbol last - 1 (its units, n_units with one unit of look-ahead, n_out, first_state,
+ * first_bit, chunk_units, index_in = index_dev + that chunk; out and index_out are
+ * NULL: the caller sets out, sub->n_out bytes).  Decoding *sub yields the window at
+ * out + *skip.  The sub-stream trusts the record of its first chunk (the ones after it
+ * are proven as usual).  Host only; no reference counterpart. */
+int mans_stream_range(const mans_stream_desc* full, const mans_sync_point* index_host,
+                      const mans_sync_point* index_dev, uint64_t first, uint64_t last,
+                      mans_stream_desc* sub, uint64_t* skip);
 
 /* --------------------------------------------------------- host-side codec */
 /* Symbols s = min(255, floor(X)), X ~ Exp(lambda); element i depends only on (seed, i). */

@@ -33,7 +33,8 @@ class DecodeArgs(C.Structure):
                 ("chunk_units", C.c_uint32), ("checkpoint_units", C.c_uint32),
                 ("flags", C.c_uint32), ("workspace", C.c_void_p),
                 ("workspace_bytes", C.c_uint64), ("stream", C.c_void_p),
-                ("kernel_events", C.c_void_p)]
+                ("kernel_events", C.c_void_p), ("index_in", C.c_void_p),
+                ("index_out", C.c_void_p)]
 
 
 N_KERNEL_EVENTS = 7
@@ -44,7 +45,16 @@ class StreamDesc(C.Structure):
     _fields_ = [("units", C.c_void_p), ("n_units", C.c_uint64), ("out", C.c_void_p),
                 ("n_out", C.c_uint64), ("table", C.c_void_p), ("L", C.c_uint32),
                 ("first_state", C.c_uint32), ("first_bit", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("chunk_units", C.c_uint32), ("index_in", C.c_void_p),
+                ("index_out", C.c_void_p)]
+
+
+# mans_sync_point: one record per sync chunk (state = x - L, bit offset, symbols before)
+SYNC_POINT_DTYPE = np.dtype([("state", "<u4"), ("bit", "<u4"), ("symbols_before", "<u8")])
+
+
+class SyncPoint(C.Structure):
+    _fields_ = [("state", C.c_uint32), ("bit", C.c_uint32), ("symbols_before", C.c_uint64)]
 
 
 class BatchOpts(C.Structure):
@@ -85,6 +95,7 @@ EXPORTS = [
     "mans_make_stream", "mans_batch_graph_create", "mans_graph_launch", "mans_graph_destroy",
     "mans_set_geometry_rules", "mans_chunk_units", "mans_histogram", "mans_set_wait_polls",
     "mans_table_workspace_bytes", "mans_build_tables_device", "mans_table_status",
+    "mans_sync_point_count", "mans_stream_range", "mans_sync_points_check",
 ]
 
 _lib = None
@@ -144,6 +155,10 @@ def lib():
         "mans_table_workspace_bytes": ([C.c_size_t, C.POINTER(u64)], i32),
         "mans_build_tables_device": ([C.POINTER(TableJob), C.c_size_t, vp, u64, vp], i32),
         "mans_table_status": ([vp, C.c_size_t, vp, C.POINTER(u32)], i32),
+        "mans_sync_point_count": ([u64, u32, C.POINTER(u64)], i32),
+        "mans_sync_points_check": ([vp, u64, u32], i32),
+        "mans_stream_range": ([C.POINTER(StreamDesc), vp, vp, u64, u64, C.POINTER(StreamDesc),
+                               C.POINTER(u64)], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(l, name)
@@ -213,6 +228,28 @@ def make_stream(seed, lam, n, L, keep_symbols=True):
     check(lib().mans_make_stream(seed, lam, n, L, _ptr(sym) if keep_symbols else None, _ptr(units),
                                  len(units), C.byref(nu), C.byref(fb), C.byref(fs), _ptr(dec)))
     return units[:nu.value], fb.value, fs.value, dec, sym
+
+
+def sync_point_count(n_units, chunk_units):
+    c = C.c_uint64()
+    check(lib().mans_sync_point_count(n_units, chunk_units, C.byref(c)))
+    return c.value
+
+
+def sync_points_check(index_host, L):
+    index_host = np.ascontiguousarray(index_host, SYNC_POINT_DTYPE)
+    check(lib().mans_sync_points_check(_ptr(index_host), len(index_host), L))
+
+
+def stream_range(full, index_host, index_dev_ptr, first, last):
+    """mans_stream_range: (sub-stream StreamDesc, skip) for the output window [first, last)
+    of the stream `full` (a StreamDesc with chunk_units), given its sync points on the host
+    (SYNC_POINT_DTYPE array) and their device address."""
+    index_host = np.ascontiguousarray(index_host, SYNC_POINT_DTYPE)
+    sub, skip = StreamDesc(), C.c_uint64()
+    check(lib().mans_stream_range(C.byref(full), _ptr(index_host), C.c_void_p(index_dev_ptr),
+                                  first, last, C.byref(sub), C.byref(skip)))
+    return sub, skip.value
 
 
 class KernelTimer:

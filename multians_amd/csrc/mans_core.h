@@ -242,6 +242,14 @@ MANS_HD uint32_t bstep(const Tab& tab, BCursor& c, uint32_t& xi, uint32_t& cross
 }
 
 // ------------------------------------------------------------------ layouts
+// Sync point record (layout of the C ABI's mans_sync_point): the true (state, bit
+// offset) at a chunk start and the symbols decoded before it.
+struct SyncPoint {
+    uint32_t state;
+    uint32_t bit;
+    uint64_t before;
+};
+
 // Per-stream parameters (device copy of one mans_stream_desc plus the plan).
 struct StreamPlan {
     const uint32_t* units;
@@ -265,6 +273,8 @@ struct StreamPlan {
                          // of K), so the spec path is likely on the true one at the chunk
                          // start; 0 = none
     uint32_t pad;
+    const SyncPoint* index_in;   // optional: spec of chunk c >= 1 starts at index_in[c]
+    SyncPoint* index_out;        // optional: true sync points written after the scan
 };
 
 // Workspace arrays (global indices).
@@ -326,6 +336,10 @@ MANS_HD void lane_spec(const StreamPlan& sp, const Work& w, const Tab& tab, uint
     if (c == 0u) {
         off0 = 32u - sp.first_bit;
         xi = sp.first_state - L;
+    } else if (sp.index_in) {
+        // recorded sync point (masked into range: a wrong record only costs walks)
+        off0 = sp.index_in[c].bit & 31u;
+        xi = sp.index_in[c].state & (L - 1u);
     } else {
         off0 = 0u;
         xi = sp.first_state - L;   // guessed state (the reference guesses the same)
@@ -500,6 +514,24 @@ MANS_HD uint32_t chunk_count(const StreamPlan& sp, const Work& w, uint32_t c) {
     uint32_t m = w.mq[gc];
     if (m == kNone) return w.wtotal[gc];
     return w.wcm[gc] + (w.own_total[gc] - w.ckc[ck_slot(sp, m)]);
+}
+
+// True sync point of chunk c (valid after the fixup rounds and the scan): chunk 0 from
+// the stream start, chunk c >= 1 from its recorded walk start (then the exit of chunk
+// c-1 on the true path); symbols before it from the scanned chunk offsets.
+MANS_HD SyncPoint chunk_sync_point(const StreamPlan& sp, const Work& w, uint32_t c) {
+    const uint32_t gc = sp.chunk_base + c;
+    SyncPoint p;
+    if (c == 0u) {
+        p.state = sp.first_state - sp.L;
+        p.bit = 32u - sp.first_bit;
+    } else {
+        const uint32_t e = w.wentry[gc];
+        p.state = entry_state(e);
+        p.bit = entry_off(e);
+    }
+    p.before = w.coff[gc] - w.coff[sp.chunk_base];
+    return p;
 }
 
 // True entry and chunk-relative step count at checkpoint q.

@@ -27,7 +27,8 @@ namespace {
 int emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint64_t n_out,
                const uint32_t* table, uint32_t L, uint32_t first_state, uint32_t first_bit,
                uint32_t S, uint32_t K, uint32_t warm, uint64_t* stats, uint32_t* ck_entries_out,
-               uint64_t* ck_offsets_out) {
+               uint64_t* ck_offsets_out, const SyncPoint* index_in = nullptr,
+               SyncPoint* index_out = nullptr) {
     if (!n_out) return 0;
     std::vector<uint32_t> tab(L);
     int bad = 0, cyc = 0;
@@ -41,6 +42,7 @@ int emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint64_t n
     sp.n_out = n_out;
     sp.n_units = (uint32_t)n_units;
     sp.warm = warm;
+    sp.index_in = index_in;
     sp.L = L;
     sp.first_state = first_state;
     sp.first_bit = first_bit;
@@ -142,6 +144,9 @@ int emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint64_t n
         coff[c] = run;
         run += chunk_count(sp, w, c);
     }
+    // k_index
+    if (index_out)
+        for (uint32_t c = 0; c < nc; ++c) index_out[c] = chunk_sync_point(sp, w, c);
     // k_write
     for (uint32_t q = 0; q < nq; ++q) {
         uint32_t entry, e2;
@@ -188,6 +193,16 @@ int mans_emu_decode_warm(const uint32_t* units, uint64_t n_units, uint8_t* out, 
                          uint64_t* stats) {
     return emu_decode(units, n_units, out, n_out, table, L, first_state, first_bit, S, K, warm,
                       stats, nullptr, nullptr);
+}
+
+// The same from recorded sync points (index_in, n_chunks records, or NULL) and/or
+// writing the true ones (index_out, or NULL); no warm-up.
+int mans_emu_decode_index(const uint32_t* units, uint64_t n_units, uint8_t* out, uint64_t n_out,
+                          const uint32_t* table, uint32_t L, uint32_t first_state,
+                          uint32_t first_bit, uint32_t S, uint32_t K, const void* index_in,
+                          void* index_out, uint64_t* stats) {
+    return emu_decode(units, n_units, out, n_out, table, L, first_state, first_bit, S, K, 0u,
+                      stats, nullptr, nullptr, (const SyncPoint*)index_in, (SyncPoint*)index_out);
 }
 
 }  // extern "C"

@@ -71,6 +71,7 @@ struct KArgs {
     uint32_t n_streams;
     uint32_t n_tasks;
     uint32_t wait_polls;             // k_sync: polls of a range's wait for its predecessor
+    uint32_t any_index_out;          // some stream asks for its sync points (k_index runs)
     Work w;
 };
 
@@ -561,10 +562,15 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
     const uint32_t u0 = c * sp.S;
     const uint32_t wu = c == 0u ? 0u : min(sp.warm, u0);
     if (valid) {
-        const uint32_t off0 = c == 0u ? 32u - sp.first_bit : 0u;
+        uint32_t off0 = c == 0u ? 32u - sp.first_bit : 0u;
+        L.xi = sp.first_state - sp.L;
+        if (sp.index_in && c != 0u) {      // recorded sync point, masked into range
+            const SyncPoint p = sp.index_in[c];
+            off0 = p.bit & 31u;
+            L.xi = p.state & (sp.L - 1u);
+        }
         L.rbase = u0 - wu;
         L.u_stop = min(u0 + sp.S, sp.n_units);
-        L.xi = sp.first_state - sp.L;
         L.u = L.rbase + (off0 >> 5);
         L.o = off0 & 31u;
         L.cnt = 0u;
@@ -814,6 +820,19 @@ __global__ __launch_bounds__(kWG) void k_scan_down(KArgs a, uint32_t total, cons
     }
 }
 
+// Sync points of the streams that ask for them (index_out), one lane per chunk (grid
+// as k_check), from the proven records (mans::chunk_sync_point).
+__global__ __launch_bounds__(kWG) void k_index(KArgs a) {
+    __shared__ uint32_t s_stream;
+    const uint32_t t = blockIdx.x;
+    if (threadIdx.x == 0) s_stream = stream_of_task<false>(a, t);
+    __syncthreads();
+    const StreamPlan sp = *plan_of(a, __builtin_amdgcn_readfirstlane(s_stream));
+    if (!sp.index_out) return;
+    const uint32_t c = (t - sp.task_chunk) * kWG + threadIdx.x;
+    if (c < sp.n_chunks) sp.index_out[c] = chunk_sync_point(sp, a.w, c);
+}
+
 // ---------------------------------------------------------------- write pass
 // Persistent workgroups; each takes a contiguous range of passes, a pass being 256
 // consecutive checkpoint intervals ("items", kR units each) of one stream.  A pass's
@@ -1047,11 +1066,19 @@ static uint32_t rule_chunk(uint64_t n_units, uint64_t n_out, uint32_t L) {
 // spec pass grows (10,000-stream mix: 43.4 -> 40.6 ms; 128 / 256 / 384 / 768 units and
 // S/2 .. S/8 measured worse).  Larger streams keep none: on 2^30-symbol streams a fixed
 // warm-up measured neutral to 10% slower.
-static uint32_t choose_warm(uint64_t n_units, uint64_t n_out, uint32_t S, uint32_t K) {
+static uint32_t choose_warm(uint64_t n_units, uint64_t n_out, uint32_t S, uint32_t K,
+                            bool index_in) {
     (void)n_out;
+    if (index_in) return 0u;                  // the specs start at recorded sync points
     const uint64_t chunks = (n_units + S - 1u) / S;
     return chunks <= 256u ? (512u / K) * K : 0u;
 }
+
+// Chunk/checkpoint lengths of one stream.  A stream with sync points uses the chunk
+// length they were recorded with, as given (one record per chunk of that length).
+static int stream_geometry(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t chunk_units,
+                           uint32_t checkpoint_units, const void* index_in,
+                           const void* index_out, uint32_t* S, uint32_t* K);
 
 void choose_geometry(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t req_S,
                      uint32_t req_K, uint32_t* S, uint32_t* K) {
@@ -1085,6 +1112,24 @@ void choose_geometry(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t req_
     s = 1u << ilog2(s < k ? k : s);
     *S = s;
     *K = k;
+}
+
+static int stream_geometry(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t chunk_units,
+                           uint32_t checkpoint_units, const void* index_in,
+                           const void* index_out, uint32_t* S, uint32_t* K) {
+    if (!index_in && !index_out) {
+        choose_geometry(n_units, n_out, L, chunk_units, checkpoint_units, S, K);
+        return MANS_OK;
+    }
+    if (chunk_units < kR || chunk_units > (1u << 24) || (chunk_units & (chunk_units - 1u)))
+        return set_error(MANS_EINVAL,
+                         "chunk_units=%u: sync points need a power of two in [%u, 2^24]",
+                         chunk_units, kR);
+    if ((((uintptr_t)index_in) | ((uintptr_t)index_out)) & 7u)
+        return set_error(MANS_EINVAL, "sync point arrays must be 8-byte aligned");
+    *S = chunk_units;
+    *K = kR;
+    return MANS_OK;
 }
 
 // Checkpoint slots of a stream (see ck_slot): n_chunks * S / K.
@@ -1179,6 +1224,7 @@ static int launch_pipeline(KArgs a, uint32_t n_chunk_tasks, uint32_t n_item_task
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kWG), 0, st, bsum, nb);
         hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kWG), 0, st, a, total_chunks, bsum);
     }
+    if (a.any_index_out) hipLaunchKernelGGL(k_index, dim3(n_chunk_tasks), dim3(kWG), 0, st, a);
     mark(5);
     {
         // persistent write pass: workgroups take blocks of 4 passes from a device counter
@@ -1232,6 +1278,12 @@ extern "C" int mans_workspace_bytes_geometry(uint64_t n_units, uint32_t chunk_un
     choose_geometry(n_units, 1u, 4096u, chunk_units, checkpoint_units, &S, &K);
     uint64_t chunks = (n_units + S - 1u) / S + 2u, cks = ck_slots(chunks, S, K) + 2u;
     *bytes = make_layout(chunks, cks, 1).bytes;
+    if (chunk_units >= kR && !(chunk_units & (chunk_units - 1u))) {
+        // a decode with sync points uses chunk_units as given (not capped)
+        chunks = (n_units + chunk_units - 1u) / chunk_units + 2u;
+        cks = ck_slots(chunks, chunk_units, kR) + 2u;
+        *bytes = std::max(*bytes, make_layout(chunks, cks, 1).bytes);
+    }
     return MANS_OK;
 }
 
@@ -1242,7 +1294,9 @@ extern "C" int mans_decode(const mans_decode_args* d) {
     if (rc) return rc;
     if (d->n_out == 0) return MANS_OK;
     uint32_t S, K;
-    choose_geometry(d->n_units, d->n_out, d->L, d->chunk_units, d->checkpoint_units, &S, &K);
+    rc = stream_geometry(d->n_units, d->n_out, d->L, d->chunk_units, d->checkpoint_units,
+                         d->index_in, d->index_out, &S, &K);
+    if (rc) return rc;
     if (K < 8u && !d->checkpoint_units) K = 8u;
     StreamPlan sp;
     std::memset(&sp, 0, sizeof(sp));
@@ -1257,7 +1311,9 @@ extern "C" int mans_decode(const mans_decode_args* d) {
     sp.S = S;
     sp.K = K;
     sp.log2K = ilog2(K);
-    sp.warm = choose_warm(d->n_units, d->n_out, S, K);
+    sp.warm = choose_warm(d->n_units, d->n_out, S, K, d->index_in != nullptr);
+    sp.index_in = (const SyncPoint*)d->index_in;
+    sp.index_out = (SyncPoint*)d->index_out;
     sp.n_chunks = (uint32_t)((d->n_units + S - 1) / S);
     sp.n_ck = (uint32_t)((d->n_units + K - 1) / K);
     sp.chunk_base = 0;
@@ -1273,6 +1329,7 @@ extern "C" int mans_decode(const mans_decode_args* d) {
     a.one = sp;
     a.plans = nullptr;
     a.n_streams = 1;
+    a.any_index_out = d->index_out ? 1u : 0u;
     a.w = make_work(d->workspace, l);
     uint32_t nct = (sp.n_chunks + kWG - 1) / kWG;
     uint32_t nit = (n_write_items(sp.n_ck) + kWG - 1) / kWG;
@@ -1318,7 +1375,9 @@ static int build_plans(const mans_stream_desc* d, size_t n, std::vector<StreamPl
         StreamPlan& sp = plans[i];
         std::memset(&sp, 0, sizeof(sp));
         uint32_t S, K;
-        choose_geometry(d[i].n_units, d[i].n_out, d[i].L, 0, 0, &S, &K);
+        rc = stream_geometry(d[i].n_units, d[i].n_out, d[i].L, d[i].chunk_units, 0,
+                             d[i].index_in, d[i].index_out, &S, &K);
+        if (rc) return set_error(rc, "stream %zu: %s", i, mans_last_error());
         if (K < 8u) K = 8u;
         sp.units = d[i].units;
         sp.out = d[i].out;
@@ -1330,7 +1389,9 @@ static int build_plans(const mans_stream_desc* d, size_t n, std::vector<StreamPl
         sp.first_bit = d[i].first_bit;
         sp.S = S;
         sp.K = K;
-        sp.warm = choose_warm(d[i].n_units, d[i].n_out, S, K);
+        sp.warm = choose_warm(d[i].n_units, d[i].n_out, S, K, d[i].index_in != nullptr);
+        sp.index_in = (const SyncPoint*)d[i].index_in;
+        sp.index_out = (SyncPoint*)d[i].index_out;
         sp.log2K = ilog2(K);
         // streams with nothing to decode keep one idle chunk/item so tasks stay valid
         uint64_t nu = d[i].n_out ? d[i].n_units : 1;
@@ -1417,6 +1478,7 @@ struct BatchPart {
     std::vector<StreamPlan> plans;
     uint64_t chunks = 0, cks = 0, units = 0, syms = 0, off = 0;
     uint32_t ct = 0, it = 0, ml = 0, mk = 0;
+    uint32_t any_index_out = 0;
     Layout l;
 };
 
@@ -1450,6 +1512,8 @@ static int plan_parts(const mans_stream_desc* d, size_t n, size_t k, std::vector
         int rc = build_plans(bp.descs.data(), bp.descs.size(), bp.plans, &bp.chunks, &bp.cks,
                              &bp.ct, &bp.it, &bp.ml, &bp.mk, &bp.units, &bp.syms);
         if (rc) return rc;
+        for (const mans_stream_desc& x : bp.descs)
+            if (x.index_out) bp.any_index_out = 1u;
         bp.l = make_layout(bp.chunks, bp.cks, bp.descs.size());
         bp.off = o;
         o = align256(o + bp.l.bytes);
@@ -1537,6 +1601,7 @@ static KArgs part_args(const BatchPart& bp, void* workspace) {
     a.sorder = a.order + bp.ct;
     a.n_streams = (uint32_t)bp.descs.size();
     a.n_tasks = bp.ct;
+    a.any_index_out = bp.any_index_out;
     a.w = make_work(base, bp.l);
     return a;
 }
@@ -1773,6 +1838,86 @@ extern "C" int mans_set_geometry_rules(const mans_geometry_rule* rules, size_t n
     }
     std::lock_guard<std::mutex> lk(g_rules_mu);
     g_rules.swap(v);
+    return MANS_OK;
+}
+
+extern "C" int mans_sync_point_count(uint64_t n_units, uint32_t chunk_units, uint64_t* count) {
+    if (!count) return set_error(MANS_EINVAL, "count is NULL");
+    if (chunk_units < kR || chunk_units > (1u << 24) || (chunk_units & (chunk_units - 1u)))
+        return set_error(MANS_EINVAL, "chunk_units=%u is not a power of two in [%u, 2^24]",
+                         chunk_units, kR);
+    *count = (n_units + chunk_units - 1u) / chunk_units;
+    return MANS_OK;
+}
+
+extern "C" int mans_sync_points_check(const mans_sync_point* ih, uint64_t count, uint32_t L) {
+    if (!ih && count) return set_error(MANS_EINVAL, "NULL argument");
+    if (L < 2u || L > 16384u || (L & (L - 1u)))
+        return set_error(MANS_EINVAL, "L=%u is not a power of two in [2, 16384]", L);
+    for (uint64_t c = 0; c < count; ++c) {
+        const uint64_t prev = c ? ih[c - 1u].symbols_before : 0u;
+        if (ih[c].state >= L || ih[c].bit > (c ? 31u : 32u) || ih[c].symbols_before < prev ||
+            (c == 0u && ih[c].symbols_before != 0u))
+            return set_error(MANS_ECORRUPT, "sync point %llu: state %u, bit %u, before %llu",
+                             (unsigned long long)c, ih[c].state, ih[c].bit,
+                             (unsigned long long)ih[c].symbols_before);
+    }
+    return MANS_OK;
+}
+
+extern "C" int mans_stream_range(const mans_stream_desc* f, const mans_sync_point* ih,
+                                 const mans_sync_point* idev, uint64_t first, uint64_t last,
+                                 mans_stream_desc* sub, uint64_t* skip) {
+    if (!f || !ih || !idev || !sub || !skip) return set_error(MANS_EINVAL, "NULL argument");
+    uint64_t n = 0;
+    int rc = mans_sync_point_count(f->n_units, f->chunk_units, &n);
+    if (rc) return rc;
+    if (first >= last || last > f->n_out)
+        return set_error(MANS_EINVAL, "window [%llu, %llu) is empty or beyond n_out=%llu",
+                         (unsigned long long)first, (unsigned long long)last,
+                         (unsigned long long)f->n_out);
+    if (n == 0 || ih[0].symbols_before != 0u)
+        return set_error(MANS_ECORRUPT, "sync points do not start at symbol 0");
+    const uint64_t S = f->chunk_units;
+    // ca: the last chunk starting at or before `first`; cb: the first chunk starting at or
+    // after `last` (n if none).  Records are ascending in symbols_before.
+    uint64_t lo = 0, hi = n - 1u;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi + 1u) >> 1;
+        if (ih[mid].symbols_before <= first) lo = mid; else hi = mid - 1u;
+    }
+    const uint64_t ca = lo;
+    lo = ca + 1u;
+    hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (ih[mid].symbols_before >= last) hi = mid; else lo = mid + 1u;
+    }
+    const uint64_t cb = lo;
+    const uint64_t before = ih[ca].symbols_before;
+    const uint64_t end = cb < n ? ih[cb].symbols_before : f->n_out;
+    // the records used must bracket the window (mans_sync_points_check validates all)
+    if (before > first || (ca + 1u < n && ih[ca + 1u].symbols_before <= first) ||
+        ih[cb - 1u].symbols_before >= last || end < last || end > f->n_out)
+        return set_error(MANS_ECORRUPT, "sync points are not ascending within [0, n_out]");
+    *sub = *f;
+    if (ca > 0u) {
+        if (ih[ca].state >= f->L || ih[ca].bit > 31u)
+            return set_error(MANS_ECORRUPT, "sync point %llu out of range",
+                             (unsigned long long)ca);
+        sub->first_state = ih[ca].state + f->L;
+        sub->first_bit = 32u - ih[ca].bit;
+    }
+    const uint64_t u0 = ca * S;
+    // one unit past the last chunk: its last steps may read bits from there
+    const uint64_t u1 = std::min<uint64_t>(cb * S + 1u, f->n_units);
+    sub->units = f->units + u0;
+    sub->n_units = u1 - u0;
+    sub->n_out = end - before;
+    sub->out = nullptr;
+    sub->index_in = idev + ca;
+    sub->index_out = nullptr;
+    *skip = first - before;
     return MANS_OK;
 }
 

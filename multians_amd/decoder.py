@@ -68,7 +68,7 @@ class DeviceStream:
                               dtype=torch.uint8, device=device)
 
     def args(self, stream=None, chunk_units=0, checkpoint_units=0, flags=0, n_out=None,
-             kernel_events=None):
+             kernel_events=None, index_in=None, index_out=None):
         torch = _torch()
         st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         a = _lib.DecodeArgs()
@@ -87,12 +87,55 @@ class DeviceStream:
         a.workspace_bytes = self.ws.numel()
         a.stream = st
         a.kernel_events = kernel_events
+        a.index_in = index_in.data_ptr() if index_in is not None else None
+        a.index_out = index_out.data_ptr() if index_out is not None else None
         return a
 
     def decode(self, **kw):
+        """index_in / index_out: device tensors of sync_point_count(n_units, chunk_units)
+        records (new_index), with chunk_units given."""
         a = self.args(**kw)
         _lib.check(_lib.lib().mans_decode(C.byref(a)))
         return self.out
+
+    def new_index(self, chunk_units):
+        """Zeroed device buffer for this stream's sync points at this chunk length."""
+        torch = _torch()
+        n = _lib.sync_point_count(self.es.n_units, chunk_units)
+        return torch.zeros(max(n, 1) * 16, dtype=torch.uint8, device=self.units.device)
+
+    def desc(self, chunk_units=0):
+        d = _lib.StreamDesc()
+        d.units, d.n_units = self.units.data_ptr(), self.es.n_units
+        d.out, d.n_out = self.out.data_ptr(), self.es.n_symbols
+        d.table, d.L = self.table.data_ptr(), self.es.L
+        d.first_state, d.first_bit = self.es.first_state, self.es.first_bit
+        d.chunk_units = chunk_units
+        return d
+
+    def decode_range(self, first, last, index, chunk_units, index_host=None, stream=None):
+        """Output symbols [first, last) (decode order) decoded alone from the sync points
+        `index` (device tensor from a decode with index_out at this chunk length): the
+        chunks that hold the window, starting at the recorded sync point of the first.
+        Returns a new device tensor of last - first symbols."""
+        torch = _torch()
+        if index_host is None:
+            index_host = index.cpu().numpy().view(_lib.SYNC_POINT_DTYPE)
+        sub, skip = _lib.stream_range(self.desc(chunk_units), index_host, index.data_ptr(),
+                                      first, last)
+        out = torch.empty(max(sub.n_out, 1), dtype=torch.uint8, device=self.units.device)
+        ws = torch.empty(workspace_bytes(sub.n_units, sub.L, chunk_units, 16), dtype=torch.uint8,
+                         device=self.units.device)
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        a = _lib.DecodeArgs()
+        a.units, a.n_units, a.out, a.n_out = sub.units, sub.n_units, out.data_ptr(), sub.n_out
+        a.table, a.L, a.first_state, a.first_bit = sub.table, sub.L, sub.first_state, sub.first_bit
+        a.chunk_units, a.checkpoint_units = chunk_units, 0
+        a.workspace, a.workspace_bytes, a.stream = ws.data_ptr(), ws.numel(), st
+        a.index_in = sub.index_in
+        _lib.check(_lib.lib().mans_decode(C.byref(a)))
+        self._range_ws = ws                 # kept alive until the next call
+        return out[skip:skip + (last - first)]
 
     def status(self, stream=None):
         torch = _torch()
@@ -108,7 +151,9 @@ class DeviceBatch:
     decoded together by mans_decode_batch.  Units of every stream start 16-byte
     aligned and are padded to a multiple of four units; outputs are packed."""
 
-    def __init__(self, streams, device="cuda"):
+    def __init__(self, streams, device="cuda", sync_points=False):
+        """sync_points: give every stream a sync-point buffer (self.index, packed, stream i
+        at record ioff[i]) at its tuned chunk length, for decode(index=...)."""
         torch = _torch()
         self.streams = list(streams)
         n = len(self.streams)
@@ -140,13 +185,49 @@ class DeviceBatch:
             d.L = es.L
             d.first_state = es.first_state
             d.first_bit = es.first_bit
+        self.index = None
+        self._idescs = {}
+        if sync_points:
+            cu = np.zeros(n, np.int64)
+            ioff = np.zeros(n + 1, np.int64)
+            for i, es in enumerate(self.streams):
+                c = C.c_uint32()
+                _lib.check(_lib.lib().mans_chunk_units(es.n_units, es.n_symbols, es.L,
+                                                       C.byref(c)))
+                cu[i] = max(c.value, 16)
+                ioff[i + 1] = ioff[i] + _lib.sync_point_count(es.n_units, int(cu[i]))
+            self.chunk_units, self.ioff = cu, ioff
+            self.index = torch.zeros(max(int(ioff[-1]), 1) * 16, dtype=torch.uint8, device=device)
+            ib = self.index.data_ptr()
+            for mode in ("out", "in", "both"):
+                ds = (_lib.StreamDesc * n)()
+                C.memmove(ds, self.descs, C.sizeof(ds))
+                for i in range(n):
+                    ds[i].chunk_units = int(cu[i])
+                    p = ib + 16 * int(ioff[i])
+                    if mode in ("in", "both"):
+                        ds[i].index_in = p
+                    if mode in ("out", "both"):
+                        ds[i].index_out = p
+                self._idescs[mode] = ds
         b = C.c_uint64()
-        _lib.check(_lib.lib().mans_batch_workspace_bytes(self.descs, n, C.byref(b)))
-        self.ws = torch.zeros(b.value, dtype=torch.uint8, device=device)
+        need = 0
+        for ds in [self.descs] + list(self._idescs.values()):
+            _lib.check(_lib.lib().mans_batch_workspace_bytes(ds, n, C.byref(b)))
+            need = max(need, b.value)
+        self.ws = torch.zeros(need, dtype=torch.uint8, device=device)
 
-    def decode(self, stream=None, kernel_events=None, one_pipeline=False):
-        """one_pipeline: no split into concurrent parts (MANS_BATCH_ONE_PIPELINE)."""
+    def sync_points(self, i):
+        """Host copy of stream i's sync points (SYNC_POINT_DTYPE records)."""
+        a, b = int(self.ioff[i]), int(self.ioff[i + 1])
+        return self.index[16 * a:16 * b].cpu().numpy().view(_lib.SYNC_POINT_DTYPE)
+
+    def decode(self, stream=None, kernel_events=None, one_pipeline=False, index=None):
+        """one_pipeline: no split into concurrent parts (MANS_BATCH_ONE_PIPELINE).
+        index (batches built with sync_points): "out" writes every stream's sync points,
+        "in" starts the decode from them, "both" does both."""
         torch = _torch()
+        descs = self.descs if index is None else self._idescs[index]
         st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         o = _lib.BatchOpts()
         o.kernel_events = kernel_events
@@ -155,7 +236,7 @@ class DeviceBatch:
         o.workspace_bytes = self.ws.numel()
         o.stream = st
         stats = _lib.BatchStats()
-        _lib.check(_lib.lib().mans_decode_batch(self.descs, len(self.streams), C.byref(o),
+        _lib.check(_lib.lib().mans_decode_batch(descs, len(self.streams), C.byref(o),
                                                 C.byref(stats)))
         return stats
 
