@@ -146,6 +146,68 @@ class DeviceStream:
         return s.value
 
 
+def plan_windows(full_descs, records, windows):
+    """Sub-stream descriptors for windows [(stream, first, last), ...] (decode order):
+    full_descs[stream] has chunk_units and its sync points on the device in index_in,
+    records(stream) returns the same records on the host.  Returns (descs, skip per
+    window, output offset per window) for one mans_decode_batch over the sub-streams."""
+    n = len(windows)
+    subs = (_lib.StreamDesc * max(n, 1))()
+    skips = np.zeros(n, np.int64)
+    ooff = np.zeros(n + 1, np.int64)
+    for k, (i, first, last) in enumerate(windows):
+        rec = records(i)
+        if rec is None or not full_descs[i].index_in:
+            raise ValueError("stream %d has no sync points" % i)
+        sub, skip = _lib.stream_range(full_descs[i], rec, full_descs[i].index_in, first, last)
+        subs[k] = sub
+        skips[k] = skip
+        ooff[k + 1] = ooff[k] + sub.n_out
+    return subs, skips, ooff
+
+
+class WindowDecoder:
+    """Decodes planned windows (plan_windows) with one mans_decode_batch call into an
+    arena it keeps (and grows) on the device."""
+
+    def __init__(self, device="cuda"):
+        self.device = device
+        self.arena = None
+        self.ws = None
+
+    def decode(self, planned, windows, stream=None, one_pipeline=False):
+        """Returns (arena, [(offset, length)]): window k at arena[offset:offset+length]."""
+        torch = _torch()
+        subs, skips, ooff = planned
+        n = len(windows)
+        need = max(int(ooff[-1]), 1)
+        if self.arena is None or self.arena.numel() < need:
+            self.arena = torch.empty(need, dtype=torch.uint8, device=self.device)
+        base = self.arena.data_ptr()
+        for k in range(n):
+            subs[k].out = base + int(ooff[k])
+        b = C.c_uint64()
+        _lib.check(_lib.lib().mans_batch_workspace_bytes(subs, n, C.byref(b)))
+        if self.ws is None or self.ws.numel() < b.value:
+            self.ws = torch.zeros(max(b.value, 256), dtype=torch.uint8, device=self.device)
+        o = _lib.BatchOpts()
+        o.workspace, o.workspace_bytes = self.ws.data_ptr(), self.ws.numel()
+        o.stream = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        o.flags = _lib.MANS_BATCH_ONE_PIPELINE if one_pipeline else 0
+        stats = _lib.BatchStats()
+        _lib.check(_lib.lib().mans_decode_batch(subs, n, C.byref(o), C.byref(stats)))
+        spans = [(int(ooff[k] + skips[k]), int(w[2] - w[1])) for k, w in enumerate(windows)]
+        return self.arena, spans
+
+    def status(self, stream=None):
+        torch = _torch()
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        s = C.c_int()
+        _lib.check(_lib.lib().mans_decode_status(C.c_void_p(self.ws.data_ptr()), C.c_void_p(st),
+                                                 C.byref(s)))
+        return s.value
+
+
 class DeviceBatch:
     """Many independent streams (each with its own table) resident on the device,
     decoded together by mans_decode_batch.  Units of every stream start 16-byte
@@ -221,6 +283,15 @@ class DeviceBatch:
         """Host copy of stream i's sync points (SYNC_POINT_DTYPE records)."""
         a, b = int(self.ioff[i]), int(self.ioff[i + 1])
         return self.index[16 * a:16 * b].cpu().numpy().view(_lib.SYNC_POINT_DTYPE)
+
+    def plan_windows(self, windows, index_host=None):
+        """plan_windows over this batch's streams (after a decode with index="out");
+        index_host: host copy of self.index (copied here if not given)."""
+        if index_host is None:
+            index_host = self.index.cpu().numpy().view(_lib.SYNC_POINT_DTYPE)
+        ioff = self.ioff
+        return plan_windows(self._idescs["in"], lambda i: index_host[ioff[i]:ioff[i + 1]],
+                            windows)
 
     def decode(self, stream=None, kernel_events=None, one_pipeline=False, index=None):
         """one_pipeline: no split into concurrent parts (MANS_BATCH_ONE_PIPELINE).
