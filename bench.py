@@ -99,7 +99,7 @@ def bench_batched(args, torch):
     total_units = sum(es.n_units for es, _ in items)
     log("batch: %d streams, %.2f GB symbols, %.2f GB compressed, generated in %.1fs" %
         (len(items), total_sym / 1e9, 4.0 * total_units / 1e9, time.time() - t0))
-    b = DeviceBatch([es for es, _ in items])
+    b = DeviceBatch([es for es, _ in items], sync_points=not args.no_index)
     dec = (lambda: b.decode(one_pipeline=True)) if args.one_pipeline else b.decode
     dec()
     torch.cuda.synchronize()
@@ -119,6 +119,8 @@ def bench_batched(args, torch):
     ms_batch = time_events(torch, dec, args.steps)
     res = {"ms_batch": ms_batch, "ms_plain": None, "kernels_ms": kms,
            "total_symbols": total_sym, "total_units": total_units, "n_streams": len(items)}
+    if not args.no_index:
+        res["sync_points"] = bench_sync_points(args, torch, b, items)
     if args.no_plain:
         del b
         torch.cuda.empty_cache()
@@ -144,6 +146,84 @@ def bench_batched(args, torch):
     del b, ws
     torch.cuda.empty_cache()
     return res, items
+
+
+def bench_sync_points(args, torch, b, items):
+    """The batch from its recorded sync points at the tuned chunk lengths and at
+    256-unit chunks, each also as windows decoded alone (mans_stream_range)."""
+    out = {"tuned_chunks": sync_point_leg(args, torch, b, items)}
+    b.use_index(256)
+    out["chunks_256"] = sync_point_leg(args, torch, b, items)
+    b.use_index()
+    return out
+
+
+def sync_point_leg(args, torch, b, items):
+    """The same batch again from its recorded sync points (index_in), and batches of
+    windows decoded alone from them (mans_stream_range), both checked bit-exact."""
+    from multians_amd import _lib
+    from multians_amd.decoder import WindowDecoder
+    total_sym = sum(es.n_symbols for es, _ in items)
+    b.decode(index="out")
+    torch.cuda.synchronize()
+    n_rec = int(b.ioff[-1])
+    b.out.zero_()
+    b.decode(index="in")
+    torch.cuda.synchronize()
+    bad = verify_batch(b, items, torch)
+    log("decode from sync points bit-exact: %s (status %d)" % (not bad, b.status()))
+    if bad or b.status():
+        raise SystemExit("decode from sync points mismatch: streams %s" % bad[:10])
+    kt = _lib.KernelTimer()
+    b.decode(index="in", kernel_events=kt.ptr)
+    torch.cuda.synchronize()
+    kms = kt.times_ms()
+    kt.close()
+    dec = (lambda: b.decode(index="in", one_pipeline=True)) if args.one_pipeline else \
+        (lambda: b.decode(index="in"))
+    for _ in range(args.warmup):
+        dec()
+    ms = time_events(torch, dec, args.steps)
+    out = {"ms": round(ms, 4), "GBps": round(total_sym / ms / 1e6, 3),
+           "kernels_ms": {k: round(v, 4) for k, v in kms.items()},
+           "records": n_rec, "record_bytes": 16 * n_rec}
+    log("from sync points", json.dumps(out))
+    # windows of WIN symbols at random positions of the streams that hold one
+    win, count = 65536, args.windows
+    rng = np.random.default_rng(4)
+    big = [i for i, (es, _) in enumerate(items) if es.n_symbols >= win]
+    if not big:
+        return out
+    wins = []
+    for _ in range(count):
+        i = int(rng.choice(big))
+        a = int(rng.integers(0, items[i][0].n_symbols - win + 1))
+        wins.append((i, a, a + win))
+    index_host = b.index.cpu().numpy().view(_lib.SYNC_POINT_DTYPE)
+    t0 = time.time()
+    planned = b.plan_windows(wins, index_host)
+    plan_ms = (time.time() - t0) * 1e3
+    wd = WindowDecoder()
+    arena, spans = wd.decode(planned, wins)
+    torch.cuda.synchronize()
+    host = arena.cpu().numpy()
+    bad = [k for k, ((i, a, z), (o, ln)) in enumerate(zip(wins, spans))
+           if not np.array_equal(host[o:o + ln], items[i][1][::-1][a:z])]
+    log("windows bit-exact: %s (status %d)" % (not bad, wd.status()))
+    if bad or wd.status():
+        raise SystemExit("window decode mismatch: windows %s" % bad[:10])
+    decoded = int(planned[2][-1])
+    for _ in range(args.warmup):
+        wd.decode(planned, wins)
+    wms = time_events(torch, lambda: wd.decode(planned, wins), args.steps)
+    out["windows"] = {"count": count, "window_symbols": win, "ms": round(wms, 4),
+                      "windows_per_s": round(count / wms * 1e3, 1),
+                      "window_GBps": round(count * win / wms / 1e6, 3),
+                      "decoded_symbols": decoded,
+                      "decoded_GBps": round(decoded / wms / 1e6, 3),
+                      "host_plan_ms": round(plan_ms, 2)}
+    log("windows", json.dumps(out["windows"]))
+    return out
 
 
 def bench_single(args, torch):
@@ -375,6 +455,10 @@ def main():
                     help="also decode two 2^32-symbol streams (lambda 0.002 and 3.0)")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the per-stream plain-launch leg")
+    ap.add_argument("--no-index", action="store_true",
+                    help="skip the sync-point legs (decode from recorded sync points, windows)")
+    ap.add_argument("--windows", type=int, default=10000,
+                    help="windows of 64 Ki symbols per call in the window leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one elementwise kernel that reads and writes MIB MiB (known "
                          "HBM traffic, for scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -426,6 +510,8 @@ def main():
                         "traffic_source": "profiles/batch10k_kernels.md (rocprofv3 --pmc)",
                         "whole_decode_frac": round((units4 + bres["total_symbols"]) / ms / 1e6
                                                    / HBM_PEAK_GBS, 4)}}
+    if "sync_points" in bres:
+        res["from_sync_points"] = bres["sync_points"]
     log("batched", json.dumps({k: res[k] for k in ("batched_GBps", "per_stream_plain_GBps",
                                                    "batched_vs_plain", "kernels_ms")}))
     if not args.no_single:

@@ -247,37 +247,59 @@ class DeviceBatch:
             d.L = es.L
             d.first_state = es.first_state
             d.first_bit = es.first_bit
+        self.device = device
         self.index = None
         self._idescs = {}
+        self.ws = None
         if sync_points:
-            cu = np.zeros(n, np.int64)
-            ioff = np.zeros(n + 1, np.int64)
-            for i, es in enumerate(self.streams):
+            self.use_index()
+        else:
+            self._fit_workspace([self.descs])
+
+    def _fit_workspace(self, desc_sets):
+        torch = _torch()
+        b = C.c_uint64()
+        need = 0
+        for ds in desc_sets:
+            _lib.check(_lib.lib().mans_batch_workspace_bytes(ds, len(self.streams), C.byref(b)))
+            need = max(need, b.value)
+        if self.ws is None or self.ws.numel() < need:
+            self.ws = None
+            self.ws = torch.zeros(need, dtype=torch.uint8, device=self.device)
+
+    def use_index(self, chunk_units=None):
+        """(Re)build the sync-point buffer and descriptors: every stream at its tuned
+        chunk length, or at `chunk_units` (a power of two >= 16) for all."""
+        torch = _torch()
+        n = len(self.streams)
+        cu = np.zeros(n, np.int64)
+        ioff = np.zeros(n + 1, np.int64)
+        for i, es in enumerate(self.streams):
+            if chunk_units:
+                cu[i] = chunk_units
+            else:
                 c = C.c_uint32()
                 _lib.check(_lib.lib().mans_chunk_units(es.n_units, es.n_symbols, es.L,
                                                        C.byref(c)))
                 cu[i] = max(c.value, 16)
-                ioff[i + 1] = ioff[i] + _lib.sync_point_count(es.n_units, int(cu[i]))
-            self.chunk_units, self.ioff = cu, ioff
-            self.index = torch.zeros(max(int(ioff[-1]), 1) * 16, dtype=torch.uint8, device=device)
-            ib = self.index.data_ptr()
-            for mode in ("out", "in", "both"):
-                ds = (_lib.StreamDesc * n)()
-                C.memmove(ds, self.descs, C.sizeof(ds))
-                for i in range(n):
-                    ds[i].chunk_units = int(cu[i])
-                    p = ib + 16 * int(ioff[i])
-                    if mode in ("in", "both"):
-                        ds[i].index_in = p
-                    if mode in ("out", "both"):
-                        ds[i].index_out = p
-                self._idescs[mode] = ds
-        b = C.c_uint64()
-        need = 0
-        for ds in [self.descs] + list(self._idescs.values()):
-            _lib.check(_lib.lib().mans_batch_workspace_bytes(ds, n, C.byref(b)))
-            need = max(need, b.value)
-        self.ws = torch.zeros(need, dtype=torch.uint8, device=device)
+            ioff[i + 1] = ioff[i] + _lib.sync_point_count(es.n_units, int(cu[i]))
+        self.chunk_units, self.ioff = cu, ioff
+        self.index = None
+        self.index = torch.zeros(max(int(ioff[-1]), 1) * 16, dtype=torch.uint8, device=self.device)
+        ib = self.index.data_ptr()
+        self._idescs = {}
+        for mode in ("out", "in", "both"):
+            ds = (_lib.StreamDesc * n)()
+            C.memmove(ds, self.descs, C.sizeof(ds))
+            for i in range(n):
+                ds[i].chunk_units = int(cu[i])
+                p = ib + 16 * int(ioff[i])
+                if mode in ("in", "both"):
+                    ds[i].index_in = p
+                if mode in ("out", "both"):
+                    ds[i].index_out = p
+            self._idescs[mode] = ds
+        self._fit_workspace([self.descs] + list(self._idescs.values()))
 
     def sync_points(self, i):
         """Host copy of stream i's sync points (SYNC_POINT_DTYPE records)."""
