@@ -304,6 +304,43 @@ int mans_build_tables_device(const mans_table_job* jobs, size_t n_jobs, void* wo
  * waits for the stream. */
 int mans_table_status(const void* workspace, size_t n_jobs, void* stream, uint32_t* status);
 
+/* ------------------------------------------------------- encoder on the device */
+/* Encodes symbols already on the device into the decoder's stream format, bit-exact
+ * with mans_encode (units, first_bit, first_state), which restates ANSEncoder::encode
+ * (src/ans_encoder.cc:13-97) and the unit reversal of CUHDInputBuffer
+ * (src/multians_input_buffer.cc:29); the reference encodes only on the host.  Each job
+ * gives its decoder table (L u32, CUHDCodetableItem layout, as built by
+ * mans_build_tables / mans_build_tables_device / ANSTableGenerator): the encoder
+ * derives its own tables from it.  Streams are cut into 4,096-symbol chunks encoded in
+ * parallel from a guessed state, proven by merge walks and fixup rounds, then written
+ * at their exact bit offsets.  Enqueued on `stream`; the plans are copied in first
+ * (synchronously).  No allocation. */
+typedef struct mans_encode_job {
+    const uint8_t* symbols;     /* device, n bytes */
+    uint64_t n;
+    const uint32_t* table;      /* device, L entries (decoder table) */
+    uint32_t L;                 /* power of two in [2, 16384] */
+    uint32_t reserved;
+    uint32_t* units;            /* device, units_capacity u32 (>= mans_max_encoded_units(n)) */
+    uint64_t units_capacity;
+} mans_encode_job;
+
+typedef struct mans_encode_result {
+    uint64_t n_units;
+    uint32_t first_bit;
+    uint32_t first_state;
+    uint32_t status;            /* 0 ok; 1 bad table, 2 symbol without a state, 4 capacity */
+    uint32_t reserved;
+} mans_encode_result;
+
+int mans_encode_workspace_bytes(const mans_encode_job* jobs, size_t n_jobs, uint64_t* bytes);
+int mans_encode_device(const mans_encode_job* jobs, size_t n_jobs, void* workspace,
+                       uint64_t workspace_bytes, void* stream);
+/* Results of the last mans_encode_device with these jobs and workspace (synchronises
+ * `stream`). */
+int mans_encode_results(const void* workspace, const mans_encode_job* jobs, size_t n_jobs,
+                        void* stream, mans_encode_result* results);
+
 #ifdef __cplusplus
 }
 #endif

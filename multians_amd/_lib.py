@@ -71,6 +71,17 @@ class TableJob(C.Structure):
                 ("reserved", C.c_uint32), ("table", C.c_void_p), ("counts", C.c_void_p)]
 
 
+class EncodeJob(C.Structure):
+    _fields_ = [("symbols", C.c_void_p), ("n", C.c_uint64), ("table", C.c_void_p),
+                ("L", C.c_uint32), ("reserved", C.c_uint32), ("units", C.c_void_p),
+                ("units_capacity", C.c_uint64)]
+
+
+class EncodeResult(C.Structure):
+    _fields_ = [("n_units", C.c_uint64), ("first_bit", C.c_uint32), ("first_state", C.c_uint32),
+                ("status", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class GeometryRule(C.Structure):
     _fields_ = [("L_max", C.c_uint32), ("chunk_units", C.c_uint32), ("spu_max", C.c_double),
                 ("units_max", C.c_uint64)]
@@ -96,6 +107,7 @@ EXPORTS = [
     "mans_set_geometry_rules", "mans_chunk_units", "mans_histogram", "mans_set_wait_polls",
     "mans_table_workspace_bytes", "mans_build_tables_device", "mans_table_status",
     "mans_sync_point_count", "mans_stream_range", "mans_sync_points_check",
+    "mans_encode_workspace_bytes", "mans_encode_device", "mans_encode_results",
 ]
 
 _lib = None
@@ -156,6 +168,10 @@ def lib():
         "mans_build_tables_device": ([C.POINTER(TableJob), C.c_size_t, vp, u64, vp], i32),
         "mans_table_status": ([vp, C.c_size_t, vp, C.POINTER(u32)], i32),
         "mans_sync_point_count": ([u64, u32, C.POINTER(u64)], i32),
+        "mans_encode_workspace_bytes": ([C.POINTER(EncodeJob), C.c_size_t, C.POINTER(u64)], i32),
+        "mans_encode_device": ([C.POINTER(EncodeJob), C.c_size_t, vp, u64, vp], i32),
+        "mans_encode_results": ([vp, C.POINTER(EncodeJob), C.c_size_t, vp,
+                                 C.POINTER(EncodeResult)], i32),
         "mans_sync_points_check": ([vp, u64, u32], i32),
         "mans_stream_range": ([C.POINTER(StreamDesc), vp, vp, u64, u64, C.POINTER(StreamDesc),
                                C.POINTER(u64)], i32),
@@ -228,6 +244,28 @@ def make_stream(seed, lam, n, L, keep_symbols=True):
     check(lib().mans_make_stream(seed, lam, n, L, _ptr(sym) if keep_symbols else None, _ptr(units),
                                  len(units), C.byref(nu), C.byref(fb), C.byref(fs), _ptr(dec)))
     return units[:nu.value], fb.value, fs.value, dec, sym
+
+
+def encode_device(jobs, stream=None):
+    """Encode on the device.  jobs: list of (symbols u8 tensor, decoder table tensor, L,
+    units tensor of int32 with capacity >= max_encoded_units(n)).  Returns a list of
+    (n_units, first_bit, first_state, status); waits for the stream."""
+    import torch
+    n = len(jobs)
+    js = (EncodeJob * n)()
+    for k, (sym, table, L, units) in enumerate(jobs):
+        js[k].symbols = sym.data_ptr() if sym.numel() else units.data_ptr()
+        js[k].n = sym.numel()
+        js[k].table, js[k].L = table.data_ptr(), L
+        js[k].units, js[k].units_capacity = units.data_ptr(), units.numel()
+    b = C.c_uint64()
+    check(lib().mans_encode_workspace_bytes(js, n, C.byref(b)))
+    ws = torch.empty(max(b.value, 256), dtype=torch.uint8, device="cuda")
+    st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    check(lib().mans_encode_device(js, n, C.c_void_p(ws.data_ptr()), ws.numel(), C.c_void_p(st)))
+    res = (EncodeResult * n)()
+    check(lib().mans_encode_results(C.c_void_p(ws.data_ptr()), js, n, C.c_void_p(st), res))
+    return [(r.n_units, r.first_bit, r.first_state, r.status) for r in res]
 
 
 def sync_point_count(n_units, chunk_units):

@@ -1,0 +1,566 @@
+// mans_encode.hip -- tANS encoder on the device, bit-exact with the host codec's
+// mans_encode (which restates ANSEncoder::encode, src/ans_encoder.cc:13-97, and the unit
+// reversal of CUHDInputBuffer, src/multians_input_buffer.cc:29): state x starts at L,
+// each symbol appends its code MSB-first and moves x on, the last unit keeps first_bit
+// bits left-aligned (a zero unit if none), and the units are stored in decoder order.
+//
+// Encoding from a wrong state also converges on the true state path (the state is
+// shifted right before each lookup, which maps many states to one), so a stream is cut
+// into chunks that encode in parallel:
+//   k_enc_sync   per 256-chunk range: each lane encodes its chunk from the start state
+//                L (a guess for chunks >= 1), storing the state and the bits emitted at
+//                every checkpoint (kEK symbols); then each lane encodes its chunk again
+//                from its predecessor's exit until the state meets the stored one.
+//   k_enc_fix    per stream: rounds that re-walk every chunk whose recorded start is not
+//                its predecessor's exit (range seams, rare failed walks), then the
+//                exclusive scan of bits per chunk, the stream's results, and zeroed
+//                units where chunks share one.
+//   k_enc_write  per range: each lane encodes its chunk from its true start and writes
+//                its bits at their offset (whole units by plain stores, units shared with
+//                a neighbour chunk by atomicOr into the zeroed unit).
+// The encoder's table comes from the decoder table (the decoder's own input): symbol s
+// owns f_s states y whose `next` fields are f_s .. 2f_s - 1; encoding x emits
+// nb = m_s - (x < f_s << m_s) bits, m_s = log2(L) - floor(log2 f_s), and moves to the
+// state y of s with next = x >> nb.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "mans_c_api.h"
+#include "mans_internal.h"
+
+namespace {
+
+constexpr uint32_t kEWG = 256;          // lanes (chunks) per workgroup
+constexpr uint32_t kEC = 4096;          // symbols per chunk
+constexpr uint32_t kEK = 32;            // symbols per checkpoint
+constexpr uint32_t kECK = kEC / kEK;    // checkpoints per chunk
+constexpr uint32_t kNoneE = 0xFFFFFFFFu;
+
+// status bits (mans_encode_result.status)
+constexpr uint32_t kEncBadTable = 1u;   // a `next` field outside its symbol's range
+constexpr uint32_t kEncNoCode = 2u;     // a symbol with no state in the table
+constexpr uint32_t kEncCapacity = 4u;   // more units than units_capacity
+
+struct EncPlan {
+    const uint8_t* sym;
+    const uint32_t* table;
+    uint32_t* units;
+    uint64_t n;
+    uint64_t cap;
+    uint32_t L, R;
+    uint32_t n_chunks, chunk_base;
+    uint64_t ck_base;
+    uint32_t task_base, pad;
+};
+
+struct EncWork {
+    const EncPlan* plans;
+    uint32_t n_jobs, n_tasks;
+    // per chunk
+    uint32_t *entry, *mq, *wbits, *wexit, *wtot, *sexit, *stot, *fl, *fl2, *ff;
+    uint64_t* off;
+    // per checkpoint
+    uint32_t *ckx, *ckb;
+    // per job
+    mans_encode_result* res;
+};
+
+__device__ uint32_t job_of_task(const EncWork& w, uint32_t t) {
+    uint32_t lo = 0, hi = w.n_jobs - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (w.plans[mid].task_base <= t) lo = mid; else hi = mid - 1u;
+    }
+    return lo;
+}
+
+// Encoder tables in LDS: per symbol f | base << 16 (base: first slot of the symbol's
+// states in ys), thr = f << m, m; ys[base + i - f] = state index y with next(y) = i.
+struct EncTab {
+    uint32_t* info;
+    uint32_t* thr;
+    uint32_t* mb;
+    uint16_t* ys;
+};
+
+constexpr size_t enc_lds_bytes(uint32_t L) { return 4u * 256u * 4u + 2u * L; }
+
+// Builds the tables for plan p (all threads); returns status bits (uniform).
+__device__ uint32_t stage_enc(const EncPlan& p, uint32_t* lds, EncTab& t) {
+    __shared__ uint32_t s_bad;
+    t.info = lds;
+    t.thr = lds + 256;
+    t.mb = lds + 512;
+    uint32_t* cnt = lds + 768;
+    t.ys = (uint16_t*)(lds + 1024);
+    if (threadIdx.x == 0) s_bad = 0u;
+    cnt[threadIdx.x] = 0u;
+    for (uint32_t k = threadIdx.x; k < p.L; k += kEWG) t.ys[k] = 0xFFFFu;
+    __syncthreads();
+    for (uint32_t y = threadIdx.x; y < p.L; y += kEWG) atomicAdd(&cnt[(p.table[y] >> 16) & 0xFFu], 1u);
+    __syncthreads();
+    if (threadIdx.x < 64u) {               // exclusive prefix of the counts (one wave)
+        uint32_t v[4], s = 0;
+        for (uint32_t k = 0; k < 4; ++k) { v[k] = cnt[threadIdx.x * 4u + k]; s += v[k]; }
+        uint32_t incl = s;
+        for (uint32_t d = 1; d < 64u; d <<= 1) {
+            const uint32_t o = __shfl_up(incl, d, 64);
+            if (threadIdx.x >= d) incl += o;
+        }
+        uint32_t run = incl - s;
+        for (uint32_t k = 0; k < 4; ++k) {
+            const uint32_t sy = threadIdx.x * 4u + k, f = v[k];
+            t.info[sy] = f | (run << 16);
+            const uint32_t m = f ? p.R - (31u - (uint32_t)__builtin_clz(f)) : 0u;
+            t.thr[sy] = f << m;
+            t.mb[sy] = m;
+            run += f;
+        }
+    }
+    __syncthreads();
+    for (uint32_t y = threadIdx.x; y < p.L; y += kEWG) {
+        const uint32_t e = p.table[y];
+        const uint32_t s = (e >> 16) & 0xFFu, i = e & 0xFFFFu, f = t.info[s] & 0xFFFFu;
+        if (i < f || i >= 2u * f) {
+            s_bad = 1u;
+        } else {
+            t.ys[(t.info[s] >> 16) + i - f] = (uint16_t)y;
+        }
+    }
+    __syncthreads();
+    // every (symbol, next) pair exactly once: no slot left unwritten
+    for (uint32_t k = threadIdx.x; k < p.L; k += kEWG)
+        if (t.ys[k] == 0xFFFFu) s_bad = 1u;
+    __syncthreads();
+    return s_bad ? kEncBadTable : 0u;
+}
+
+// One symbol: x (in [L, 2L)) -> new state; *code / *nb the emitted bits; *bad set when
+// the symbol has no state.
+__device__ __forceinline__ uint32_t enc_step(const EncTab& t, uint32_t L, uint32_t s, uint32_t x,
+                                             uint32_t* code, uint32_t* nb, uint32_t* bad) {
+    const uint32_t info = t.info[s];
+    const uint32_t f = info & 0xFFFFu;
+    if (f == 0u) {
+        *bad = 1u;
+        *code = 0u;
+        *nb = 0u;
+        return x;
+    }
+    const uint32_t n = t.mb[s] - (x < t.thr[s] ? 1u : 0u);
+    *nb = n;
+    *code = x & ((1u << n) - 1u);
+    return L + t.ys[(info >> 16) + (x >> n) - f];
+}
+
+// Symbols of one chunk, read 16 at a time.
+struct SymReader {
+    const uint8_t* p;
+    uint64_t i, e;
+    uint4 v;
+    uint32_t k;
+    __device__ void init(const uint8_t* base, uint64_t b, uint64_t end) {
+        p = base;
+        i = b;
+        e = end;
+        k = 16u;
+    }
+    __device__ __forceinline__ uint32_t next() {
+        if (k == 16u) {
+            if (i + 16u <= e && (((uintptr_t)(p + i)) & 15u) == 0u) {
+                v = *(const uint4*)(p + i);
+            } else {
+                uint32_t b[4] = {0u, 0u, 0u, 0u};
+                for (uint32_t q = 0; q < 16u && i + q < e; ++q) b[q >> 2] |= (uint32_t)p[i + q] << (8u * (q & 3u));
+                v = make_uint4(b[0], b[1], b[2], b[3]);
+            }
+            k = 0u;
+            i += 16u;
+        }
+        const uint32_t w = k < 8u ? (k < 4u ? v.x : v.y) : (k < 12u ? v.z : v.w);
+        const uint32_t s = (w >> (8u * (k & 3u))) & 0xFFu;
+        ++k;
+        return s;
+    }
+};
+
+__device__ __forceinline__ uint64_t chunk_begin(uint32_t c) { return (uint64_t)c * kEC; }
+
+// Exit state and bits of chunk c along its current record.
+__device__ __forceinline__ uint32_t enc_exit(const EncWork& w, const EncPlan& p, uint32_t c) {
+    const uint32_t g = p.chunk_base + c;
+    return w.mq[g] == kNoneE ? w.wexit[g] : w.sexit[g];
+}
+__device__ __forceinline__ uint32_t enc_bits(const EncWork& w, const EncPlan& p, uint32_t c) {
+    const uint32_t g = p.chunk_base + c;
+    const uint32_t m = w.mq[g];
+    if (m == kNoneE) return w.wtot[g];
+    return w.wbits[g] + (w.stot[g] - w.ckb[p.ck_base + (uint64_t)c * kECK + m]);
+}
+
+// Walk chunk c (c >= 1) from state e until it meets the stored spec state at a
+// checkpoint; records entry, and mq / wbits or (no meeting) wexit / wtot.
+__device__ void enc_walk(const EncWork& w, const EncPlan& p, const EncTab& t, uint32_t c,
+                         uint32_t e, uint32_t* bad) {
+    const uint32_t g = p.chunk_base + c;
+    const uint64_t b = chunk_begin(c), end = min(p.n, b + kEC);
+    const uint64_t q0 = p.ck_base + (uint64_t)c * kECK;
+    w.entry[g] = e;
+    if (w.ckx[q0] == e) {
+        w.mq[g] = 0u;
+        w.wbits[g] = 0u;
+        return;
+    }
+    SymReader rd;
+    rd.init(p.sym, b, end);
+    uint32_t x = e, bits = 0u;
+    const uint32_t n = (uint32_t)(end - b);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (i && (i % kEK) == 0u && w.ckx[q0 + i / kEK] == x) {
+            w.mq[g] = i / kEK;
+            w.wbits[g] = bits;
+            return;
+        }
+        uint32_t code, nb;
+        x = enc_step(t, p.L, rd.next(), x, &code, &nb, bad);
+        bits += nb;
+    }
+    w.mq[g] = kNoneE;
+    w.wexit[g] = x;
+    w.wtot[g] = bits;
+}
+
+__global__ __launch_bounds__(kEWG) void k_enc_sync(EncWork w) {
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t s_exit[kEWG];
+    __shared__ uint32_t s_job;
+    if (threadIdx.x == 0) s_job = job_of_task(w, blockIdx.x);
+    __syncthreads();
+    const EncPlan p = w.plans[__builtin_amdgcn_readfirstlane(s_job)];
+    EncTab t;
+    const uint32_t st = stage_enc(p, lds, t);
+    const uint32_t c0 = (blockIdx.x - p.task_base) * kEWG;
+    const uint32_t c = c0 + threadIdx.x;
+    const bool valid = c < p.n_chunks;
+    uint32_t bad = 0u;
+    uint32_t x = p.L;
+    if (valid && !st) {
+        // spec from the start state, every checkpoint stored
+        const uint64_t b = chunk_begin(c), end = min(p.n, b + kEC);
+        const uint64_t q0 = p.ck_base + (uint64_t)c * kECK;
+        const uint32_t g = p.chunk_base + c;
+        SymReader rd;
+        rd.init(p.sym, b, end);
+        uint32_t bits = 0u;
+        const uint32_t n = (uint32_t)(end - b);
+        for (uint32_t i = 0; i < n; ++i) {
+            if ((i % kEK) == 0u) {
+                w.ckx[q0 + i / kEK] = x;
+                w.ckb[q0 + i / kEK] = bits;
+            }
+            uint32_t code, nb;
+            x = enc_step(t, p.L, rd.next(), x, &code, &nb, &bad);
+            bits += nb;
+        }
+        w.sexit[g] = x;
+        w.stot[g] = bits;
+        w.ff[g] = 0u;
+        if (c == 0u) {
+            w.entry[g] = p.L;
+            w.mq[g] = 0u;
+            w.wbits[g] = 0u;
+        }
+    }
+    s_exit[threadIdx.x] = x;
+    __syncthreads();
+    if (valid && !st && c > 0u) {
+        if (c == c0) {
+            w.entry[p.chunk_base + c] = kNoneE;     // range seam: left to k_enc_fix
+            w.mq[p.chunk_base + c] = 0u;
+            w.wbits[p.chunk_base + c] = 0u;
+        } else {
+            enc_walk(w, p, t, c, s_exit[threadIdx.x - 1u], &bad);
+        }
+    }
+    if (bad || (st && threadIdx.x == 0)) atomicOr(&w.res[s_job].status, bad ? kEncNoCode : st);
+}
+
+// Per stream: consistency rounds, scan, results, zeroed shared units.
+__global__ __launch_bounds__(kEWG) void k_enc_fix(EncWork w) {
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t s_n, s_n2;
+    __shared__ uint64_t s_w[kEWG / 64];
+    const uint32_t j = blockIdx.x;
+    const EncPlan p = w.plans[j];
+    mans_encode_result* r = &w.res[j];
+    if (p.n == 0u) {                       // no symbols: one zero unit, state L
+        if (threadIdx.x == 0) {
+            r->n_units = 1u;
+            r->first_bit = 0u;
+            r->first_state = p.L;
+            if (p.cap < 1u) r->status |= kEncCapacity; else p.units[0] = 0u;
+        }
+        return;
+    }
+    EncTab t;
+    const uint32_t st = stage_enc(p, lds, t);
+    if (st || r->status) return;           // status set before this kernel (uniform read)
+    uint32_t* list = w.fl + p.chunk_base;
+    uint32_t* next = w.fl2 + p.chunk_base;
+    uint32_t bad = 0u;
+    if (threadIdx.x == 0) s_n = 0u;
+    __syncthreads();
+    for (uint32_t c = 1u + threadIdx.x; c < p.n_chunks; c += kEWG)
+        if (w.entry[p.chunk_base + c] != enc_exit(w, p, c - 1u)) list[atomicAdd(&s_n, 1u)] = c;
+    __syncthreads();
+    while (s_n) {
+        const uint32_t n = s_n;
+        for (uint32_t i = threadIdx.x; i < n; i += kEWG) {
+            next[i] = enc_exit(w, p, list[i] - 1u);
+            w.ff[p.chunk_base + list[i]] = 1u;      // listed this round
+        }
+        __syncthreads();                    // entries captured before any walk writes
+        for (uint32_t i = threadIdx.x; i < n; i += kEWG) enc_walk(w, p, t, list[i], next[i], &bad);
+        __threadfence_block();
+        __syncthreads();
+        if (threadIdx.x == 0) s_n2 = 0u;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += kEWG) {
+            const uint32_t c = list[i];
+            if (w.entry[p.chunk_base + c] != enc_exit(w, p, c - 1u)) next[atomicAdd(&s_n2, 1u)] = c;
+            if (c + 1u < p.n_chunks && !w.ff[p.chunk_base + c + 1u] &&
+                w.entry[p.chunk_base + c + 1u] != enc_exit(w, p, c))
+                next[atomicAdd(&s_n2, 1u)] = c + 1u;       // each chunk at most once
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += kEWG) w.ff[p.chunk_base + list[i]] = 0u;
+        if (threadIdx.x == 0) s_n = s_n2;
+        uint32_t* tmp = list; list = next; next = tmp;
+        __syncthreads();
+    }
+    // exclusive scan of bits per chunk
+    uint64_t carry = 0u;
+    for (uint32_t b0 = 0; b0 < p.n_chunks; b0 += kEWG) {
+        const uint32_t c = b0 + threadIdx.x;
+        const uint64_t v = c < p.n_chunks ? enc_bits(w, p, c) : 0u;
+        uint64_t incl = v;
+        const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+        for (uint32_t d = 1; d < 64u; d <<= 1) {
+            const uint64_t o = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += o;
+        }
+        if (lane == 63u) s_w[wv] = incl;
+        __syncthreads();
+        uint64_t add = carry;
+        for (uint32_t k = 0; k < wv; ++k) add += s_w[k];
+        if (c < p.n_chunks) w.off[p.chunk_base + c] = add + incl - v;
+        carry += s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        __syncthreads();
+    }
+    const uint64_t T = carry;
+    const uint64_t nu = T / 32u + 1u;
+    if (threadIdx.x == 0) {
+        r->n_units = nu;
+        r->first_bit = (uint32_t)(T % 32u);
+        r->first_state = enc_exit(w, p, p.n_chunks - 1u);
+        if (nu > p.cap) r->status |= kEncCapacity;
+        if (bad) r->status |= kEncNoCode;
+    }
+    if (nu > p.cap) return;
+    // units shared by chunks (and the last one) start at zero: the write pass ORs into them
+    for (uint32_t c = threadIdx.x; c < p.n_chunks; c += kEWG) {
+        const uint64_t u = w.off[p.chunk_base + c] / 32u;
+        p.units[nu - 1u - u] = 0u;
+    }
+    if (threadIdx.x == 0) p.units[0] = 0u;      // forward unit T / 32
+}
+
+__global__ __launch_bounds__(kEWG) void k_enc_write(EncWork w) {
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t s_job;
+    if (threadIdx.x == 0) s_job = job_of_task(w, blockIdx.x);
+    __syncthreads();
+    const EncPlan p = w.plans[__builtin_amdgcn_readfirstlane(s_job)];
+    if (w.res[s_job].status) return;            // uniform: written by k_enc_fix
+    EncTab t;
+    if (stage_enc(p, lds, t)) return;
+    const uint32_t c = (blockIdx.x - p.task_base) * kEWG + threadIdx.x;
+    if (c >= p.n_chunks) return;
+    const uint64_t nu = w.res[s_job].n_units;
+    const uint64_t b = chunk_begin(c), end = min(p.n, b + kEC);
+    uint32_t x = c == 0u ? p.L : w.entry[p.chunk_base + c];
+    const uint64_t pos = w.off[p.chunk_base + c];
+    uint64_t u = pos / 32u;                      // forward index of the unit being filled
+    uint32_t fill = (uint32_t)(pos % 32u);       // bits of that unit before this chunk
+    bool shared = fill != 0u;                    // its leading bits belong to a predecessor
+    uint64_t acc = 0u;
+    uint32_t bad = 0u;
+    SymReader rd;
+    rd.init(p.sym, b, end);
+    const uint32_t n = (uint32_t)(end - b);
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t code, nb;
+        x = enc_step(t, p.L, rd.next(), x, &code, &nb, &bad);
+        acc = (acc << nb) | code;
+        fill += nb;
+        if (fill >= 32u) {
+            fill -= 32u;
+            const uint32_t word = (uint32_t)(acc >> fill);
+            acc &= (1ull << fill) - 1u;
+            uint32_t* dst = p.units + (nu - 1u - u);
+            if (u < nu) {                        // always, when the records are right
+                if (shared) atomicOr(dst, word); else *dst = word;
+            }
+            shared = false;
+            ++u;
+        }
+    }
+    if (fill) {
+        // the unit this chunk ends in is shared with the next chunk (or is the last one):
+        // acc holds this chunk's bits of it, placed below any predecessor's bits
+        const uint32_t word = (uint32_t)(acc << (32u - fill));
+        if (u < nu) atomicOr(p.units + (nu - 1u - u), word);
+    }
+}
+
+uint64_t align256(uint64_t v) { return (v + 255u) & ~255ull; }
+
+struct EncLayout {
+    uint64_t plans, res, chunk[10], off, ckx, ckb, bytes;
+};
+
+EncLayout enc_layout(size_t n_jobs, uint64_t chunks, uint64_t cks) {
+    EncLayout l;
+    uint64_t o = 0;
+    auto take = [&](uint64_t b) { uint64_t r = o; o = align256(o + b); return r; };
+    l.plans = take(sizeof(EncPlan) * n_jobs);
+    l.res = take(sizeof(mans_encode_result) * n_jobs);
+    for (int i = 0; i < 10; ++i) l.chunk[i] = take(4 * chunks + 4);
+    l.off = take(8 * chunks + 8);
+    l.ckx = take(4 * cks + 4);
+    l.ckb = take(4 * cks + 4);
+    l.bytes = o;
+    return l;
+}
+
+int enc_plans(const mans_encode_job* jobs, size_t n, std::vector<EncPlan>& plans,
+              uint64_t* chunks, uint64_t* cks, uint32_t* tasks, uint32_t* maxL) {
+    if (!jobs && n) return set_error(MANS_EINVAL, "jobs is NULL");
+    if (n == 0 || n > 0xFFFFFFu) return set_error(MANS_EINVAL, "%zu jobs", n);
+    plans.resize(n);
+    uint64_t cb = 0, qb = 0, tb = 0;
+    uint32_t ml = 2;
+    for (size_t i = 0; i < n; ++i) {
+        const mans_encode_job& jb = jobs[i];
+        if (jb.L < 2u || jb.L > 16384u || (jb.L & (jb.L - 1u)))
+            return set_error(MANS_EINVAL, "job %zu: L=%u is not a power of two in [2, 16384]", i, jb.L);
+        if (!jb.table || !jb.units || (jb.n && !jb.symbols))
+            return set_error(MANS_EINVAL, "job %zu: NULL pointer", i);
+        if (jb.n >= (1ull << 42)) return set_error(MANS_EINVAL, "job %zu: n too large", i);
+        EncPlan& p = plans[i];
+        std::memset(&p, 0, sizeof(p));
+        p.sym = (const uint8_t*)jb.symbols;
+        p.table = jb.table;
+        p.units = jb.units;
+        p.n = jb.n;
+        p.cap = jb.units_capacity;
+        p.L = jb.L;
+        uint32_t R = 0;
+        while ((1u << R) < jb.L) ++R;
+        p.R = R;
+        const uint64_t nc = (jb.n + kEC - 1u) / kEC;
+        p.n_chunks = (uint32_t)nc;
+        p.chunk_base = (uint32_t)cb;
+        p.ck_base = qb;
+        p.task_base = (uint32_t)tb;
+        cb += std::max<uint64_t>(nc, 1u);
+        qb += std::max<uint64_t>(nc, 1u) * kECK;
+        tb += std::max<uint64_t>((nc + kEWG - 1u) / kEWG, 1u);
+        ml = std::max(ml, jb.L);
+    }
+    if (cb >= 0xFFFFFFF0ull || tb >= 0x7FFFFFFFull) return set_error(MANS_EINVAL, "too many chunks");
+    *chunks = cb;
+    *cks = qb;
+    *tasks = (uint32_t)tb;
+    *maxL = ml;
+    return MANS_OK;
+}
+
+EncWork enc_work(void* ws, const EncLayout& l, size_t n_jobs, uint32_t tasks) {
+    char* b = (char*)ws;
+    EncWork w;
+    w.plans = (const EncPlan*)(b + l.plans);
+    w.n_jobs = (uint32_t)n_jobs;
+    w.n_tasks = tasks;
+    uint32_t** pc[10] = {&w.entry, &w.mq, &w.wbits, &w.wexit, &w.wtot, &w.sexit, &w.stot, &w.fl,
+                         &w.fl2, &w.ff};
+    for (int i = 0; i < 10; ++i) *pc[i] = (uint32_t*)(b + l.chunk[i]);
+    w.off = (uint64_t*)(b + l.off);
+    w.ckx = (uint32_t*)(b + l.ckx);
+    w.ckb = (uint32_t*)(b + l.ckb);
+    w.res = (mans_encode_result*)(b + l.res);
+    return w;
+}
+
+}  // namespace
+
+extern "C" int mans_encode_workspace_bytes(const mans_encode_job* jobs, size_t n_jobs,
+                                           uint64_t* bytes) {
+    if (!bytes) return set_error(MANS_EINVAL, "bytes is NULL");
+    std::vector<EncPlan> plans;
+    uint64_t chunks, cks;
+    uint32_t tasks, ml;
+    int rc = enc_plans(jobs, n_jobs, plans, &chunks, &cks, &tasks, &ml);
+    if (rc) return rc;
+    *bytes = enc_layout(n_jobs, chunks, cks).bytes;
+    return MANS_OK;
+}
+
+extern "C" int mans_encode_device(const mans_encode_job* jobs, size_t n_jobs, void* workspace,
+                                  uint64_t workspace_bytes, void* stream) {
+    std::vector<EncPlan> plans;
+    uint64_t chunks, cks;
+    uint32_t tasks, ml;
+    int rc = enc_plans(jobs, n_jobs, plans, &chunks, &cks, &tasks, &ml);
+    if (rc) return rc;
+    const EncLayout l = enc_layout(n_jobs, chunks, cks);
+    if (!workspace || workspace_bytes < l.bytes)
+        return set_error(MANS_ENOMEM, "workspace of %llu bytes < %llu needed",
+                         (unsigned long long)workspace_bytes, (unsigned long long)l.bytes);
+    hipStream_t st = (hipStream_t)stream;
+    const EncWork w = enc_work(workspace, l, n_jobs, tasks);
+    // plans, and zeroed results (status words are ORed into)
+    std::vector<char> img(l.chunk[0] - l.plans, 0);
+    std::memcpy(img.data(), plans.data(), sizeof(EncPlan) * n_jobs);
+    hipError_t e = hipMemcpyAsync((char*)workspace + l.plans, img.data(), img.size(),
+                                  hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);    // the host image is freed on return
+    if (e != hipSuccess) return set_error(MANS_EHIP, "plan upload: %s", hipGetErrorString(e));
+    const size_t lds = enc_lds_bytes(ml);
+    hipLaunchKernelGGL(k_enc_sync, dim3(tasks), dim3(kEWG), lds, st, w);
+    hipLaunchKernelGGL(k_enc_fix, dim3((uint32_t)n_jobs), dim3(kEWG), lds, st, w);
+    hipLaunchKernelGGL(k_enc_write, dim3(tasks), dim3(kEWG), lds, st, w);
+    e = hipGetLastError();
+    if (e != hipSuccess) return set_error(MANS_EHIP, "encode launch: %s", hipGetErrorString(e));
+    return MANS_OK;
+}
+
+extern "C" int mans_encode_results(const void* workspace, const mans_encode_job* jobs,
+                                   size_t n_jobs, void* stream, mans_encode_result* out) {
+    if (!workspace || !out) return set_error(MANS_EINVAL, "NULL argument");
+    std::vector<EncPlan> plans;
+    uint64_t chunks, cks;
+    uint32_t tasks, ml;
+    int rc = enc_plans(jobs, n_jobs, plans, &chunks, &cks, &tasks, &ml);
+    if (rc) return rc;
+    const EncLayout l = enc_layout(n_jobs, chunks, cks);
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = hipMemcpyAsync(out, (const char*)workspace + l.res,
+                                  sizeof(mans_encode_result) * n_jobs, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return set_error(MANS_EHIP, "results: %s", hipGetErrorString(e));
+    return MANS_OK;
+}
