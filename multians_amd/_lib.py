@@ -120,6 +120,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("libmultians_amd.so not built: run `python -m multians_amd.build`")
+    try:
+        # torch loads its own copy of the HIP runtime (same soname, by path); loaded
+        # first, it also serves this library.  Loaded the other way round, the process
+        # would hold two HIP runtimes, and the second finds no device.
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     l = C.CDLL(LIB_PATH)
     vp, u64, u32, i32 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int
     sig = {

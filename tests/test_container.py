@@ -75,7 +75,8 @@ def test_container_exports_everything_declared():
     src = open(os.path.join(ROOT, "include", "mans_container.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     names = sorted(set(re.findall(r"\b(mans_[a-z0-9_]+)\s*\(", src)))
-    lib = C.CDLL(os.path.join(ROOT, "multians_amd", "lib", "libmultians_amd.so"))
+    from multians_amd import _lib
+    lib = _lib.lib()
     assert [n for n in names if not hasattr(lib, n)] == []
     assert sorted(container.EXPORTS) == names
 

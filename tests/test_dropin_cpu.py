@@ -28,7 +28,8 @@ def _declared(header):
 
 
 def test_c_abi_exports_everything_declared():
-    lib = C.CDLL(os.path.join(LIB, "libmultians_amd.so"))
+    from multians_amd import _lib
+    lib = _lib.lib()
     names = _declared("mans_c_api.h")
     assert len(names) > 20
     missing = [n for n in names if not hasattr(lib, n)]
