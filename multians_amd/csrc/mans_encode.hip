@@ -156,11 +156,10 @@ __device__ __forceinline__ uint32_t enc_step(const EncTab& t, uint32_t L, uint32
     return L + t.ys[(info >> 16) + (x >> n) - f];
 }
 
-// Symbols of one chunk, read 16 at a time.
+// Symbols of one chunk, read 16 at a time (two u64 in registers, bytes taken by shifts).
 struct SymReader {
     const uint8_t* p;
-    uint64_t i, e;
-    uint4 v;
+    uint64_t i, e, lo, hi;
     uint32_t k;
     __device__ void init(const uint8_t* base, uint64_t b, uint64_t end) {
         p = base;
@@ -171,17 +170,22 @@ struct SymReader {
     __device__ __forceinline__ uint32_t next() {
         if (k == 16u) {
             if (i + 16u <= e && (((uintptr_t)(p + i)) & 15u) == 0u) {
-                v = *(const uint4*)(p + i);
-            } else {
-                uint32_t b[4] = {0u, 0u, 0u, 0u};
-                for (uint32_t q = 0; q < 16u && i + q < e; ++q) b[q >> 2] |= (uint32_t)p[i + q] << (8u * (q & 3u));
-                v = make_uint4(b[0], b[1], b[2], b[3]);
+                const uint4 v = *(const uint4*)(p + i);
+                lo = (uint64_t)v.x | ((uint64_t)v.y << 32);
+                hi = (uint64_t)v.z | ((uint64_t)v.w << 32);
+            } else {                       // edge or unaligned: byte loads
+                lo = 0u;
+                hi = 0u;
+                for (uint32_t q = 0; q < 16u && i + q < e; ++q) {
+                    const uint64_t byte = p[i + q];
+                    if (q < 8u) lo |= byte << (8u * q); else hi |= byte << (8u * (q - 8u));
+                }
             }
             k = 0u;
             i += 16u;
         }
-        const uint32_t w = k < 8u ? (k < 4u ? v.x : v.y) : (k < 12u ? v.z : v.w);
-        const uint32_t s = (w >> (8u * (k & 3u))) & 0xFFu;
+        const uint64_t word = k < 8u ? lo : hi;
+        const uint32_t s = (uint32_t)(word >> (8u * (k & 7u))) & 0xFFu;
         ++k;
         return s;
     }
@@ -202,35 +206,38 @@ __device__ __forceinline__ uint32_t enc_bits(const EncWork& w, const EncPlan& p,
 }
 
 // Walk chunk c (c >= 1) from state e until it meets the stored spec state at a
-// checkpoint; records entry, and mq / wbits or (no meeting) wexit / wtot.
+// checkpoint; records entry, and mq / wbits or (no meeting) wexit / wtot.  Every lane
+// stores all five fields once, at the end: with the stores in the two exits, the device
+// compiler merged them into stores whose base pointers it selected per lane from a
+// scratch copy of the work struct, and records came out mixed up.
 __device__ void enc_walk(const EncWork& w, const EncPlan& p, const EncTab& t, uint32_t c,
                          uint32_t e, uint32_t* bad) {
     const uint32_t g = p.chunk_base + c;
     const uint64_t b = chunk_begin(c), end = min(p.n, b + kEC);
     const uint64_t q0 = p.ck_base + (uint64_t)c * kECK;
-    w.entry[g] = e;
+    uint32_t mq = kNoneE, wb = 0u, x = e, bits = 0u;
     if (w.ckx[q0] == e) {
-        w.mq[g] = 0u;
-        w.wbits[g] = 0u;
-        return;
-    }
-    SymReader rd;
-    rd.init(p.sym, b, end);
-    uint32_t x = e, bits = 0u;
-    const uint32_t n = (uint32_t)(end - b);
-    for (uint32_t i = 0; i < n; ++i) {
-        if (i && (i % kEK) == 0u && w.ckx[q0 + i / kEK] == x) {
-            w.mq[g] = i / kEK;
-            w.wbits[g] = bits;
-            return;
+        mq = 0u;
+    } else {
+        SymReader rd;
+        rd.init(p.sym, b, end);
+        const uint32_t n = (uint32_t)(end - b);
+        for (uint32_t i = 0; i < n; ++i) {
+            if (i && (i % kEK) == 0u && w.ckx[q0 + i / kEK] == x) {
+                mq = i / kEK;
+                wb = bits;
+                break;
+            }
+            uint32_t code, nb;
+            x = enc_step(t, p.L, rd.next(), x, &code, &nb, bad);
+            bits += nb;
         }
-        uint32_t code, nb;
-        x = enc_step(t, p.L, rd.next(), x, &code, &nb, bad);
-        bits += nb;
     }
-    w.mq[g] = kNoneE;
-    w.wexit[g] = x;
-    w.wtot[g] = bits;
+    w.entry[g] = e;
+    w.mq[g] = mq;
+    w.wbits[g] = wb;
+    w.wexit[g] = mq == kNoneE ? x : 0u;
+    w.wtot[g] = mq == kNoneE ? bits : 0u;
 }
 
 __global__ __launch_bounds__(kEWG) void k_enc_sync(EncWork w) {
@@ -268,21 +275,21 @@ __global__ __launch_bounds__(kEWG) void k_enc_sync(EncWork w) {
         w.sexit[g] = x;
         w.stot[g] = bits;
         w.ff[g] = 0u;
-        if (c == 0u) {
-            w.entry[g] = p.L;
-            w.mq[g] = 0u;
-            w.wbits[g] = 0u;
-        }
     }
     s_exit[threadIdx.x] = x;
     __syncthreads();
-    if (valid && !st && c > 0u) {
-        if (c == c0) {
-            w.entry[p.chunk_base + c] = kNoneE;     // range seam: left to k_enc_fix
-            w.mq[p.chunk_base + c] = 0u;
-            w.wbits[p.chunk_base + c] = 0u;
-        } else {
+    if (valid && !st) {
+        if (c > c0) {
             enc_walk(w, p, t, c, s_exit[threadIdx.x - 1u], &bad);
+        } else {
+            // chunk 0 starts on the true path; a range's first chunk (a seam) gets an
+            // entry that matches no exit, so k_enc_fix walks it
+            const uint32_t g = p.chunk_base + c;
+            w.entry[g] = c == 0u ? p.L : kNoneE;
+            w.mq[g] = 0u;
+            w.wbits[g] = 0u;
+            w.wexit[g] = 0u;
+            w.wtot[g] = 0u;
         }
     }
     if (bad || (st && threadIdx.x == 0)) atomicOr(&w.res[s_job].status, bad ? kEncNoCode : st);
