@@ -97,7 +97,7 @@ def test_encode_batch_and_tables():
         cases.append((sym, L))
         jobs.append(job)
     res = _lib.encode_device(jobs)
-    for (sym, L), r, j in zip(cases, res):
+    for (sym, L), r, j in zip(cases, res, jobs):
         _check(sym, L, r, j)
 
 
