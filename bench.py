@@ -226,6 +226,64 @@ def sync_point_leg(args, torch, b, items):
     return out
 
 
+def bench_encode(args, torch, items):
+    """Encode every stream of the batch on the device (mans_encode_device, one call, each
+    stream with its own table) from symbols resident in HBM; checked unit-exact against
+    the host codec's streams."""
+    from multians_amd import _lib
+    n = len(items)
+    soff = np.zeros(n + 1, np.int64)
+    uoff = np.zeros(n + 1, np.int64)
+    for i, (es, _) in enumerate(items):
+        soff[i + 1] = soff[i] + ((es.n_symbols + 15) // 16) * 16
+        uoff[i + 1] = uoff[i] + ((_lib.lib().mans_max_encoded_units(es.n_symbols) + 3) // 4) * 4
+    sym = torch.empty(int(soff[-1]), dtype=torch.uint8, device="cuda")
+    for i, (es, s) in enumerate(items):
+        sym[int(soff[i]):int(soff[i]) + es.n_symbols].copy_(torch.from_numpy(s))
+    toff = np.concatenate([[0], np.cumsum([es.L for es, _ in items])])
+    tables = torch.from_numpy(np.concatenate([es.table for es, _ in items]).view(np.int32)).cuda()
+    units = torch.empty(int(uoff[-1]), dtype=torch.int32, device="cuda")
+    js = (_lib.EncodeJob * n)()
+    sb, tb, ub = sym.data_ptr(), tables.data_ptr(), units.data_ptr()
+    for i, (es, _) in enumerate(items):
+        j = js[i]
+        j.symbols, j.n = sb + int(soff[i]), es.n_symbols
+        j.table, j.L = tb + 4 * int(toff[i]), es.L
+        j.units, j.units_capacity = ub + 4 * int(uoff[i]), int(uoff[i + 1] - uoff[i])
+    b = C.c_uint64()
+    _lib.check(_lib.lib().mans_encode_workspace_bytes(js, n, C.byref(b)))
+    ws = torch.empty(b.value, dtype=torch.uint8, device="cuda")
+    L = _lib.lib()
+    st = torch.cuda.current_stream().cuda_stream
+
+    def enc():
+        _lib.check(L.mans_encode_device(js, n, C.c_void_p(ws.data_ptr()), ws.numel(),
+                                        C.c_void_p(st)))
+
+    enc()
+    res = (_lib.EncodeResult * n)()
+    _lib.check(L.mans_encode_results(C.c_void_p(ws.data_ptr()), js, n, C.c_void_p(st), res))
+    host = units.cpu().numpy().view(np.uint32)
+    bad = [i for i, (es, _) in enumerate(items)
+           if res[i].status or res[i].n_units != es.n_units or res[i].first_bit != es.first_bit
+           or res[i].first_state != es.first_state
+           or not np.array_equal(host[int(uoff[i]):int(uoff[i]) + es.n_units], es.units)]
+    log("device encode unit-exact: %s" % (not bad))
+    if bad:
+        raise SystemExit("device encode mismatch: streams %s" % bad[:10])
+    for _ in range(args.warmup):
+        enc()
+    ms = time_events(torch, enc, args.steps)
+    total = sum(es.n_symbols for es, _ in items)
+    out = {"streams": n, "symbols": total, "ms": round(ms, 3),
+           "symbols_GBps": round(total / ms / 1e6, 3),
+           "note": "one mans_encode_device call (plans copied in synchronously each call)"}
+    log("encode", json.dumps(out))
+    del sym, tables, units, ws
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_single(args, torch):
     """Single-stream at-scale breakdown: 2^log2n symbols per lambda, L = 4096."""
     from multians_amd import _lib
@@ -457,6 +515,8 @@ def main():
                     help="skip the per-stream plain-launch leg")
     ap.add_argument("--no-index", action="store_true",
                     help="skip the sync-point legs (decode from recorded sync points, windows)")
+    ap.add_argument("--no-encode", action="store_true",
+                    help="skip the device-encode leg")
     ap.add_argument("--windows", type=int, default=10000,
                     help="windows of 64 Ki symbols per call in the window leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -514,6 +574,8 @@ def main():
         res["from_sync_points"] = bres["sync_points"]
     log("batched", json.dumps({k: res[k] for k in ("batched_GBps", "per_stream_plain_GBps",
                                                    "batched_vs_plain", "kernels_ms")}))
+    if not args.no_encode:
+        res["device_encode"] = bench_encode(args, torch, bitems)
     if not args.no_single:
         per, _ = bench_single(args, torch)
         res["single_stream"] = per
