@@ -77,12 +77,12 @@ __device__ uint32_t job_of_task(const EncWork& w, uint32_t t) {
     return lo;
 }
 
-// Encoder tables in LDS: per symbol f | base << 16 (base: first slot of the symbol's
-// states in ys), thr = f << m, m; ys[base + i - f] = state index y with next(y) = i.
+// Encoder tables in LDS: per symbol one 64-bit word {f | base << 16, thr | m << 16}
+// (f: the symbol's state count, base: its first slot in ys, m = log2 L - floor(log2 f),
+// thr = f << m), read with one ds_read_b64 per step; ys[base + i - f] = the state index y
+// of the symbol with next(y) = i.
 struct EncTab {
-    uint32_t* info;
-    uint32_t* thr;
-    uint32_t* mb;
+    const uint2* sym;
     uint16_t* ys;
 };
 
@@ -91,10 +91,9 @@ constexpr size_t enc_lds_bytes(uint32_t L) { return 4u * 256u * 4u + 2u * L; }
 // Builds the tables for plan p (all threads); returns status bits (uniform).
 __device__ uint32_t stage_enc(const EncPlan& p, uint32_t* lds, EncTab& t) {
     __shared__ uint32_t s_bad;
-    t.info = lds;
-    t.thr = lds + 256;
-    t.mb = lds + 512;
-    uint32_t* cnt = lds + 768;
+    uint2* sym = (uint2*)lds;                  // 256 x 8 bytes
+    uint32_t* cnt = lds + 512;
+    t.sym = sym;
     t.ys = (uint16_t*)(lds + 1024);
     if (threadIdx.x == 0) s_bad = 0u;
     cnt[threadIdx.x] = 0u;
@@ -103,31 +102,30 @@ __device__ uint32_t stage_enc(const EncPlan& p, uint32_t* lds, EncTab& t) {
     for (uint32_t y = threadIdx.x; y < p.L; y += kEWG) atomicAdd(&cnt[(p.table[y] >> 16) & 0xFFu], 1u);
     __syncthreads();
     if (threadIdx.x < 64u) {               // exclusive prefix of the counts (one wave)
-        uint32_t v[4], s = 0;
-        for (uint32_t k = 0; k < 4; ++k) { v[k] = cnt[threadIdx.x * 4u + k]; s += v[k]; }
-        uint32_t incl = s;
+        uint32_t v[4], tot = 0;
+        for (uint32_t k = 0; k < 4; ++k) { v[k] = cnt[threadIdx.x * 4u + k]; tot += v[k]; }
+        uint32_t incl = tot;
         for (uint32_t d = 1; d < 64u; d <<= 1) {
             const uint32_t o = __shfl_up(incl, d, 64);
             if (threadIdx.x >= d) incl += o;
         }
-        uint32_t run = incl - s;
+        uint32_t run = incl - tot;
         for (uint32_t k = 0; k < 4; ++k) {
             const uint32_t sy = threadIdx.x * 4u + k, f = v[k];
-            t.info[sy] = f | (run << 16);
             const uint32_t m = f ? p.R - (31u - (uint32_t)__builtin_clz(f)) : 0u;
-            t.thr[sy] = f << m;
-            t.mb[sy] = m;
+            sym[sy] = make_uint2(f | (run << 16), (f << m) | (m << 16));
             run += f;
         }
     }
     __syncthreads();
     for (uint32_t y = threadIdx.x; y < p.L; y += kEWG) {
         const uint32_t e = p.table[y];
-        const uint32_t s = (e >> 16) & 0xFFu, i = e & 0xFFFFu, f = t.info[s] & 0xFFFFu;
+        const uint32_t s = (e >> 16) & 0xFFu, i = e & 0xFFFFu;
+        const uint32_t f = sym[s].x & 0xFFFFu;
         if (i < f || i >= 2u * f) {
             s_bad = 1u;
         } else {
-            t.ys[(t.info[s] >> 16) + i - f] = (uint16_t)y;
+            t.ys[(sym[s].x >> 16) + i - f] = (uint16_t)y;
         }
     }
     __syncthreads();
@@ -138,22 +136,21 @@ __device__ uint32_t stage_enc(const EncPlan& p, uint32_t* lds, EncTab& t) {
     return s_bad ? kEncBadTable : 0u;
 }
 
-// One symbol: x (in [L, 2L)) -> new state; *code / *nb the emitted bits; *bad set when
-// the symbol has no state.
-__device__ __forceinline__ uint32_t enc_step(const EncTab& t, uint32_t L, uint32_t s, uint32_t x,
+// One symbol with table word q: x (in [L, 2L)) -> new state; *code / *nb the emitted
+// bits; *bad set when the symbol has no state.
+__device__ __forceinline__ uint32_t enc_step(const EncTab& t, uint32_t L, uint2 q, uint32_t x,
                                              uint32_t* code, uint32_t* nb, uint32_t* bad) {
-    const uint32_t info = t.info[s];
-    const uint32_t f = info & 0xFFFFu;
+    const uint32_t f = q.x & 0xFFFFu;
     if (f == 0u) {
         *bad = 1u;
         *code = 0u;
         *nb = 0u;
         return x;
     }
-    const uint32_t n = t.mb[s] - (x < t.thr[s] ? 1u : 0u);
+    const uint32_t n = (q.y >> 16) - (x < (q.y & 0xFFFFu) ? 1u : 0u);
     *nb = n;
     *code = x & ((1u << n) - 1u);
-    return L + t.ys[(info >> 16) + (x >> n) - f];
+    return L + t.ys[(q.x >> 16) + (x >> n) - f];
 }
 
 // Symbols of one chunk, read 16 at a time (two u64 in registers, bytes taken by shifts).
@@ -222,14 +219,17 @@ __device__ void enc_walk(const EncWork& w, const EncPlan& p, const EncTab& t, ui
         SymReader rd;
         rd.init(p.sym, b, end);
         const uint32_t n = (uint32_t)(end - b);
+        uint2 q = t.sym[rd.next()];
         for (uint32_t i = 0; i < n; ++i) {
             if (i && (i % kEK) == 0u && w.ckx[q0 + i / kEK] == x) {
                 mq = i / kEK;
                 wb = bits;
                 break;
             }
+            const uint2 cur = q;
+            if (i + 1u < n) q = t.sym[rd.next()];      // next symbol's word, ahead
             uint32_t code, nb;
-            x = enc_step(t, p.L, rd.next(), x, &code, &nb, bad);
+            x = enc_step(t, p.L, cur, x, &code, &nb, bad);
             bits += nb;
         }
     }
@@ -263,13 +263,16 @@ __global__ __launch_bounds__(kEWG) void k_enc_sync(EncWork w) {
         rd.init(p.sym, b, end);
         uint32_t bits = 0u;
         const uint32_t n = (uint32_t)(end - b);
+        uint2 q = t.sym[rd.next()];
         for (uint32_t i = 0; i < n; ++i) {
             if ((i % kEK) == 0u) {
                 w.ckx[q0 + i / kEK] = x;
                 w.ckb[q0 + i / kEK] = bits;
             }
+            const uint2 cur = q;
+            if (i + 1u < n) q = t.sym[rd.next()];
             uint32_t code, nb;
-            x = enc_step(t, p.L, rd.next(), x, &code, &nb, &bad);
+            x = enc_step(t, p.L, cur, x, &code, &nb, &bad);
             bits += nb;
         }
         w.sexit[g] = x;
@@ -408,9 +411,12 @@ __global__ __launch_bounds__(kEWG) void k_enc_write(EncWork w) {
     SymReader rd;
     rd.init(p.sym, b, end);
     const uint32_t n = (uint32_t)(end - b);
+    uint2 q = t.sym[rd.next()];
     for (uint32_t i = 0; i < n; ++i) {
+        const uint2 cur = q;
+        if (i + 1u < n) q = t.sym[rd.next()];
         uint32_t code, nb;
-        x = enc_step(t, p.L, rd.next(), x, &code, &nb, &bad);
+        x = enc_step(t, p.L, cur, x, &code, &nb, &bad);
         acc = (acc << nb) | code;
         fill += nb;
         if (fill >= 32u) {
