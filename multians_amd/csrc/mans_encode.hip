@@ -86,7 +86,7 @@ struct EncTab {
     uint16_t* ys;
 };
 
-constexpr size_t enc_lds_bytes(uint32_t L) { return 4u * 256u * 4u + 2u * L; }
+__host__ __device__ constexpr size_t enc_lds_bytes(uint32_t L) { return 4u * 256u * 4u + 2u * L; }
 
 // Builds the tables for plan p (all threads); returns status bits (uniform).
 __device__ uint32_t stage_enc(const EncPlan& p, uint32_t* lds, EncTab& t) {
@@ -190,6 +190,67 @@ struct SymReader {
 
 __device__ __forceinline__ uint64_t chunk_begin(uint32_t c) { return (uint64_t)c * kEC; }
 
+// Symbols staged through LDS for loops in which every lane of a workgroup takes all of
+// its chunk's symbols in order (spec and write passes): rounds of kER symbols per lane.
+// The 16-byte pieces of all lanes' segments are loaded by consecutive threads (kEP
+// threads per 64-byte segment, instead of each lane loading its own 16 bytes at a 4 KiB
+// stride), into registers one round ahead, then through LDS to their lane.
+constexpr uint32_t kER = 64;            // symbols per lane per round
+constexpr uint32_t kEP = kER / 16;      // 16-byte pieces per lane segment
+constexpr size_t kStageBytes = (size_t)kEWG * kER;
+
+__device__ __forceinline__ size_t stage_offset(uint32_t L) { return (enc_lds_bytes(L) + 15u) & ~15ull; }
+
+__device__ __forceinline__ uint4 load_piece(const EncPlan& p, uint32_t c, uint64_t o) {
+    const uint64_t end = min(p.n, chunk_begin(c) + kEC);
+    if (o + 16u <= end && (((uintptr_t)(p.sym + o)) & 15u) == 0u) return *(const uint4*)(p.sym + o);
+    uint64_t lo = 0u, hi = 0u;                 // edge or unaligned: byte loads
+    for (uint32_t q = 0; q < 16u && o + q < end; ++q) {
+        const uint64_t byte = p.sym[o + q];
+        if (q < 8u) lo |= byte << (8u * q); else hi |= byte << (8u * (q - 8u));
+    }
+    return make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+}
+
+__device__ __forceinline__ void stage_load(const EncPlan& p, uint32_t c0, uint32_t r, uint4* pf) {
+#pragma unroll
+    for (uint32_t j = 0; j < kEP; ++j) {
+        const uint32_t idx = threadIdx.x + kEWG * j;
+        const uint32_t c = c0 + idx / kEP;
+        pf[j] = c < p.n_chunks ? load_piece(p, c, chunk_begin(c) + (uint64_t)r * kER + 16u * (idx % kEP))
+                               : make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+
+// Runs f(i, symbol) for i = 0 .. n-1 of this lane's chunk (lanes with run == false only
+// take part in the loads and barriers).  All threads of the workgroup must call it.
+template <typename F>
+__device__ void staged_symbols(const EncPlan& p, uint32_t c0, uint4* sbuf, bool run, uint32_t n,
+                               F&& f) {
+    const uint64_t left = p.n - chunk_begin(c0);
+    const uint32_t rounds = (uint32_t)((min((uint64_t)kEC, left) + kER - 1u) / kER);
+    uint4 pf[kEP];
+    stage_load(p, c0, 0u, pf);
+    for (uint32_t r = 0; r < rounds; ++r) {
+        __syncthreads();                       // previous round's reads done
+#pragma unroll
+        for (uint32_t j = 0; j < kEP; ++j) sbuf[threadIdx.x + kEWG * j] = pf[j];
+        __syncthreads();
+        if (r + 1u < rounds) stage_load(p, c0, r + 1u, pf);
+        if (!run) continue;
+#pragma unroll
+        for (uint32_t k = 0; k < kEP; ++k) {
+            const uint4 v = sbuf[threadIdx.x * kEP + k];
+            const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (uint32_t b = 0; b < 16u; ++b) {
+                const uint32_t i = r * kER + k * 16u + b;
+                if (i < n) f(i, (wd[b >> 2] >> (8u * (b & 3u))) & 0xFFu);
+            }
+        }
+    }
+}
+
 // Exit state and bits of chunk c along its current record.
 __device__ __forceinline__ uint32_t enc_exit(const EncWork& w, const EncPlan& p, uint32_t c) {
     const uint32_t g = p.chunk_base + c;
@@ -254,30 +315,27 @@ __global__ __launch_bounds__(kEWG) void k_enc_sync(EncWork w) {
     const bool valid = c < p.n_chunks;
     uint32_t bad = 0u;
     uint32_t x = p.L;
-    if (valid && !st) {
+    if (!st) {
         // spec from the start state, every checkpoint stored
-        const uint64_t b = chunk_begin(c), end = min(p.n, b + kEC);
         const uint64_t q0 = p.ck_base + (uint64_t)c * kECK;
-        const uint32_t g = p.chunk_base + c;
-        SymReader rd;
-        rd.init(p.sym, b, end);
+        const uint32_t n = valid ? (uint32_t)(min(p.n, chunk_begin(c) + kEC) - chunk_begin(c)) : 0u;
         uint32_t bits = 0u;
-        const uint32_t n = (uint32_t)(end - b);
-        uint2 q = t.sym[rd.next()];
-        for (uint32_t i = 0; i < n; ++i) {
+        uint4* sbuf = (uint4*)((char*)lds + stage_offset(p.L));
+        staged_symbols(p, c0, sbuf, valid, n, [&](uint32_t i, uint32_t sy) {
             if ((i % kEK) == 0u) {
                 w.ckx[q0 + i / kEK] = x;
                 w.ckb[q0 + i / kEK] = bits;
             }
-            const uint2 cur = q;
-            if (i + 1u < n) q = t.sym[rd.next()];
             uint32_t code, nb;
-            x = enc_step(t, p.L, cur, x, &code, &nb, &bad);
+            x = enc_step(t, p.L, t.sym[sy], x, &code, &nb, &bad);
             bits += nb;
+        });
+        if (valid) {
+            const uint32_t g = p.chunk_base + c;
+            w.sexit[g] = x;
+            w.stot[g] = bits;
+            w.ff[g] = 0u;
         }
-        w.sexit[g] = x;
-        w.stot[g] = bits;
-        w.ff[g] = 0u;
     }
     s_exit[threadIdx.x] = x;
     __syncthreads();
@@ -397,26 +455,26 @@ __global__ __launch_bounds__(kEWG) void k_enc_write(EncWork w) {
     if (w.res[s_job].status) return;            // uniform: written by k_enc_fix
     EncTab t;
     if (stage_enc(p, lds, t)) return;
-    const uint32_t c = (blockIdx.x - p.task_base) * kEWG + threadIdx.x;
-    if (c >= p.n_chunks) return;
+    const uint32_t c0 = (blockIdx.x - p.task_base) * kEWG;
+    const uint32_t c = c0 + threadIdx.x;
+    const bool valid = c < p.n_chunks;
     const uint64_t nu = w.res[s_job].n_units;
-    const uint64_t b = chunk_begin(c), end = min(p.n, b + kEC);
-    uint32_t x = c == 0u ? p.L : w.entry[p.chunk_base + c];
-    const uint64_t pos = w.off[p.chunk_base + c];
+    const uint32_t n = valid ? (uint32_t)(min(p.n, chunk_begin(c) + kEC) - chunk_begin(c)) : 0u;
+    uint32_t x = p.L;
+    uint64_t pos = 0u;
+    if (valid) {
+        x = c == 0u ? p.L : w.entry[p.chunk_base + c];
+        pos = w.off[p.chunk_base + c];
+    }
     uint64_t u = pos / 32u;                      // forward index of the unit being filled
     uint32_t fill = (uint32_t)(pos % 32u);       // bits of that unit before this chunk
     bool shared = fill != 0u;                    // its leading bits belong to a predecessor
     uint64_t acc = 0u;
     uint32_t bad = 0u;
-    SymReader rd;
-    rd.init(p.sym, b, end);
-    const uint32_t n = (uint32_t)(end - b);
-    uint2 q = t.sym[rd.next()];
-    for (uint32_t i = 0; i < n; ++i) {
-        const uint2 cur = q;
-        if (i + 1u < n) q = t.sym[rd.next()];
+    uint4* sbuf = (uint4*)((char*)lds + stage_offset(p.L));
+    staged_symbols(p, c0, sbuf, valid, n, [&](uint32_t, uint32_t sy) {
         uint32_t code, nb;
-        x = enc_step(t, p.L, cur, x, &code, &nb, &bad);
+        x = enc_step(t, p.L, t.sym[sy], x, &code, &nb, &bad);
         acc = (acc << nb) | code;
         fill += nb;
         if (fill >= 32u) {
@@ -430,8 +488,8 @@ __global__ __launch_bounds__(kEWG) void k_enc_write(EncWork w) {
             shared = false;
             ++u;
         }
-    }
-    if (fill) {
+    });
+    if (valid && fill) {
         // the unit this chunk ends in is shared with the next chunk (or is the last one):
         // acc holds this chunk's bits of it, placed below any predecessor's bits
         const uint32_t word = (uint32_t)(acc << (32u - fill));
@@ -553,9 +611,10 @@ extern "C" int mans_encode_device(const mans_encode_job* jobs, size_t n_jobs, vo
     if (e == hipSuccess) e = hipStreamSynchronize(st);    // the host image is freed on return
     if (e != hipSuccess) return set_error(MANS_EHIP, "plan upload: %s", hipGetErrorString(e));
     const size_t lds = enc_lds_bytes(ml);
-    hipLaunchKernelGGL(k_enc_sync, dim3(tasks), dim3(kEWG), lds, st, w);
+    const size_t lds_staged = ((lds + 15u) & ~(size_t)15u) + kStageBytes;
+    hipLaunchKernelGGL(k_enc_sync, dim3(tasks), dim3(kEWG), lds_staged, st, w);
     hipLaunchKernelGGL(k_enc_fix, dim3((uint32_t)n_jobs), dim3(kEWG), lds, st, w);
-    hipLaunchKernelGGL(k_enc_write, dim3(tasks), dim3(kEWG), lds, st, w);
+    hipLaunchKernelGGL(k_enc_write, dim3(tasks), dim3(kEWG), lds_staged, st, w);
     e = hipGetLastError();
     if (e != hipSuccess) return set_error(MANS_EHIP, "encode launch: %s", hipGetErrorString(e));
     return MANS_OK;
