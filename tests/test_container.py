@@ -222,3 +222,34 @@ def test_pack_rejects_bad_input(streams):
     sp2[3] = r
     with pytest.raises(_lib.MansError):
         container.pack(es, sp2, cus)
+
+
+def test_container_edge_cases(tmp_path):
+    """No streams; one empty stream; a 16,384-state table; the image written to a file and
+    read back as bytes."""
+    from multians_amd import _lib, container
+    from multians_amd.decoder import EncodedStream
+    img = container.pack([])
+    c = container.Container(img)
+    assert (c.info.n_streams, c.info.n_tables, c.info.total_symbols) == (0, 0, 0)
+    assert c.descs(1 << 32, 0, 0)[1].tolist() == [0]
+    units, fb, fs, dec, sym = _lib.make_stream(5, 0.7, 300_000, 16384, True)
+    es = EncodedStream(units, fb, fs, dec, 16384, len(sym))
+    L = 1024
+    empty = EncodedStream(np.zeros(0, np.uint32), 0, L,
+                          (np.arange(L, dtype=np.uint32) + L) | (1 << 24), L, 0)
+    rec = _truth(units, fb, fs, dec, 16384, len(sym), 512)
+    img = container.pack([es, empty], [rec, None], [512, 0])
+    f = tmp_path / "batch.mans"
+    f.write_bytes(img.tobytes())
+    c = container.Container(f.read_bytes())
+    assert c.info.n_streams == 2 and c.info.total_symbols == len(sym)
+    assert np.array_equal(c.sync_points(0), rec)
+    assert c.sync_points(1) is None
+    ds, offs = c.descs(1 << 32, 1 << 40, len(sym), True)
+    assert ds[0].L == 16384 and ds[0].chunk_units == 512 and ds[1].n_out == 0
+    assert offs.tolist() == [0, len(sym), len(sym)]
+    with pytest.raises(_lib.MansError):                      # output arena too small
+        c.descs(1 << 32, 1 << 40, len(sym) - 1, True)
+    with pytest.raises(_lib.MansError):                      # device image not 256-aligned
+        c.descs((1 << 32) + 16, 1 << 40, len(sym), True)
