@@ -333,14 +333,15 @@ class DeviceBatch:
                                                 C.byref(stats)))
         return stats
 
-    def graph(self):
+    def graph(self, index=None):
         """Capture this batch's decode into a hipGraph (see mans_batch_graph_create);
-        returns a BatchGraph whose launch() replays it."""
+        returns a BatchGraph whose launch() replays it.  index as in decode()."""
+        descs = self.descs if index is None else self._idescs[index]
         o = _lib.BatchOpts()
         o.workspace = self.ws.data_ptr()
         o.workspace_bytes = self.ws.numel()
         h = C.c_void_p()
-        _lib.check(_lib.lib().mans_batch_graph_create(self.descs, len(self.streams), C.byref(o),
+        _lib.check(_lib.lib().mans_batch_graph_create(descs, len(self.streams), C.byref(o),
                                                       C.byref(h)))
         return BatchGraph(h, self)
 

@@ -67,7 +67,6 @@ def test_single_wrong_records(how):
     """Wrong records (planted on the host, masked into range by the decoder) cost walks
     and fixup rounds, never correctness."""
     import torch
-    from multians_amd import _lib
     from multians_amd.decoder import DeviceStream
     print(f"case wrong records {how}", flush=True)
     sym, es, st = _stream(82, 0.1, 1_500_000)
@@ -91,7 +90,6 @@ def test_single_wrong_records(how):
     assert ds.status() == 0
     assert np.array_equal(out[:es.n_symbols].cpu().numpy()[::-1], sym)
     assert np.array_equal(_host(idx_out), truth)
-    del _lib
 
 
 def test_single_rejects_bad_chunk_units():
@@ -174,3 +172,36 @@ def test_range_decode(lam, S):
         got = ds.decode_range(first, last, idx, S, index_host=host)
         torch.cuda.synchronize()
         assert np.array_equal(got.cpu().numpy(), full[first:last]), (first, last)
+
+
+def test_graph_from_sync_points():
+    """A captured graph of a batch (600 streams: concurrent parts) that decodes from its
+    sync points and writes them again replays bit-exact, with the program's own eager
+    decodes between the replays."""
+    import torch
+    from multians_amd import _lib
+    from multians_amd.decoder import DeviceBatch, EncodedStream
+    print("case graph sync points", flush=True)
+    rng = np.random.default_rng(44)
+    items = []
+    for i in range(600):
+        n = int(np.exp(rng.uniform(np.log(2000), np.log(200_000))))
+        units, fb, fs, dec, sym = _lib.make_stream(7000 + i, float(rng.uniform(0.01, 2.0)), n,
+                                                   4096, True)
+        items.append((EncodedStream(units, fb, fs, dec, 4096, n), sym))
+    b = DeviceBatch([e for e, _ in items], sync_points=True)
+    b.decode(index="out")
+    torch.cuda.synchronize()
+    rec = b.index.clone()
+    g = b.graph(index="both")
+    for rep in range(3):
+        b.out.zero_()
+        g.launch()
+        torch.cuda.synchronize()
+        assert b.status() == 0
+        assert torch.equal(b.index, rec), rep
+        for i in range(0, 600, 7):
+            assert np.array_equal(b.output(i).cpu().numpy()[::-1], items[i][1]), (rep, i)
+        b.decode()                                  # eager work between the replays
+        torch.cuda.synchronize()
+    g.close()
