@@ -177,7 +177,7 @@ int emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint64_t n
 extern "C" {
 
 // Returns 0, -4 for a rejected table.  stats (optional, 8 u64): n_chunks, n_ck,
-// walk failures (round 1), fixup rounds, total re-walks, merges, 0, 0.
+// chunks listed by the consistency check, fixup rounds, total re-walks, merges, 0, 0.
 int mans_emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint64_t n_out,
                     const uint32_t* table, uint32_t L, uint32_t first_state, uint32_t first_bit,
                     uint32_t S, uint32_t K, uint64_t* stats, uint32_t* ck_entries_out,
