@@ -134,6 +134,12 @@ typedef struct mans_decode_args {
 #define MANS_N_KERNEL_EVENTS 7
 
 #define MANS_F_SYNC_ERRORS 1u   /* synchronise and report corrupt-stream errors */
+/* Trust index_in (needs chunk_units and index_in, no index_out): one decode pass writes
+ * each chunk's symbols from its record, with no proof of the records (wrong records give
+ * wrong output; writes stay inside out[0..n_out)).  For records this library wrote for
+ * the same stream and chunk length and that the caller has kept intact, or a slice of
+ * them from mans_stream_range (symbols_before counts from index_in[0]). */
+#define MANS_F_TRUST_SYNC_POINTS 2u
 
 /* Upper bound of the workspace any tuning choice needs for n_units. */
 int mans_workspace_bytes(uint64_t n_units, uint32_t L, uint64_t* bytes);
@@ -167,6 +173,9 @@ typedef struct mans_stream_desc {
 /* mans_batch_opts.flags: decode as one pipeline even when the batch is large enough to
  * be split into concurrent parts (a caller that keeps several batches in flight itself). */
 #define MANS_BATCH_ONE_PIPELINE 1u
+/* mans_batch_opts.flags: every stream decodes from its trusted index_in, as with
+ * MANS_F_TRUST_SYNC_POINTS (also honoured by mans_batch_graph_create). */
+#define MANS_BATCH_TRUST_SYNC_POINTS 2u
 
 typedef struct mans_batch_opts {
     void* workspace;            /* device, >= mans_batch_workspace_bytes(...) */

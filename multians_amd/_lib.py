@@ -18,6 +18,7 @@ MANS_ENOMEM = -3
 MANS_ETABLE = -4
 MANS_ECORRUPT = -5
 MANS_F_SYNC_ERRORS = 1
+MANS_F_TRUST_SYNC_POINTS = 2
 
 
 class MansError(RuntimeError):
@@ -64,6 +65,7 @@ class BatchOpts(C.Structure):
 
 
 MANS_BATCH_ONE_PIPELINE = 1
+MANS_BATCH_TRUST_SYNC_POINTS = 2
 
 
 class TableJob(C.Structure):

@@ -187,11 +187,12 @@ class DeviceContainer:
         """decoder.plan_windows over this container's streams (stored sync points)."""
         return plan_windows(self.descs[True], self.c.sync_points, windows)
 
-    def decode_windows(self, windows, stream=None, one_pipeline=False, planned=None):
+    def decode_windows(self, windows, stream=None, one_pipeline=False, planned=None,
+                       trust=False):
         """Decode windows [(stream, first, last), ...] (decode order) in one batch call,
         each from the stored sync point of the chunk holding its first symbol.  Returns
         (arena, [(offset, length)]) with window k at arena[offset:offset + length]."""
         if self._win is None:
             self._win = WindowDecoder(self.device)
         planned = planned if planned is not None else self.plan_windows(windows)
-        return self._win.decode(planned, windows, stream, one_pipeline)
+        return self._win.decode(planned, windows, stream, one_pipeline, trust)

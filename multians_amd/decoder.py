@@ -175,8 +175,9 @@ class WindowDecoder:
         self.arena = None
         self.ws = None
 
-    def decode(self, planned, windows, stream=None, one_pipeline=False):
-        """Returns (arena, [(offset, length)]): window k at arena[offset:offset+length]."""
+    def decode(self, planned, windows, stream=None, one_pipeline=False, trust=False):
+        """Returns (arena, [(offset, length)]): window k at arena[offset:offset+length].
+        trust: decode from the records without proving them (MANS_BATCH_TRUST_SYNC_POINTS)."""
         torch = _torch()
         subs, skips, ooff = planned
         n = len(windows)
@@ -193,7 +194,8 @@ class WindowDecoder:
         o = _lib.BatchOpts()
         o.workspace, o.workspace_bytes = self.ws.data_ptr(), self.ws.numel()
         o.stream = stream if stream is not None else torch.cuda.current_stream().cuda_stream
-        o.flags = _lib.MANS_BATCH_ONE_PIPELINE if one_pipeline else 0
+        o.flags = (_lib.MANS_BATCH_ONE_PIPELINE if one_pipeline else 0) | \
+            (_lib.MANS_BATCH_TRUST_SYNC_POINTS if trust else 0)
         stats = _lib.BatchStats()
         _lib.check(_lib.lib().mans_decode_batch(subs, n, C.byref(o), C.byref(stats)))
         spans = [(int(ooff[k] + skips[k]), int(w[2] - w[1])) for k, w in enumerate(windows)]
@@ -315,16 +317,19 @@ class DeviceBatch:
         return plan_windows(self._idescs["in"], lambda i: index_host[ioff[i]:ioff[i + 1]],
                             windows)
 
-    def decode(self, stream=None, kernel_events=None, one_pipeline=False, index=None):
+    def decode(self, stream=None, kernel_events=None, one_pipeline=False, index=None,
+               trust=False):
         """one_pipeline: no split into concurrent parts (MANS_BATCH_ONE_PIPELINE).
         index (batches built with sync_points): "out" writes every stream's sync points,
-        "in" starts the decode from them, "both" does both."""
+        "in" starts the decode from them, "both" does both.  trust (with index="in"):
+        one pass from the records, unproven (MANS_BATCH_TRUST_SYNC_POINTS)."""
         torch = _torch()
         descs = self.descs if index is None else self._idescs[index]
         st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         o = _lib.BatchOpts()
         o.kernel_events = kernel_events
-        o.flags = _lib.MANS_BATCH_ONE_PIPELINE if one_pipeline else 0
+        o.flags = (_lib.MANS_BATCH_ONE_PIPELINE if one_pipeline else 0) | \
+            (_lib.MANS_BATCH_TRUST_SYNC_POINTS if trust else 0)
         o.workspace = self.ws.data_ptr()
         o.workspace_bytes = self.ws.numel()
         o.stream = st

@@ -205,3 +205,100 @@ def test_graph_from_sync_points():
         b.decode()                                  # eager work between the replays
         torch.cuda.synchronize()
     g.close()
+
+
+@pytest.mark.parametrize("lam,S", [(0.01, 1024), (0.3, 256), (2.0, 64), (3.0, 16)])
+def test_trusted_single(lam, S):
+    """MANS_F_TRUST_SYNC_POINTS: one pass from true records, bit-exact."""
+    import torch
+    from multians_amd import _lib
+    from multians_amd.decoder import DeviceStream
+    print(f"case trusted lam={lam} S={S}", flush=True)
+    sym, es, st = _stream(85, lam, 2_500_003)
+    ds = DeviceStream(es, chunk_units=S, checkpoint_units=16)
+    idx = torch.from_numpy(_truth(st, S).view(np.uint8).copy()).cuda()
+    out = ds.decode(chunk_units=S, index_in=idx, flags=_lib.MANS_F_TRUST_SYNC_POINTS)
+    torch.cuda.synchronize()
+    assert ds.status() == 0
+    assert np.array_equal(out[:es.n_symbols].cpu().numpy()[::-1], sym)
+
+
+def test_trusted_edge_streams():
+    """One-symbol (zero-bit) stream, tiny streams, a stream of one partial chunk."""
+    import torch
+    from multians_amd import _lib
+    from multians_amd.decoder import DeviceStream, encode_symbols
+    print("case trusted edges", flush=True)
+    cases = [np.full(100_000, 7, np.uint8), ol.gen_symbols(86, 0.5, 1),
+             ol.gen_symbols(87, 0.5, 37), ol.gen_symbols(88, 0.5, 5000)]
+    for sym in cases:
+        es = encode_symbols(sym, 1024)
+        st = ol.Stream(es.units, es.first_bit, es.first_state, es.table, 1024, len(sym))
+        ds = DeviceStream(es, chunk_units=64, checkpoint_units=16)
+        idx = torch.from_numpy(_truth(st, 64).view(np.uint8).copy()).cuda()
+        out = ds.decode(chunk_units=64, index_in=idx, flags=_lib.MANS_F_TRUST_SYNC_POINTS)
+        torch.cuda.synchronize()
+        assert ds.status() == 0
+        assert np.array_equal(out[:len(sym)].cpu().numpy()[::-1], sym), len(sym)
+
+
+def test_trusted_wrong_records_stay_in_bounds():
+    """Unproven records planted wrong (random states, bits and offsets): the decode ends,
+    and nothing is written past n_out (guard bytes after the output)."""
+    import torch
+    from multians_amd import _lib
+    from multians_amd.decoder import DeviceStream
+    print("case trusted wrong records", flush=True)
+    sym, es, st = _stream(89, 0.2, 1_000_000)
+    S = 256
+    bad = _truth(st, S)
+    rng = np.random.default_rng(6)
+    bad["state"] = rng.integers(0, 1 << 32, len(bad), dtype=np.uint64).astype(np.uint32)
+    bad["bit"] = rng.integers(0, 1 << 32, len(bad), dtype=np.uint64).astype(np.uint32)
+    bad["symbols_before"][1:] = np.sort(rng.integers(0, 3 * es.n_symbols, len(bad) - 1))
+    ds = DeviceStream(es, chunk_units=S, checkpoint_units=16)
+    n = es.n_symbols
+    ds.out = torch.full((n + 65536,), 0xAB, dtype=torch.uint8, device="cuda")
+    idx = torch.from_numpy(bad.view(np.uint8).copy()).cuda()
+    ds.decode(chunk_units=S, index_in=idx, flags=_lib.MANS_F_TRUST_SYNC_POINTS)
+    torch.cuda.synchronize()
+    assert ds.status() == 0
+    assert bool((ds.out[n:] == 0xAB).all())
+
+
+def test_trusted_batch_and_windows():
+    """A 700-stream batch (concurrent parts) from trusted records, and 2,000 windows from
+    trusted records, bit-exact."""
+    import torch
+    from multians_amd import _lib
+    from multians_amd.decoder import DeviceBatch, EncodedStream, WindowDecoder
+    print("case trusted batch", flush=True)
+    rng = np.random.default_rng(91)
+    items = []
+    for i in range(700):
+        n = int(np.exp(rng.uniform(np.log(1000), np.log(400_000))))
+        units, fb, fs, dec, sym = _lib.make_stream(9100 + i, float(rng.uniform(0.01, 2.5)), n,
+                                                   4096, True)
+        items.append((EncodedStream(units, fb, fs, dec, 4096, n), sym))
+    b = DeviceBatch([e for e, _ in items], sync_points=True)
+    b.decode(index="out")
+    torch.cuda.synchronize()
+    b.out.zero_()
+    b.decode(index="in", trust=True)
+    torch.cuda.synchronize()
+    assert b.status() == 0
+    for i, (_, sym) in enumerate(items):
+        assert np.array_equal(b.output(i).cpu().numpy()[::-1], sym), i
+    wins = []
+    for _ in range(2000):
+        i = int(rng.integers(0, 700))
+        nn = items[i][0].n_symbols
+        a = int(rng.integers(0, nn))
+        wins.append((i, a, int(min(nn, a + 1 + rng.integers(0, 50_000)))))
+    wd = WindowDecoder()
+    arena, spans = wd.decode(b.plan_windows(wins), wins, trust=True)
+    torch.cuda.synchronize()
+    assert wd.status() == 0
+    host = arena.cpu().numpy()
+    for (i, a, z), (o, ln) in zip(wins, spans):
+        assert np.array_equal(host[o:o + ln], items[i][1][::-1][a:z]), (i, a, z)
