@@ -127,6 +127,10 @@ typedef struct mans_decode_args {
      * sync points (may equal index_in). */
     const mans_sync_point* index_in;
     mans_sync_point* index_out;
+    /* Optional (device): a record per 32 units (mans_sync_point_count(n_units, 32)),
+     * written by the write pass whatever the sync-chunk length; what
+     * MANS_F_TRUST_SYNC_POINTS reads back as index_in with chunk_units 32. */
+    mans_sync_point* items_out;
 } mans_decode_args;
 
 /* kernel_events[i] is recorded before kernel i (0 init, 1 sync, 2 check, 3 fixup,
@@ -134,11 +138,11 @@ typedef struct mans_decode_args {
 #define MANS_N_KERNEL_EVENTS 7
 
 #define MANS_F_SYNC_ERRORS 1u   /* synchronise and report corrupt-stream errors */
-/* Trust index_in (needs chunk_units and index_in, no index_out): one decode pass writes
- * each chunk's symbols from its record, with no proof of the records (wrong records give
- * wrong output; writes stay inside out[0..n_out)).  For records this library wrote for
- * the same stream and chunk length and that the caller has kept intact, or a slice of
- * them from mans_stream_range (symbols_before counts from index_in[0]). */
+/* Trust index_in at chunk_units 32 (a record per 32 units, as items_out writes them; no
+ * index_out): the write pass alone decodes each 32-unit item from its record, with no
+ * proof (wrong records give wrong output; writes stay inside out[0..n_out)).  For
+ * records this library wrote for the same stream and that the caller kept intact, or a
+ * slice of them from mans_stream_range (symbols_before counts from index_in[0]). */
 #define MANS_F_TRUST_SYNC_POINTS 2u
 
 /* Upper bound of the workspace any tuning choice needs for n_units. */
@@ -168,6 +172,7 @@ typedef struct mans_stream_desc {
     uint32_t chunk_units;       /* sync-chunk length; 0 = tuned (then no index_in/out) */
     const mans_sync_point* index_in;   /* optional, as in mans_decode_args */
     mans_sync_point* index_out;        /* optional, as in mans_decode_args */
+    mans_sync_point* items_out;        /* optional, as in mans_decode_args */
 } mans_stream_desc;
 
 /* mans_batch_opts.flags: decode as one pipeline even when the batch is large enough to

@@ -35,7 +35,7 @@ class DecodeArgs(C.Structure):
                 ("flags", C.c_uint32), ("workspace", C.c_void_p),
                 ("workspace_bytes", C.c_uint64), ("stream", C.c_void_p),
                 ("kernel_events", C.c_void_p), ("index_in", C.c_void_p),
-                ("index_out", C.c_void_p)]
+                ("index_out", C.c_void_p), ("items_out", C.c_void_p)]
 
 
 N_KERNEL_EVENTS = 7
@@ -47,7 +47,7 @@ class StreamDesc(C.Structure):
                 ("n_out", C.c_uint64), ("table", C.c_void_p), ("L", C.c_uint32),
                 ("first_state", C.c_uint32), ("first_bit", C.c_uint32),
                 ("chunk_units", C.c_uint32), ("index_in", C.c_void_p),
-                ("index_out", C.c_void_p)]
+                ("index_out", C.c_void_p), ("items_out", C.c_void_p)]
 
 
 # mans_sync_point: one record per sync chunk (state = x - L, bit offset, symbols before)

@@ -275,6 +275,7 @@ struct StreamPlan {
     uint32_t pad;
     const SyncPoint* index_in;   // optional: spec of chunk c >= 1 starts at index_in[c]
     SyncPoint* index_out;        // optional: true sync points written after the scan
+    SyncPoint* items_out;        // optional: a record per write item (32 units), by k_write
 };
 
 // Workspace arrays (global indices).

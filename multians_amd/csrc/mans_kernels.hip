@@ -72,7 +72,7 @@ struct KArgs {
     uint32_t n_tasks;
     uint32_t wait_polls;             // k_sync: polls of a range's wait for its predecessor
     uint32_t any_index_out;          // some stream asks for its sync points (k_index runs)
-    uint32_t direct;                 // trusted sync points: k_direct instead of the pipeline
+    uint32_t direct;                 // trusted item records: the write pass alone
     Work w;
 };
 
@@ -834,136 +834,6 @@ __global__ __launch_bounds__(kWG) void k_index(KArgs a) {
     if (c < sp.n_chunks) sp.index_out[c] = chunk_sync_point(sp, a.w, c);
 }
 
-// Decode from trusted sync points (MANS_F_TRUST_SYNC_POINTS, grid as k_sync): lane c
-// decodes chunk c from its record and writes the symbols of the steps it owns, as many
-// as the records give (symbols_before of chunk c+1, or n_out, minus its own), at the
-// recorded offset (both counted from record 0).  Nothing is proven: a wrong record gives wrong output (the writes
-// stay inside the output and every lane stops at its chunk's end).  Units come through
-// LDS in rounds as in the sync kernels; symbols are packed four to a dword store.
-__global__ __launch_bounds__(kWG) void k_direct(KArgs a) {
-    extern __shared__ uint32_t lds[];
-    __shared__ uint32_t s_stream;
-    __shared__ uint32_t s_rbase[kWG];
-    const uint32_t t = chunk_task(a, blockIdx.x);
-    if (threadIdx.x == 0) s_stream = stream_of_task<false>(a, t);
-    __syncthreads();
-    const StreamPlan sp = *plan_of(a, __builtin_amdgcn_readfirstlane(s_stream));
-    const uint32_t sf = stage_table(sp, lds, a.w.status);
-    if (sf & 2u) return;                   // rejected table (reported by stage_table)
-    LdsTab tab{lds_addr(lds) >> 2};
-    RoundIO io{lds + table_words(sp.L), s_rbase, sp.units, sp.n_units};
-    const uint32_t c = (t - sp.task_chunk) * kWG + threadIdx.x;
-    RLane L;
-    L.active = 0u;
-    uint64_t left = 0u;
-    uint8_t* dst = nullptr;
-    if (c < sp.n_chunks) {
-        const uint32_t u0 = c * sp.S;
-        // offsets count from record 0 (a sub-stream's records are a slice of its
-        // stream's, whose symbols_before count from the stream's start)
-        const uint64_t base = sp.index_in[0].before;
-        auto rel = [&](uint64_t v) { return v > base ? v - base : 0ull; };
-        uint32_t off0 = 32u - sp.first_bit, xi = sp.first_state - sp.L;
-        uint64_t b = 0u;
-        if (c != 0u) {
-            const SyncPoint q = sp.index_in[c];
-            off0 = q.bit & 31u;
-            xi = q.state & (sp.L - 1u);
-            b = rel(q.before);
-        }
-        const uint64_t e = c + 1u < sp.n_chunks ? rel(sp.index_in[c + 1u].before) : sp.n_out;
-        b = min(b, sp.n_out);
-        const uint64_t en = min(e, sp.n_out);
-        left = en > b ? en - b : 0u;
-        dst = sp.out + b;
-        L.rbase = u0;
-        L.u_stop = min(u0 + sp.S, sp.n_units);
-        L.u = u0 + (off0 >> 5);
-        L.o = off0 & 31u;
-        L.xi = xi;
-        L.cnt = 0u;
-        L.active = left && L.u < L.u_stop ? 1u : 0u;
-    }
-    // output packing: single bytes up to 4-byte alignment, then dwords
-    uint32_t head = (uint32_t)((4u - ((uintptr_t)dst & 3u)) & 3u);
-    uint32_t acc = 0u, na = 0u;
-    auto emit = [&](uint32_t sym) {
-        if (head) {
-            *dst++ = (uint8_t)sym;
-            --head;
-        } else {
-            acc |= sym << (8u * na);
-            if (++na == 4u) {
-                *(uint32_t*)dst = acc;
-                dst += 4;
-                acc = 0u;
-                na = 0u;
-            }
-        }
-        --left;
-    };
-    io.rbase[threadIdx.x] = L.active ? L.rbase : kNone;
-    const uint32_t nr_max = (sp.S + kR - 1u) / kR;
-    __syncthreads();
-    Prefetch pf;
-    round_load(io, 0u, pf);
-    for (uint32_t r = 0; r < nr_max; ++r) {
-        __syncthreads();
-        round_store(io, pf);
-        __syncthreads();
-        if (r + 1u < nr_max) round_load(io, r + 1u, pf);
-        if (L.active) {
-            // steps starting in this round's units, at most `left` of them
-            const uint32_t rs = L.rbase + r * kR;
-            const uint32_t re = min(rs + kR, L.u_stop);
-            const uint32_t pb0 = lds_addr(io.buf + threadIdx.x * kLS) * 8u;
-            const uint32_t end = pb0 + (re - rs) * 32u;
-            uint32_t p = pb0 + (L.u - rs) * 32u + L.o;
-            uint32_t lo, hi;
-            lds_window_load(p, lo, hi);
-            uint32_t i = L.xi + tab.tb;
-            uint32_t w = lds_word(i << 2);
-            while (p < end && left) {
-                const uint32_t win = funnel_shr(hi, lo, p);
-                const uint32_t i1 = (w >> 16) + low_bits(win, w);
-                const uint32_t w1 = lds_word(i1 << 2);
-                const uint32_t p1 = p + (w & 0xFFu);
-                emit((w >> 8) & 0xFFu);
-                if (p1 >= end || !left) {      // the round (or the chunk's count) ends here
-                    i = i1;
-                    w = w1;
-                    p = p1;
-                    break;
-                }
-                i = (w1 >> 16) + __builtin_amdgcn_ubfe(win, w, w1);
-                w = lds_word(i << 2);
-                p = p1 + (w1 & 0xFFu);
-                emit((w1 >> 8) & 0xFFu);
-                lds_window_load(p, lo, hi);
-            }
-            p -= pb0;
-            L.u = rs + (p >> 5);
-            L.o = p & 31u;
-            L.xi = i - tab.tb;
-            if (!left || L.u >= L.u_stop) L.active = 0u;
-            if (!L.active) io.rbase[threadIdx.x] = kNone;
-        }
-        if (!__syncthreads_or(L.active)) break;
-    }
-    // Steps left at the chunk's end are zero-bit steps at the end of the stream (a stream
-    // whose last symbols cost no bits, or one with no coded bits at all): they read no
-    // units, so they follow the table alone (a step that would read bits reads zeros).
-    if (c < sp.n_chunks) {
-        uint32_t i = L.xi + tab.tb;
-        while (left) {
-            const uint32_t w = lds_word(i << 2);
-            emit((w >> 8) & 0xFFu);
-            i = w >> 16;
-        }
-    }
-    for (uint32_t k = 0; k < na; ++k) dst[k] = (uint8_t)(acc >> (8u * k));
-}
-
 // ---------------------------------------------------------------- write pass
 // Persistent workgroups; each takes a contiguous range of passes, a pass being 256
 // consecutive checkpoint intervals ("items", kR units each) of one stream.  A pass's
@@ -1083,6 +953,21 @@ __device__ __forceinline__ void item_write(const LdsTab& tab, const uint32_t* lb
     for (; i < cnt; ++i) dst[i] = (uint8_t)next_sym();
 }
 
+// Trusted item record `it` (MANS_F_TRUST_SYNC_POINTS: one record per write item, 32
+// units): its entry (item 0: the stream start) and its output offset, counted from
+// record 0 and kept inside [0, n_out].
+__device__ __forceinline__ uint64_t item_record(const StreamPlan& sp, uint32_t it, uint32_t* entry) {
+    const SyncPoint q = sp.index_in[it];
+    const uint64_t base = sp.index_in[0].before;
+    if (it == 0u) {
+        *entry = pack_entry(sp.first_state - sp.L, 32u - sp.first_bit);
+        return 0u;
+    }
+    *entry = pack_entry(q.state & (sp.L - 1u), q.bit & 31u);
+    const uint64_t v = q.before > base ? q.before - base : 0u;
+    return v < sp.n_out ? v : sp.n_out;
+}
+
 __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint32_t per_wg) {
     // Persistent workgroups take blocks of per_wg consecutive passes from a device
     // counter (ctick[0], zeroed by k_init), so slow passes (high symbols per unit) do
@@ -1128,12 +1013,28 @@ __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint3
         const uint32_t nitems = min((uint32_t)kWG, n_write_items(sp.n_ck) - i0);
         pass_store(buf, pf);
         uint32_t entry = 0;
-        if (threadIdx.x < nitems)
-            s_off[threadIdx.x] = checkpoint_offset(sp, a.w, (i0 + threadIdx.x) * kWF, &entry);
+        const uint32_t n_items = n_write_items(sp.n_ck);
+        if (threadIdx.x < nitems) {
+            const uint32_t it = i0 + threadIdx.x;
+            if (a.direct) {                     // trusted records, one per item
+                s_off[threadIdx.x] = item_record(sp, it, &entry);
+            } else {
+                s_off[threadIdx.x] = checkpoint_offset(sp, a.w, it * kWF, &entry);
+            }
+            if (sp.items_out) {
+                SyncPoint q;
+                q.state = entry_state(entry);
+                q.bit = entry_off(entry);
+                q.before = s_off[threadIdx.x];
+                sp.items_out[it] = q;
+            }
+        }
         if (threadIdx.x == 0) {
             uint32_t e2;
-            const uint32_t qe = (i0 + nitems) * kWF;
-            s_off[nitems] = qe < sp.n_ck ? checkpoint_offset(sp, a.w, qe, &e2) : sp.n_out;
+            const uint32_t ie = i0 + nitems;
+            s_off[nitems] = ie >= n_items ? sp.n_out
+                          : a.direct ? item_record(sp, ie, &e2)
+                                     : checkpoint_offset(sp, a.w, ie * kWF, &e2);
         }
         __syncthreads();
         const uint32_t tn = t + 1u < t_end ? t + 1u : __builtin_amdgcn_readfirstlane(s_next);
@@ -1342,9 +1243,13 @@ static int launch_pipeline(KArgs a, uint32_t n_chunk_tasks, uint32_t n_item_task
     hipLaunchKernelGGL(k_init, dim3((std::max({a.n_streams, a.n_tasks, 64u}) + 255u) / 256u),
                        dim3(256), 0, st, a, 64u);
     mark(1);
-    if (a.direct) {                         // trusted sync points: one decode pass
-        hipLaunchKernelGGL(k_direct, dim3(n_chunk_tasks), dim3(kWG), lds_s, st, a);
-        for (int i = 2; i <= 6; ++i) mark(i);
+    if (a.direct) {                         // trusted item records: the write pass alone
+        for (int i = 2; i <= 5; ++i) mark(i);
+        const uint32_t per = 4u;
+        const uint32_t nwg = std::min<uint32_t>((n_item_tasks + per - 1) / per, 2048u);
+        const size_t lds_w = lds + (size_t)kWG * kWL * 4u;
+        hipLaunchKernelGGL(k_write, dim3(nwg), dim3(kWG), lds_w, st, a, n_item_tasks, per);
+        mark(6);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return set_error(MANS_EHIP, "kernel launch: %s", hipGetErrorString(e));
         return MANS_OK;
@@ -1452,6 +1357,7 @@ extern "C" int mans_decode(const mans_decode_args* d) {
     sp.warm = choose_warm(d->n_units, d->n_out, S, K, d->index_in != nullptr);
     sp.index_in = (const SyncPoint*)d->index_in;
     sp.index_out = (SyncPoint*)d->index_out;
+    sp.items_out = (SyncPoint*)d->items_out;
     sp.n_chunks = (uint32_t)((d->n_units + S - 1) / S);
     sp.n_ck = (uint32_t)((d->n_units + K - 1) / K);
     sp.chunk_base = 0;
@@ -1462,8 +1368,12 @@ extern "C" int mans_decode(const mans_decode_args* d) {
     if (!d->workspace || d->workspace_bytes < l.bytes)
         return set_error(MANS_ENOMEM, "workspace of %llu bytes < %llu needed",
                          (unsigned long long)d->workspace_bytes, (unsigned long long)l.bytes);
-    if ((d->flags & MANS_F_TRUST_SYNC_POINTS) && (!d->index_in || d->index_out))
-        return set_error(MANS_EINVAL, "MANS_F_TRUST_SYNC_POINTS needs index_in (and no index_out)");
+    if ((d->flags & MANS_F_TRUST_SYNC_POINTS) &&
+        (!d->index_in || d->index_out || d->chunk_units != kWI))
+        return set_error(MANS_EINVAL, "MANS_F_TRUST_SYNC_POINTS needs index_in at chunk_units "
+                                      "%u (and no index_out)", kWI);
+    if (((uintptr_t)d->items_out) & 7u)
+        return set_error(MANS_EINVAL, "items_out must be 8-byte aligned");
     KArgs a;
     std::memset(&a, 0, sizeof(a));
     a.one = sp;
@@ -1533,6 +1443,9 @@ static int build_plans(const mans_stream_desc* d, size_t n, std::vector<StreamPl
         sp.warm = choose_warm(d[i].n_units, d[i].n_out, S, K, d[i].index_in != nullptr);
         sp.index_in = (const SyncPoint*)d[i].index_in;
         sp.index_out = (SyncPoint*)d[i].index_out;
+        sp.items_out = (SyncPoint*)d[i].items_out;
+        if (((uintptr_t)d[i].items_out) & 7u)
+            return set_error(MANS_EINVAL, "stream %zu: items_out must be 8-byte aligned", i);
         sp.log2K = ilog2(K);
         // streams with nothing to decode keep one idle chunk/item so tasks stay valid
         uint64_t nu = d[i].n_out ? d[i].n_units : 1;
@@ -1669,9 +1582,9 @@ static int mark_direct(std::vector<BatchPart>& parts, const mans_stream_desc* d,
                        uint32_t flags) {
     if (!(flags & MANS_BATCH_TRUST_SYNC_POINTS)) return MANS_OK;
     for (size_t i = 0; i < n; ++i)
-        if (d[i].n_out && (!d[i].index_in || d[i].index_out))
+        if (d[i].n_out && (!d[i].index_in || d[i].index_out || d[i].chunk_units != kWI))
             return set_error(MANS_EINVAL, "stream %zu: MANS_BATCH_TRUST_SYNC_POINTS needs "
-                                          "index_in (and no index_out)", i);
+                                          "index_in at chunk_units %u (and no index_out)", i, kWI);
     for (BatchPart& bp : parts) bp.direct = 1u;
     return MANS_OK;
 }

@@ -68,7 +68,7 @@ class DeviceStream:
                               dtype=torch.uint8, device=device)
 
     def args(self, stream=None, chunk_units=0, checkpoint_units=0, flags=0, n_out=None,
-             kernel_events=None, index_in=None, index_out=None):
+             kernel_events=None, index_in=None, index_out=None, items_out=None):
         torch = _torch()
         st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         a = _lib.DecodeArgs()
@@ -89,6 +89,7 @@ class DeviceStream:
         a.kernel_events = kernel_events
         a.index_in = index_in.data_ptr() if index_in is not None else None
         a.index_out = index_out.data_ptr() if index_out is not None else None
+        a.items_out = items_out.data_ptr() if items_out is not None else None
         return a
 
     def decode(self, **kw):
@@ -303,19 +304,48 @@ class DeviceBatch:
             self._idescs[mode] = ds
         self._fit_workspace([self.descs] + list(self._idescs.values()))
 
+    def use_items(self):
+        """Record buffer at 32-unit granularity (self.items, stream i at record
+        item_off[i]) and the descriptors "items_out" (a decode at the tuned chunk lengths
+        that writes them) and "trusted" (the write pass alone from them, decode(index=
+        "trusted", trust=True))."""
+        torch = _torch()
+        n = len(self.streams)
+        ioff = np.zeros(n + 1, np.int64)
+        for i, es in enumerate(self.streams):
+            ioff[i + 1] = ioff[i] + _lib.sync_point_count(es.n_units, 32)
+        self.item_off = ioff
+        self.items = None
+        self.items = torch.zeros(max(int(ioff[-1]), 1) * 16, dtype=torch.uint8, device=self.device)
+        ib = self.items.data_ptr()
+        for mode in ("items_out", "trusted"):
+            ds = (_lib.StreamDesc * n)()
+            C.memmove(ds, self.descs, C.sizeof(ds))
+            for i in range(n):
+                p = ib + 16 * int(ioff[i])
+                if mode == "items_out":
+                    ds[i].items_out = p
+                else:
+                    ds[i].chunk_units = 32
+                    ds[i].index_in = p
+            self._idescs[mode] = ds
+        self._fit_workspace([self.descs] + list(self._idescs.values()))
+
     def sync_points(self, i):
         """Host copy of stream i's sync points (SYNC_POINT_DTYPE records)."""
         a, b = int(self.ioff[i]), int(self.ioff[i + 1])
         return self.index[16 * a:16 * b].cpu().numpy().view(_lib.SYNC_POINT_DTYPE)
 
-    def plan_windows(self, windows, index_host=None):
-        """plan_windows over this batch's streams (after a decode with index="out");
-        index_host: host copy of self.index (copied here if not given)."""
+    def plan_windows(self, windows, index_host=None, items=False):
+        """plan_windows over this batch's streams (after a decode with index="out", or
+        with items: index="items_out"); index_host: host copy of the records (copied here
+        if not given)."""
+        src = self.items if items else self.index
         if index_host is None:
-            index_host = self.index.cpu().numpy().view(_lib.SYNC_POINT_DTYPE)
-        ioff = self.ioff
-        return plan_windows(self._idescs["in"], lambda i: index_host[ioff[i]:ioff[i + 1]],
-                            windows)
+            index_host = src.cpu().numpy().view(_lib.SYNC_POINT_DTYPE)
+        ioff = self.item_off if items else self.ioff
+        descs = self._idescs["trusted" if items else "in"]
+        return plan_windows(descs, lambda i: index_host[ioff[i]:ioff[i + 1]], windows)
 
     def decode(self, stream=None, kernel_events=None, one_pipeline=False, index=None,
                trust=False):

@@ -208,23 +208,31 @@ def test_graph_from_sync_points():
 
 
 @pytest.mark.parametrize("lam,S", [(0.01, 1024), (0.3, 256), (2.0, 64), (3.0, 16)])
-def test_trusted_single(lam, S):
-    """MANS_F_TRUST_SYNC_POINTS: one pass from true records, bit-exact."""
+def test_items_out_and_trusted(lam, S):
+    """A decode at chunk length S writes a record per 32 units (items_out): they equal
+    the oracle's at subsequence length 32.  The write pass alone from them
+    (MANS_F_TRUST_SYNC_POINTS) is bit-exact."""
     import torch
     from multians_amd import _lib
     from multians_amd.decoder import DeviceStream
-    print(f"case trusted lam={lam} S={S}", flush=True)
+    print(f"case items lam={lam} S={S}", flush=True)
     sym, es, st = _stream(85, lam, 2_500_003)
     ds = DeviceStream(es, chunk_units=S, checkpoint_units=16)
-    idx = torch.from_numpy(_truth(st, S).view(np.uint8).copy()).cuda()
-    out = ds.decode(chunk_units=S, index_in=idx, flags=_lib.MANS_F_TRUST_SYNC_POINTS)
+    items = ds.new_index(32)
+    ds.decode(chunk_units=S, items_out=items)
     torch.cuda.synchronize()
     assert ds.status() == 0
+    assert np.array_equal(_host(items), _truth(st, 32))
+    ds.out.zero_()
+    ds2 = DeviceStream(es, chunk_units=32, checkpoint_units=16)
+    out = ds2.decode(chunk_units=32, index_in=items, flags=_lib.MANS_F_TRUST_SYNC_POINTS)
+    torch.cuda.synchronize()
+    assert ds2.status() == 0
     assert np.array_equal(out[:es.n_symbols].cpu().numpy()[::-1], sym)
 
 
 def test_trusted_edge_streams():
-    """One-symbol (zero-bit) stream, tiny streams, a stream of one partial chunk."""
+    """One-symbol (zero-bit) stream, tiny streams, a stream shorter than one item."""
     import torch
     from multians_amd import _lib
     from multians_amd.decoder import DeviceStream, encode_symbols
@@ -234,12 +242,15 @@ def test_trusted_edge_streams():
     for sym in cases:
         es = encode_symbols(sym, 1024)
         st = ol.Stream(es.units, es.first_bit, es.first_state, es.table, 1024, len(sym))
-        ds = DeviceStream(es, chunk_units=64, checkpoint_units=16)
-        idx = torch.from_numpy(_truth(st, 64).view(np.uint8).copy()).cuda()
-        out = ds.decode(chunk_units=64, index_in=idx, flags=_lib.MANS_F_TRUST_SYNC_POINTS)
+        ds = DeviceStream(es, chunk_units=32, checkpoint_units=16)
+        idx = torch.from_numpy(_truth(st, 32).view(np.uint8).copy()).cuda()
+        out = ds.decode(chunk_units=32, index_in=idx, flags=_lib.MANS_F_TRUST_SYNC_POINTS)
         torch.cuda.synchronize()
         assert ds.status() == 0
         assert np.array_equal(out[:len(sym)].cpu().numpy()[::-1], sym), len(sym)
+    with pytest.raises(_lib.MansError):                 # trusted records must be per 32 units
+        ds = DeviceStream(es, chunk_units=64, checkpoint_units=16)
+        ds.decode(chunk_units=64, index_in=idx, flags=_lib.MANS_F_TRUST_SYNC_POINTS)
 
 
 def test_trusted_wrong_records_stay_in_bounds():
@@ -250,25 +261,25 @@ def test_trusted_wrong_records_stay_in_bounds():
     from multians_amd.decoder import DeviceStream
     print("case trusted wrong records", flush=True)
     sym, es, st = _stream(89, 0.2, 1_000_000)
-    S = 256
-    bad = _truth(st, S)
+    bad = _truth(st, 32)
     rng = np.random.default_rng(6)
     bad["state"] = rng.integers(0, 1 << 32, len(bad), dtype=np.uint64).astype(np.uint32)
     bad["bit"] = rng.integers(0, 1 << 32, len(bad), dtype=np.uint64).astype(np.uint32)
     bad["symbols_before"][1:] = np.sort(rng.integers(0, 3 * es.n_symbols, len(bad) - 1))
-    ds = DeviceStream(es, chunk_units=S, checkpoint_units=16)
+    ds = DeviceStream(es, chunk_units=32, checkpoint_units=16)
     n = es.n_symbols
     ds.out = torch.full((n + 65536,), 0xAB, dtype=torch.uint8, device="cuda")
     idx = torch.from_numpy(bad.view(np.uint8).copy()).cuda()
-    ds.decode(chunk_units=S, index_in=idx, flags=_lib.MANS_F_TRUST_SYNC_POINTS)
+    ds.decode(chunk_units=32, index_in=idx, flags=_lib.MANS_F_TRUST_SYNC_POINTS)
     torch.cuda.synchronize()
     assert ds.status() == 0
     assert bool((ds.out[n:] == 0xAB).all())
 
 
 def test_trusted_batch_and_windows():
-    """A 700-stream batch (concurrent parts) from trusted records, and 2,000 windows from
-    trusted records, bit-exact."""
+    """A 700-stream batch (concurrent parts) writes its item records in a decode at the
+    tuned chunk lengths; the write pass alone from them, and 2,000 windows from them,
+    are bit-exact."""
     import torch
     from multians_amd import _lib
     from multians_amd.decoder import DeviceBatch, EncodedStream, WindowDecoder
@@ -280,11 +291,12 @@ def test_trusted_batch_and_windows():
         units, fb, fs, dec, sym = _lib.make_stream(9100 + i, float(rng.uniform(0.01, 2.5)), n,
                                                    4096, True)
         items.append((EncodedStream(units, fb, fs, dec, 4096, n), sym))
-    b = DeviceBatch([e for e, _ in items], sync_points=True)
-    b.decode(index="out")
+    b = DeviceBatch([e for e, _ in items])
+    b.use_items()
+    b.decode(index="items_out")
     torch.cuda.synchronize()
     b.out.zero_()
-    b.decode(index="in", trust=True)
+    b.decode(index="trusted", trust=True)
     torch.cuda.synchronize()
     assert b.status() == 0
     for i, (_, sym) in enumerate(items):
@@ -296,7 +308,7 @@ def test_trusted_batch_and_windows():
         a = int(rng.integers(0, nn))
         wins.append((i, a, int(min(nn, a + 1 + rng.integers(0, 50_000)))))
     wd = WindowDecoder()
-    arena, spans = wd.decode(b.plan_windows(wins), wins, trust=True)
+    arena, spans = wd.decode(b.plan_windows(wins, items=True), wins, trust=True)
     torch.cuda.synchronize()
     assert wd.status() == 0
     host = arena.cpu().numpy()
