@@ -150,11 +150,79 @@ def bench_batched(args, torch):
 
 def bench_sync_points(args, torch, b, items):
     """The batch from its recorded sync points at the tuned chunk lengths and at
-    256-unit chunks, each also as windows decoded alone (mans_stream_range)."""
+    256-unit chunks, each also as windows decoded alone (mans_stream_range), and from
+    trusted item records."""
     out = {"tuned_chunks": sync_point_leg(args, torch, b, items)}
     b.use_index(256)
     out["chunks_256"] = sync_point_leg(args, torch, b, items)
     b.use_index()
+    out["trusted_items"] = trusted_leg(args, torch, b, items)
+    return out
+
+
+def trusted_leg(args, torch, b, items):
+    """A decode at the tuned chunk lengths that also writes a record per 32 units
+    (items_out), then the write pass alone from those records (MANS_BATCH_TRUST_
+    SYNC_POINTS), and 64 Ki-symbol windows from them; all checked bit-exact."""
+    from multians_amd import _lib
+    from multians_amd.decoder import WindowDecoder
+    total_sym = sum(es.n_symbols for es, _ in items)
+    b.use_items()
+    wdec = lambda: b.decode(index="items_out")
+    wdec()
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        wdec()
+    ms_w = time_events(torch, wdec, args.steps)
+    b.out.zero_()
+    b.decode(index="trusted", trust=True)
+    torch.cuda.synchronize()
+    bad = verify_batch(b, items, torch)
+    log("trusted item records bit-exact: %s (status %d)" % (not bad, b.status()))
+    if bad or b.status():
+        raise SystemExit("trusted decode mismatch: streams %s" % bad[:10])
+    kt = _lib.KernelTimer()
+    b.decode(index="trusted", trust=True, kernel_events=kt.ptr)
+    torch.cuda.synchronize()
+    kms = kt.times_ms()
+    kt.close()
+    tdec = lambda: b.decode(index="trusted", trust=True)
+    for _ in range(args.warmup):
+        tdec()
+    ms = time_events(torch, tdec, args.steps)
+    n_rec = int(b.item_off[-1])
+    out = {"writing_records_ms": round(ms_w, 4), "ms": round(ms, 4),
+           "GBps": round(total_sym / ms / 1e6, 3),
+           "kernels_ms": {k: round(v, 4) for k, v in kms.items() if v > 0.0},
+           "records": n_rec, "record_bytes": 16 * n_rec}
+    log("trusted item records", json.dumps(out))
+    win, count = 65536, args.windows
+    rng = np.random.default_rng(4)
+    big = [i for i, (es, _) in enumerate(items) if es.n_symbols >= win]
+    if big:
+        wins = []
+        for _ in range(count):
+            i = int(rng.choice(big))
+            a = int(rng.integers(0, items[i][0].n_symbols - win + 1))
+            wins.append((i, a, a + win))
+        planned = b.plan_windows(wins, items=True)
+        wd = WindowDecoder()
+        arena, spans = wd.decode(planned, wins, trust=True)
+        torch.cuda.synchronize()
+        host = arena.cpu().numpy()
+        bad = [k for k, ((i, a, z), (o, ln)) in enumerate(zip(wins, spans))
+               if not np.array_equal(host[o:o + ln], items[i][1][::-1][a:z])]
+        log("trusted windows bit-exact: %s (status %d)" % (not bad, wd.status()))
+        if bad or wd.status():
+            raise SystemExit("trusted window mismatch: windows %s" % bad[:10])
+        for _ in range(args.warmup):
+            wd.decode(planned, wins, trust=True)
+        wms = time_events(torch, lambda: wd.decode(planned, wins, trust=True), args.steps)
+        out["windows"] = {"count": count, "window_symbols": win, "ms": round(wms, 4),
+                          "windows_per_s": round(count / wms * 1e3, 1),
+                          "window_GBps": round(count * win / wms / 1e6, 3)}
+        log("trusted windows", json.dumps(out["windows"]))
+    b.items = None
     return out
 
 
