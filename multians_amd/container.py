@@ -158,8 +158,10 @@ class DeviceContainer:
             self._ws[key] = ws
         return ws
 
-    def decode(self, use_sync_points=True, stream=None, one_pipeline=False):
-        """Decode every stream (mans_decode_batch); returns the batch stats."""
+    def decode(self, use_sync_points=True, stream=None, one_pipeline=False, trust=False):
+        """Decode every stream (mans_decode_batch); returns the batch stats.  trust: the
+        stored records are 32-unit item records to decode from without proof
+        (MANS_BATCH_TRUST_SYNC_POINTS; every stream must hold them)."""
         import torch
         n = self.c.info.n_streams
         ds = self.descs[use_sync_points]
@@ -167,7 +169,8 @@ class DeviceContainer:
         o = _lib.BatchOpts()
         o.workspace, o.workspace_bytes = ws.data_ptr(), ws.numel()
         o.stream = stream if stream is not None else torch.cuda.current_stream().cuda_stream
-        o.flags = _lib.MANS_BATCH_ONE_PIPELINE if one_pipeline else 0
+        o.flags = (_lib.MANS_BATCH_ONE_PIPELINE if one_pipeline else 0) | \
+            (_lib.MANS_BATCH_TRUST_SYNC_POINTS if trust else 0)
         stats = _lib.BatchStats()
         _lib.check(_lib.lib().mans_decode_batch(ds, n, C.byref(o), C.byref(stats)))
         return stats

@@ -74,3 +74,32 @@ def test_container_windows():
         for (i, a, z), (o, ln) in zip(wins, spans):
             assert ln == z - a
             assert np.array_equal(host[o:o + ln], items[i][1][::-1][a:z]), (i, a, z)
+
+
+def test_container_item_records_trusted():
+    """Item records (one per 32 units) written by a decode, stored in a container, decode
+    the container's streams by the write pass alone (MANS_BATCH_TRUST_SYNC_POINTS)."""
+    import torch
+    from multians_amd import _lib, container
+    from multians_amd.decoder import DeviceBatch, EncodedStream
+    print("case container item records", flush=True)
+    rng = np.random.default_rng(12)
+    items = []
+    for i in range(300):
+        n = int(np.exp(rng.uniform(np.log(100), np.log(500_000))))
+        units, fb, fs, dec, sym = _lib.make_stream(12000 + i, float(rng.uniform(0.01, 2.5)), n,
+                                                   4096 if i % 3 else 1024, True)
+        items.append((EncodedStream(units, fb, fs, dec, 4096 if i % 3 else 1024, n), sym))
+    b = DeviceBatch([e for e, _ in items])
+    b.use_items()
+    b.decode(index="items_out")
+    torch.cuda.synchronize()
+    host = b.items.cpu().numpy().view(_lib.SYNC_POINT_DTYPE)
+    recs = [host[b.item_off[i]:b.item_off[i + 1]].copy() for i in range(len(items))]
+    img = container.pack([e for e, _ in items], recs, [32] * len(items))
+    dc = container.DeviceContainer(container.Container(img))
+    dc.decode(trust=True)
+    torch.cuda.synchronize()
+    assert dc.status(("all", True)) == 0
+    for i, (_, sym) in enumerate(items):
+        assert np.array_equal(dc.output(i).cpu().numpy()[::-1], sym), i
