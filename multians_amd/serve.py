@@ -79,7 +79,9 @@ class StreamPool:
     aligned, padded to whole 16-byte blocks), packed tables, and optionally the
     original symbols (decode order) for checking results."""
 
-    def __init__(self, streams, symbols=None, device="cuda"):
+    def __init__(self, streams, symbols=None, device="cuda", records=False):
+        """records: keep a buffer of 32-unit item records per stream (written by the
+        stream's first decode, read by later ones; Server(use_records=True))."""
         import torch
         self.streams = list(streams)
         n = len(self.streams)
@@ -104,6 +106,14 @@ class StreamPool:
         self.L = np.array([es.L for es in self.streams], np.int64)
         self.first_state = np.array([es.first_state for es in self.streams], np.int64)
         self.first_bit = np.array([es.first_bit for es in self.streams], np.int64)
+        self.have_rec = np.zeros(n, bool)
+        self.rec = None
+        if records:
+            roff = np.zeros(n + 1, np.int64)
+            for i, es in enumerate(self.streams):
+                roff[i + 1] = roff[i] + _lib.sync_point_count(es.n_units, 32)
+            self.rec = torch.zeros(max(int(roff[-1]), 1) * 16, dtype=torch.uint8, device=device)
+            self.rptr = self.rec.data_ptr() + 16 * roff[:-1]
         self.sym = None
         if symbols is not None:
             soff = np.zeros(n + 1, np.int64)
@@ -137,6 +147,7 @@ class _Batch:
     descs: object
     t_launch: float
     check: object = None
+    fresh: list = None        # pool streams whose item records this batch writes
 
 
 @dataclass
@@ -169,60 +180,92 @@ class ServeStats:
 class Server:
     """Continuous-batching decode loop over a StreamPool."""
 
-    def __init__(self, pool, budget_bytes, max_inflight=3, max_batch=4096, device="cuda"):
+    def __init__(self, pool, budget_bytes, max_inflight=3, max_batch=4096, device="cuda",
+                 use_records=False):
+        """use_records (pool built with records=True): a stream's first decode also
+        writes its item records; later requests for it run the write pass alone from
+        them (MANS_BATCH_TRUST_SYNC_POINTS), as a second launch of their batch."""
         import torch
         self.torch = torch
         self.pool = pool
+        self.use_records = bool(use_records and pool.rec is not None)
         self.arena = DeviceArena(budget_bytes, device)
         self.max_inflight = max_inflight
         self.max_batch = max_batch
         self.streams = [torch.cuda.Stream(device=device) for _ in range(max_inflight)]
         self.lib = _lib.lib()
 
+    def _groups(self, reqs):
+        """The batch's launches: [(requests, trusted)], trusted = decode from records."""
+        if not self.use_records:
+            return [(reqs, False)]
+        have = self.pool.have_rec
+        cached = [r for r in reqs if have[r.stream]]
+        fresh = [r for r in reqs if not have[r.stream]]
+        return [(g, t) for g, t in ((fresh, False), (cached, True)) if g]
+
     def _batch_block(self, reqs):
-        """Bytes of one batch's block: outputs (256-aligned each) + workspace."""
+        """Bytes of one batch's block: outputs (256-aligned each) + workspace (the
+        largest of its launches', which run one after the other on one stream)."""
         p = self.pool
-        n = len(reqs)
-        descs = (_lib.StreamDesc * n)()
         out_bytes = 0
-        for j, r in enumerate(reqs):
-            i = r.stream
-            d = descs[j]
-            d.units = int(p.uptr[i])
-            d.n_units = int(p.n_units[i])
-            d.out = 0
-            d.n_out = int(p.n_sym[i])
-            d.table = int(p.tptr[i])
-            d.L = int(p.L[i])
-            d.first_state = int(p.first_state[i])
-            d.first_bit = int(p.first_bit[i])
+        for r in reqs:
             r.out_off = out_bytes
-            out_bytes += _up(d.n_out)
-        # workspace sizing only reads sizes and pointers, so out may be a placeholder
-        for j in range(n):
-            descs[j].out = 1 << 12
-        b = C.c_uint64()
-        _lib.check(self.lib.mans_batch_workspace_bytes(descs, n, C.byref(b)))
-        return descs, out_bytes, int(b.value)
+            out_bytes += _up(int(p.n_sym[r.stream]))
+        launches = []
+        ws = 0
+        for group, trusted in self._groups(reqs):
+            n = len(group)
+            descs = (_lib.StreamDesc * n)()
+            for j, r in enumerate(group):
+                i = r.stream
+                d = descs[j]
+                d.units = int(p.uptr[i])
+                d.n_units = int(p.n_units[i])
+                # workspace sizing only reads sizes and pointers: out is set at launch
+                d.out = 1 << 12
+                d.n_out = int(p.n_sym[i])
+                d.table = int(p.tptr[i])
+                d.L = int(p.L[i])
+                d.first_state = int(p.first_state[i])
+                d.first_bit = int(p.first_bit[i])
+                if self.use_records:
+                    if trusted:
+                        d.chunk_units = 32
+                        d.index_in = int(p.rptr[i])
+                    else:
+                        d.items_out = int(p.rptr[i])
+            b = C.c_uint64()
+            _lib.check(self.lib.mans_batch_workspace_bytes(descs, n, C.byref(b)))
+            ws = max(ws, int(b.value))
+            launches.append((group, trusted, descs))
+        return launches, out_bytes, ws
 
     def _launch(self, reqs, t_now, hs, check_every):
-        descs, out_bytes, ws_bytes = self._batch_block(reqs)
+        launches, out_bytes, ws_bytes = self._batch_block(reqs)
         blk = self.arena.alloc(out_bytes + ws_bytes)
         if blk is None:
             return None
         off, nb = blk
         base = self.arena.base + off
-        for j, r in enumerate(reqs):
-            descs[j].out = base + r.out_off
+        for r in reqs:
             r.out_off += off
-        o = _lib.BatchOpts()
-        o.workspace = base + out_bytes
-        o.workspace_bytes = ws_bytes
-        o.stream = hs.cuda_stream
-        # one pipeline per batch: the server already keeps max_inflight batches in
-        # flight on its own streams
-        o.flags = _lib.MANS_BATCH_ONE_PIPELINE
-        _lib.check(self.lib.mans_decode_batch(descs, len(reqs), C.byref(o), None))
+        fresh = []
+        for group, trusted, descs in launches:
+            for j, r in enumerate(group):
+                descs[j].out = self.arena.base + r.out_off
+            o = _lib.BatchOpts()
+            o.workspace = base + out_bytes
+            o.workspace_bytes = ws_bytes
+            o.stream = hs.cuda_stream
+            # one pipeline per batch: the server already keeps max_inflight batches in
+            # flight on its own streams
+            o.flags = _lib.MANS_BATCH_ONE_PIPELINE | \
+                (_lib.MANS_BATCH_TRUST_SYNC_POINTS if trusted else 0)
+            _lib.check(self.lib.mans_decode_batch(descs, len(group), C.byref(o), None))
+            if self.use_records and not trusted:
+                fresh.extend(r.stream for r in group)
+        descs = [d for _, _, d in launches]
         check = None
         if check_every and self.pool.sym is not None:
             torch = self.torch
@@ -236,7 +279,7 @@ class Server:
                     check = bad if check is None else check + bad
         ev = self.torch.cuda.Event()
         ev.record(hs)
-        return _Batch(reqs, blk, ev, hs, descs, t_now, check)
+        return _Batch(reqs, blk, ev, hs, descs, t_now, check, fresh)
 
     def run(self, arrivals, stream_ids, check_every=0, poll_sleep=20e-6):
         """arrivals: sorted arrival times (s); stream_ids: pool index per request.
@@ -272,6 +315,8 @@ class Server:
                         checked += sum(1 for r in b.reqs if r.rid % max(check_every, 1) == 0)
                     self.arena.free(*b.block)
                     free_streams.append(b.hip_stream)
+                    if b.fresh:                # their records are complete now
+                        self.pool.have_rec[b.fresh] = True
                 else:
                     still.append(b)
             inflight = still
@@ -283,8 +328,10 @@ class Server:
                 while pending and len(take) < self.max_batch:
                     r = pending[0]
                     n = int(self.pool.n_sym[r.stream])
-                    # outputs plus a workspace bound (about 1.5 B per unit at K = 32)
-                    est = _up(n) + 2 * int(self.pool.n_units[r.stream]) + 4096
+                    # outputs plus a workspace bound (about 1.5 B per unit at K = 32,
+                    # about 2.5 B per unit for a decode from 32-unit item records)
+                    est = _up(n) + (3 if self.use_records else 2) * \
+                        int(self.pool.n_units[r.stream]) + 4096
                     if take and need + est > room:
                         break
                     take.append(pending.popleft())

@@ -47,3 +47,36 @@ def test_served_budget_smaller_than_backlog():
     st = srv.run(np.zeros(60), np.arange(60) % pool.n, check_every=1)
     assert st.summary()["requests"] == 60 and st.mismatches == 0
     assert srv.arena.peak <= 3 << 20 and st.n_batches >= 10
+
+
+@pytest.mark.gpu
+def test_served_from_item_records():
+    """use_records: a stream's first decode writes its item records, later requests for
+    it decode from them (trusted write pass), in the same batches as first decodes.
+    Every result is checked; the records equal a fresh decode's."""
+    import torch
+    from multians_amd import _lib, serve
+    from multians_amd.decoder import DeviceStream
+    import bench
+
+    specs = serve.request_mix(30, seed=6, max_bytes=2 << 20)
+    items = bench.make_streams(specs)
+    pool = serve.StreamPool([es for es, _ in items], [sym for _, sym in items], records=True)
+    srv = serve.Server(pool, 1 << 30, max_inflight=3, use_records=True)
+    arr = serve.poisson_arrivals(4000.0, 0.5, seed=8)
+    ids = np.random.default_rng(9).integers(0, pool.n, len(arr))
+    print("serve records: %d requests over %d streams" % (len(arr), pool.n), flush=True)
+    st = srv.run(arr, ids, check_every=1)
+    assert st.checked == len(arr) and st.mismatches == 0
+    assert pool.have_rec.sum() == len(set(ids.tolist()))
+    assert srv.arena.used == 0
+    # the records a stream holds are those a decode of it alone writes
+    for i in sorted(set(ids.tolist()))[::5]:
+        es, _ = items[i]
+        ds = DeviceStream(es)
+        rec = ds.new_index(32)
+        ds.decode(items_out=rec)
+        torch.cuda.synchronize()
+        n = _lib.sync_point_count(es.n_units, 32)
+        a = int((pool.rptr[i] - pool.rec.data_ptr()) // 16)
+        assert torch.equal(pool.rec[16 * a:16 * (a + n)], rec[:16 * n])
