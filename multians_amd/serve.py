@@ -108,12 +108,9 @@ class StreamPool:
         self.first_bit = np.array([es.first_bit for es in self.streams], np.int64)
         self.have_rec = np.zeros(n, bool)
         self.rec = None
+        self.device = device
         if records:
-            roff = np.zeros(n + 1, np.int64)
-            for i, es in enumerate(self.streams):
-                roff[i + 1] = roff[i] + _lib.sync_point_count(es.n_units, 32)
-            self.rec = torch.zeros(max(int(roff[-1]), 1) * 16, dtype=torch.uint8, device=device)
-            self.rptr = self.rec.data_ptr() + 16 * roff[:-1]
+            self.attach_records()
         self.sym = None
         if symbols is not None:
             soff = np.zeros(n + 1, np.int64)
@@ -123,6 +120,16 @@ class StreamPool:
                 host[soff[i]:soff[i + 1]] = s[::-1]          # decode order
             self.sym = torch.from_numpy(host).to(device)
             self.soff = soff
+
+    def attach_records(self):
+        """Allocate the item-record buffer (empty: no stream has records yet)."""
+        import torch
+        roff = np.zeros(self.n + 1, np.int64)
+        for i, es in enumerate(self.streams):
+            roff[i + 1] = roff[i] + _lib.sync_point_count(es.n_units, 32)
+        self.rec = torch.zeros(max(int(roff[-1]), 1) * 16, dtype=torch.uint8, device=self.device)
+        self.rptr = self.rec.data_ptr() + 16 * roff[:-1]
+        self.have_rec[:] = False
 
     def expected(self, i):
         return self.sym[int(self.soff[i]):int(self.soff[i + 1])]
