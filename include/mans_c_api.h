@@ -337,6 +337,11 @@ typedef struct mans_encode_job {
     uint32_t reserved;
     uint32_t* units;            /* device, units_capacity u32 (>= mans_max_encoded_units(n)) */
     uint64_t units_capacity;
+    /* Optional (device, mans_sync_point_count(units_capacity, 32) records): the decoder's
+     * item records (one per 32 units, as mans_decode_args.items_out), written by the
+     * encoder; with them the new stream decodes by MANS_F_TRUST_SYNC_POINTS at once.
+     * Records of items that hold no step are {0, 0, n}. */
+    mans_sync_point* items_out;
 } mans_encode_job;
 
 typedef struct mans_encode_result {

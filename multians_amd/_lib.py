@@ -76,7 +76,7 @@ class TableJob(C.Structure):
 class EncodeJob(C.Structure):
     _fields_ = [("symbols", C.c_void_p), ("n", C.c_uint64), ("table", C.c_void_p),
                 ("L", C.c_uint32), ("reserved", C.c_uint32), ("units", C.c_void_p),
-                ("units_capacity", C.c_uint64)]
+                ("units_capacity", C.c_uint64), ("items_out", C.c_void_p)]
 
 
 class EncodeResult(C.Structure):
@@ -257,16 +257,20 @@ def make_stream(seed, lam, n, L, keep_symbols=True):
 
 def encode_device(jobs, stream=None):
     """Encode on the device.  jobs: list of (symbols u8 tensor, decoder table tensor, L,
-    units tensor of int32 with capacity >= max_encoded_units(n)).  Returns a list of
-    (n_units, first_bit, first_state, status); waits for the stream."""
+    units tensor of int32 with capacity >= max_encoded_units(n)[, item records tensor of
+    sync_point_count(capacity, 32) records]).  Returns a list of (n_units, first_bit,
+    first_state, status); waits for the stream."""
     import torch
     n = len(jobs)
     js = (EncodeJob * n)()
-    for k, (sym, table, L, units) in enumerate(jobs):
+    for k, job in enumerate(jobs):
+        sym, table, L, units = job[:4]
         js[k].symbols = sym.data_ptr() if sym.numel() else units.data_ptr()
         js[k].n = sym.numel()
         js[k].table, js[k].L = table.data_ptr(), L
         js[k].units, js[k].units_capacity = units.data_ptr(), units.numel()
+        if len(job) > 4 and job[4] is not None:
+            js[k].items_out = job[4].data_ptr()
     b = C.c_uint64()
     check(lib().mans_encode_workspace_bytes(js, n, C.byref(b)))
     ws = torch.empty(max(b.value, 256), dtype=torch.uint8, device="cuda")

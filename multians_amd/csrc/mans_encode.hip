@@ -48,6 +48,7 @@ struct EncPlan {
     const uint8_t* sym;
     const uint32_t* table;
     uint32_t* units;
+    mans_sync_point* items;     // optional: the decoder's 32-unit item records
     uint64_t n;
     uint64_t cap;
     uint32_t L, R;
@@ -370,6 +371,13 @@ __global__ __launch_bounds__(kEWG) void k_enc_fix(EncWork w) {
             r->first_bit = 0u;
             r->first_state = p.L;
             if (p.cap < 1u) r->status |= kEncCapacity; else p.units[0] = 0u;
+            if (p.items) {                      // item 0: state L, no bits
+                mans_sync_point q;
+                q.state = 0u;
+                q.bit = 32u;
+                q.symbols_before = 0u;
+                p.items[0] = q;
+            }
         }
         return;
     }
@@ -471,10 +479,37 @@ __global__ __launch_bounds__(kEWG) void k_enc_write(EncWork w) {
     bool shared = fill != 0u;                    // its leading bits belong to a predecessor
     uint64_t acc = 0u;
     uint32_t bad = 0u;
+    // Decoder item records: the decoder reads the forward bits backwards (its position P
+    // is forward bit 32 nu - 1 - P), so the step for symbol s starts at P = 32 nu - E_s,
+    // E_s the forward bits through symbol s, with the state after symbol s.  Item i
+    // (decoder units 32i ..) starts at the last symbol s with P_s >= 1024 i: it is
+    // recorded when the next symbol's code takes P below that boundary.
+    uint64_t dpos = 32u * nu - pos;             // P of the next step (before this chunk)
+    const uint64_t sym0 = chunk_begin(c);
     uint4* sbuf = (uint4*)((char*)lds + stage_offset(p.L));
-    staged_symbols(p, c0, sbuf, valid, n, [&](uint32_t, uint32_t sy) {
+    staged_symbols(p, c0, sbuf, valid, n, [&](uint32_t i, uint32_t sy) {
         uint32_t code, nb;
+        const uint32_t xprev = x;
         x = enc_step(t, p.L, t.sym[sy], x, &code, &nb, &bad);
+        if (p.items) {
+            const uint64_t dnew = dpos - nb;
+            const uint64_t k = dpos >> 10;
+            // boundary 1024 k in (dnew, dpos]; k = nu / 32 (the stream's end) is no item
+            if ((dpos >> 10) != (dnew >> 10) && k < (nu + 31u) / 32u) {
+                const uint64_t s = sym0 + i;        // symbols before the record: p.n - s
+                mans_sync_point r;
+                if (s == 0u) {                      // no step from here on: trailing item
+                    r.state = 0u;
+                    r.bit = 0u;
+                } else {
+                    r.state = xprev - p.L;
+                    r.bit = (uint32_t)(dpos - (k << 10));
+                }
+                r.symbols_before = p.n - s;
+                p.items[k] = r;
+            }
+            dpos = dnew;
+        }
         acc = (acc << nb) | code;
         fill += nb;
         if (fill >= 32u) {
@@ -494,6 +529,14 @@ __global__ __launch_bounds__(kEWG) void k_enc_write(EncWork w) {
         // acc holds this chunk's bits of it, placed below any predecessor's bits
         const uint32_t word = (uint32_t)(acc << (32u - fill));
         if (u < nu) atomicOr(p.units + (nu - 1u - u), word);
+    }
+    if (valid && p.items && c + 1u == p.n_chunks) {
+        // item 0: the decoder's start (the state after the last symbol, first_bit bits)
+        mans_sync_point r;
+        r.state = x - p.L;
+        r.bit = (uint32_t)dpos;                 // 32 - first_bit (32 when first_bit = 0)
+        r.symbols_before = 0u;
+        p.items[0] = r;
     }
 }
 
@@ -536,6 +579,9 @@ int enc_plans(const mans_encode_job* jobs, size_t n, std::vector<EncPlan>& plans
         p.sym = (const uint8_t*)jb.symbols;
         p.table = jb.table;
         p.units = jb.units;
+        p.items = jb.items_out;
+        if (((uintptr_t)jb.items_out) & 7u)
+            return set_error(MANS_EINVAL, "job %zu: items_out must be 8-byte aligned", i);
         p.n = jb.n;
         p.cap = jb.units_capacity;
         p.L = jb.L;

@@ -140,3 +140,43 @@ def test_encode_decode_round_trip():
     torch.cuda.synchronize()
     assert ds.status() == 0
     assert np.array_equal(out[:n].cpu().numpy()[::-1], sym)
+
+
+@pytest.mark.parametrize("lam,L,n", [(0.05, 4096, 3_000_001), (1.5, 1024, 2_000_000),
+                                     (0.3, 4096, 33 * 1024 * 4), (0.5, 1024, 1), (0.5, 1024, 0)])
+def test_encode_writes_decoder_item_records(lam, L, n):
+    """The encoder's item records (items_out) equal those a decode of the stream writes
+    (for every item that holds a step; trailing items only by symbols_before), and the
+    stream decodes bit-exact from them by the write pass alone."""
+    import torch
+    from multians_amd import _lib
+    from multians_amd.decoder import DeviceStream, EncodedStream
+    print(f"case encode items lam={lam} L={L} n={n}", flush=True)
+    sym = ol.gen_symbols(31, lam, n) if n else np.zeros(0, np.uint8)
+    dec = _host(sym if n else np.array([3], np.uint8), L)[3]
+    job = _job(sym, dec, L)
+    cap = job[3].numel()
+    items = torch.zeros(_lib.sync_point_count(cap, 32) * 16, dtype=torch.uint8, device="cuda")
+    [(nu, fb, fs, status)] = _lib.encode_device([job + (items,)])
+    assert status == 0
+    units = job[3][:nu].cpu().numpy().view(np.uint32).copy()
+    es = EncodedStream(units, fb, fs, dec, L, n)
+    nrec = _lib.sync_point_count(nu, 32)
+    enc_rec = items[:16 * nrec].cpu().numpy().view(_lib.SYNC_POINT_DTYPE)
+    if n:
+        ds = DeviceStream(es)
+        drec = ds.new_index(32)
+        ds.decode(items_out=drec)
+        torch.cuda.synchronize()
+        dec_rec = drec[:16 * nrec].cpu().numpy().view(_lib.SYNC_POINT_DTYPE)
+        assert np.array_equal(enc_rec["symbols_before"], dec_rec["symbols_before"])
+        live = enc_rec["symbols_before"] < n
+        assert np.array_equal(enc_rec[live], dec_rec[live])
+        ds2 = DeviceStream(es, chunk_units=32, checkpoint_units=16)
+        out = ds2.decode(chunk_units=32, index_in=items[:16 * nrec],
+                         flags=_lib.MANS_F_TRUST_SYNC_POINTS)
+        torch.cuda.synchronize()
+        assert ds2.status() == 0
+        assert np.array_equal(out[:n].cpu().numpy()[::-1], sym)
+    else:
+        assert (nrec, int(enc_rec["state"][0]), int(enc_rec["bit"][0])) == (1, 0, 32)
