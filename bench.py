@@ -347,7 +347,66 @@ def bench_encode(args, torch, items):
            "symbols_GBps": round(total / ms / 1e6, 3),
            "note": "one mans_encode_device call (plans copied in synchronously each call)"}
     log("encode", json.dumps(out))
-    del sym, tables, units, ws
+    if args.no_encode_records:
+        del sym, tables, units, ws
+        torch.cuda.empty_cache()
+        return out
+    # the encoder also writes the decoder's 32-unit item records; the new streams then
+    # decode by the write pass alone (MANS_BATCH_TRUST_SYNC_POINTS), checked bit-exact
+    roff = np.zeros(n + 1, np.int64)
+    for i in range(n):
+        roff[i + 1] = roff[i] + _lib.sync_point_count(int(uoff[i + 1] - uoff[i]), 32)
+    recs = torch.zeros(int(roff[-1]) * 16, dtype=torch.uint8, device="cuda")
+    for i in range(n):
+        js[i].items_out = recs.data_ptr() + 16 * int(roff[i])
+    for _ in range(max(args.warmup, 1)):
+        enc()
+    ms_r = time_events(torch, enc, args.steps)
+    _lib.check(L.mans_encode_results(C.c_void_p(ws.data_ptr()), js, n, C.c_void_p(st), res))
+    del sym
+    ooff = np.concatenate([[0], np.cumsum([es.n_symbols for es, _ in items])])
+    outb = torch.empty(max(int(ooff[-1]), 1), dtype=torch.uint8, device="cuda")
+    ds = (_lib.StreamDesc * n)()
+    for i, (es, _) in enumerate(items):
+        d = ds[i]
+        d.units, d.n_units = ub + 4 * int(uoff[i]), res[i].n_units
+        d.out, d.n_out = outb.data_ptr() + int(ooff[i]), es.n_symbols
+        d.table, d.L = tb + 4 * int(toff[i]), es.L
+        d.first_state, d.first_bit = res[i].first_state, res[i].first_bit
+        d.chunk_units, d.index_in = 32, recs.data_ptr() + 16 * int(roff[i])
+    b = C.c_uint64()
+    _lib.check(L.mans_batch_workspace_bytes(ds, n, C.byref(b)))
+    dws = torch.zeros(b.value, dtype=torch.uint8, device="cuda")
+    o = _lib.BatchOpts()
+    o.workspace, o.workspace_bytes, o.stream = dws.data_ptr(), dws.numel(), st
+    o.flags = _lib.MANS_BATCH_TRUST_SYNC_POINTS
+
+    def dec():
+        _lib.check(L.mans_decode_batch(ds, n, C.byref(o), None))
+
+    dec()
+    torch.cuda.synchronize()
+
+    def chk(i):
+        got = outb[int(ooff[i]):int(ooff[i + 1])].cpu().numpy()
+        return np.array_equal(got[::-1], items[i][1])
+
+    with ThreadPoolExecutor(THREADS) as ex:
+        ok = list(ex.map(chk, range(n)))
+    bad = [i for i, v in enumerate(ok) if not v]
+    log("decode from the encoder's item records bit-exact: %s" % (not bad))
+    if bad:
+        raise SystemExit("decode from encoder records mismatch: streams %s" % bad[:10])
+    for _ in range(args.warmup):
+        dec()
+    ms_d = time_events(torch, dec, args.steps)
+    out["with_item_records_ms"] = round(ms_r, 3)
+    out["decode_from_encoder_records_ms"] = round(ms_d, 3)
+    out["decode_from_encoder_records_GBps"] = round(total / ms_d / 1e6, 3)
+    log("encode with item records, then decode from them", json.dumps(
+        {k: out[k] for k in ("with_item_records_ms", "decode_from_encoder_records_ms",
+                             "decode_from_encoder_records_GBps")}))
+    del tables, units, ws, recs, outb, dws
     torch.cuda.empty_cache()
     return out
 
@@ -605,6 +664,9 @@ def main():
                     help="skip the sync-point legs (decode from recorded sync points, windows)")
     ap.add_argument("--no-encode", action="store_true",
                     help="skip the device-encode leg")
+    ap.add_argument("--no-encode-records", action="store_true",
+                    help="in the device-encode leg, skip the encode with item records and "
+                         "the decode from them")
     ap.add_argument("--windows", type=int, default=10000,
                     help="windows of 64 Ki symbols per call in the window leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

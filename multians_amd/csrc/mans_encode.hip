@@ -225,9 +225,13 @@ __device__ __forceinline__ void stage_load(const EncPlan& p, uint32_t c0, uint32
 
 // Runs f(i, symbol) for i = 0 .. n-1 of this lane's chunk (lanes with run == false only
 // take part in the loads and barriers).  All threads of the workgroup must call it.
-template <typename F>
+struct NoRoundHook {
+    __device__ void operator()(uint32_t, const uint4*) const {}
+};
+
+template <typename F, typename G = NoRoundHook>
 __device__ void staged_symbols(const EncPlan& p, uint32_t c0, uint4* sbuf, bool run, uint32_t n,
-                               F&& f) {
+                               F&& f, G&& after_round = G()) {
     const uint64_t left = p.n - chunk_begin(c0);
     const uint32_t rounds = (uint32_t)((min((uint64_t)kEC, left) + kER - 1u) / kER);
     uint4 pf[kEP];
@@ -249,6 +253,7 @@ __device__ void staged_symbols(const EncPlan& p, uint32_t c0, uint4* sbuf, bool 
                 if (i < n) f(i, (wd[b >> 2] >> (8u * (b & 3u))) & 0xFFu);
             }
         }
+        after_round(r, sbuf + threadIdx.x * kEP);
     }
 }
 
@@ -484,32 +489,25 @@ __global__ __launch_bounds__(kEWG) void k_enc_write(EncWork w) {
     // E_s the forward bits through symbol s, with the state after symbol s.  Item i
     // (decoder units 32i ..) starts at the last symbol s with P_s >= 1024 i: it is
     // recorded when the next symbol's code takes P below that boundary.
-    uint64_t dpos = 32u * nu - pos;             // P of the next step (before this chunk)
+    const uint64_t d0 = 32u * nu - pos;         // P of the chunk's first step
+    const uint32_t doff = 1023u - (uint32_t)(d0 & 1023u);
+    uint32_t le = 0u;                            // bits of this chunk so far
     const uint64_t sym0 = chunk_begin(c);
     uint4* sbuf = (uint4*)((char*)lds + stage_offset(p.L));
-    staged_symbols(p, c0, sbuf, valid, n, [&](uint32_t i, uint32_t sy) {
+    // a round (kER symbols of at most 14 bits) crosses at most one boundary (1024 bits),
+    // whose record is written after the round
+    static_assert(kER * 14u < 1024u, "one item boundary per round");
+    // P = d0 - le crosses a multiple of 1024 exactly when (le + doff) / 1024 changes.  The
+    // symbol loop only flags a crossing; the round is then replayed from its start state
+    // (rx, rle) out of LDS to find the crossing step (keeping that step's values in the
+    // unrolled loop took some 70 more registers and scratch).
+    uint32_t crossed = 0u, rx = x, rle = 0u;
+    staged_symbols(p, c0, sbuf, valid, n, [&](uint32_t, uint32_t sy) {
         uint32_t code, nb;
-        const uint32_t xprev = x;
         x = enc_step(t, p.L, t.sym[sy], x, &code, &nb, &bad);
-        if (p.items) {
-            const uint64_t dnew = dpos - nb;
-            const uint64_t k = dpos >> 10;
-            // boundary 1024 k in (dnew, dpos]; k = nu / 32 (the stream's end) is no item
-            if ((dpos >> 10) != (dnew >> 10) && k < (nu + 31u) / 32u) {
-                const uint64_t s = sym0 + i;        // symbols before the record: p.n - s
-                mans_sync_point r;
-                if (s == 0u) {                      // no step from here on: trailing item
-                    r.state = 0u;
-                    r.bit = 0u;
-                } else {
-                    r.state = xprev - p.L;
-                    r.bit = (uint32_t)(dpos - (k << 10));
-                }
-                r.symbols_before = p.n - s;
-                p.items[k] = r;
-            }
-            dpos = dnew;
-        }
+        const uint32_t lnew = le + nb;
+        crossed |= ((le + doff) ^ (lnew + doff)) >> 10;
+        le = lnew;
         acc = (acc << nb) | code;
         fill += nb;
         if (fill >= 32u) {
@@ -523,6 +521,42 @@ __global__ __launch_bounds__(kEWG) void k_enc_write(EncWork w) {
             shared = false;
             ++u;
         }
+    }, [&](uint32_t r, const uint4* mine) {
+        if (crossed && p.items) {
+            // replay this round to the step that crosses: item k = P / 1024 starts at the
+            // step before it (symbol s - 1, in state cx); the stream's first symbol
+            // leaves a trailing item without steps; k = nu / 32 (the stream's end) is no
+            // item
+            uint32_t xx = rx, ll = rle, dummy = 0u;
+            for (uint32_t j = 0; j < kER; ++j) {
+                const uint32_t i = r * kER + j;
+                if (i >= n) break;
+                const uint4 v = mine[j >> 4];
+                const uint32_t q = j & 15u;
+                const uint32_t wdq = q < 8u ? (q < 4u ? v.x : v.y) : (q < 12u ? v.z : v.w);
+                const uint32_t sy = (wdq >> (8u * (q & 3u))) & 0xFFu;
+                uint32_t code, nb;
+                const uint32_t xn = enc_step(t, p.L, t.sym[sy], xx, &code, &nb, &dummy);
+                if ((((ll + doff) ^ (ll + nb + doff)) >> 10) != 0u) {
+                    const uint64_t cd = d0 - ll;     // P before the crossing step
+                    const uint64_t k = cd >> 10;
+                    if (k < (nu + 31u) / 32u) {
+                        const uint64_t sg = sym0 + i;
+                        mans_sync_point rec;
+                        rec.state = sg ? xx - p.L : 0u;
+                        rec.bit = sg ? (uint32_t)(cd - (k << 10)) : 0u;
+                        rec.symbols_before = p.n - sg;
+                        p.items[k] = rec;
+                    }
+                    break;
+                }
+                xx = xn;
+                ll += nb;
+            }
+        }
+        crossed = 0u;
+        rx = x;
+        rle = le;
     });
     if (valid && fill) {
         // the unit this chunk ends in is shared with the next chunk (or is the last one):
@@ -534,7 +568,7 @@ __global__ __launch_bounds__(kEWG) void k_enc_write(EncWork w) {
         // item 0: the decoder's start (the state after the last symbol, first_bit bits)
         mans_sync_point r;
         r.state = x - p.L;
-        r.bit = (uint32_t)dpos;                 // 32 - first_bit (32 when first_bit = 0)
+        r.bit = (uint32_t)(d0 - le);            // 32 - first_bit (32 when first_bit = 0)
         r.symbols_before = 0u;
         p.items[0] = r;
     }
