@@ -229,7 +229,8 @@ struct NoRoundHook {
     __device__ void operator()(uint32_t, const uint4*) const {}
 };
 
-template <typename F, typename G = NoRoundHook>
+// With kPrev, each lane takes its predecessor lane's chunk instead (lane 0 must not run).
+template <bool kPrev = false, typename F, typename G = NoRoundHook>
 __device__ void staged_symbols(const EncPlan& p, uint32_t c0, uint4* sbuf, bool run, uint32_t n,
                                F&& f, G&& after_round = G()) {
     const uint64_t left = p.n - chunk_begin(c0);
@@ -245,7 +246,7 @@ __device__ void staged_symbols(const EncPlan& p, uint32_t c0, uint4* sbuf, bool 
         if (!run) continue;
 #pragma unroll
         for (uint32_t k = 0; k < kEP; ++k) {
-            const uint4 v = sbuf[threadIdx.x * kEP + k];
+            const uint4 v = sbuf[(threadIdx.x - (kPrev ? 1u : 0u)) * kEP + k];
             const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
             for (uint32_t b = 0; b < 16u; ++b) {
@@ -310,7 +311,7 @@ __device__ void enc_walk(const EncWork& w, const EncPlan& p, const EncTab& t, ui
 __global__ __launch_bounds__(kEWG) void k_enc_sync(EncWork w) {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_exit[kEWG];
-    __shared__ uint32_t s_job;
+    __shared__ uint32_t s_job, s_warm;
     if (threadIdx.x == 0) s_job = job_of_task(w, blockIdx.x);
     __syncthreads();
     const EncPlan p = w.plans[__builtin_amdgcn_readfirstlane(s_job)];
@@ -321,12 +322,34 @@ __global__ __launch_bounds__(kEWG) void k_enc_sync(EncWork w) {
     const bool valid = c < p.n_chunks;
     uint32_t bad = 0u;
     uint32_t x = p.L;
+    // Low-entropy tables (sum of f_s * m_s below 2L: under about two bits per symbol)
+    // shed few bits per symbol, so a state guessed at a chunk's start can take more than
+    // a chunk to meet the true path.  For them every lane but the range's first encodes
+    // its predecessor's chunk first (a warm-up, from state L, nothing stored), so that its
+    // spec starts near the true path and fewer walks fail into k_enc_fix.
+    if (threadIdx.x < 64u) {
+        uint32_t a = 0u;
+        for (uint32_t k = 0; k < 4u; ++k) {
+            const uint2 q = t.sym[threadIdx.x * 4u + k];
+            a += (q.x & 0xFFFFu) * (q.y >> 16);
+        }
+        for (uint32_t d = 32u; d; d >>= 1) a += __shfl_xor(a, d, 64);
+        if (threadIdx.x == 0) s_warm = a < 2u * p.L ? 1u : 0u;
+    }
+    __syncthreads();
     if (!st) {
         // spec from the start state, every checkpoint stored
         const uint64_t q0 = p.ck_base + (uint64_t)c * kECK;
         const uint32_t n = valid ? (uint32_t)(min(p.n, chunk_begin(c) + kEC) - chunk_begin(c)) : 0u;
         uint32_t bits = 0u;
         uint4* sbuf = (uint4*)((char*)lds + stage_offset(p.L));
+        if (s_warm) {                          // uniform; chunk c - 1 is whole for c >= 1
+            staged_symbols<true>(p, c0, sbuf, valid && threadIdx.x > 0u, kEC,
+                                 [&](uint32_t, uint32_t sy) {
+                uint32_t code, nb;
+                x = enc_step(t, p.L, t.sym[sy], x, &code, &nb, &bad);
+            });
+        }
         staged_symbols(p, c0, sbuf, valid, n, [&](uint32_t i, uint32_t sy) {
             if ((i % kEK) == 0u) {
                 w.ckx[q0 + i / kEK] = x;
