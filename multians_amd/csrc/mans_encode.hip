@@ -275,6 +275,7 @@ __device__ __forceinline__ uint32_t enc_bits(const EncWork& w, const EncPlan& p,
 // stores all five fields once, at the end: with the stores in the two exits, the device
 // compiler merged them into stores whose base pointers it selected per lane from a
 // scratch copy of the work struct, and records came out mixed up.
+template <bool kFast>
 __device__ void enc_walk(const EncWork& w, const EncPlan& p, const EncTab& t, uint32_t c,
                          uint32_t e, uint32_t* bad) {
     const uint32_t g = p.chunk_base + c;
@@ -283,6 +284,65 @@ __device__ void enc_walk(const EncWork& w, const EncPlan& p, const EncTab& t, ui
     uint32_t mq = kNoneE, wb = 0u, x = e, bits = 0u;
     if (w.ckx[q0] == e) {
         mq = 0u;
+    } else if (kFast && (((uintptr_t)p.sym) & 15u) == 0u) {
+        // (k_enc_fix, whose last rounds run few walks on a nearly idle device; the extra
+        // registers do not fit k_enc_sync.)  Symbols 16-byte aligned (chunk starts are
+        // multiples of 16 too): 64 symbols per
+        // iteration from four block slots, each reloaded in place with the block four
+        // ahead once taken, and the states of the next even and odd checkpoints read two
+        // checkpoints ahead.  Every load is unconditional straight-line code, so the
+        // compiler keeps them in flight instead of waiting for each.  Block addresses are
+        // clamped to the last block holding a symbol of the chunk; bytes past its end are
+        // loaded but never taken as symbols (at most n steps).
+        typedef uint32_t v4 __attribute__((ext_vector_type(4)));
+        typedef const __attribute__((address_space(1))) v4* gv4p;
+        const uint32_t n = (uint32_t)(end - b);
+        gv4p qb = (gv4p)(p.sym + b);
+        const gv4p last = (gv4p)(((uintptr_t)(p.sym + end) - 1u) & ~(uintptr_t)15);
+        v4 s0 = *(qb < last ? qb : last);
+        v4 s1 = *(qb + 1 < last ? qb + 1 : last);
+        v4 s2 = *(qb + 2 < last ? qb + 2 : last);
+        v4 s3 = *(qb + 3 < last ? qb + 3 : last);
+        qb += 4;
+        uint32_t ckE = 0u, ckO = w.ckx[q0 + 1u];
+        uint2 qn = t.sym[s0.x & 0xFFu];
+        uint32_t i = 0u;
+        // 16 steps from block s (nx: the next block, for the look-ahead at its end);
+        // true when the chunk's symbols are all taken
+        auto blk = [&](v4& s, const v4& nx) __attribute__((always_inline)) -> bool {
+#pragma unroll
+            for (uint32_t kk = 0; kk < 16u; ++kk) {
+                if (i >= n) return true;
+                const uint32_t k1 = kk + 1u;
+                const uint32_t w1 = k1 < 4u ? s.x : k1 < 8u ? s.y : k1 < 12u ? s.z : s.w;
+                const uint32_t nbyte = kk < 15u ? (w1 >> (8u * (k1 & 3u))) & 0xFFu : nx.x & 0xFFu;
+                const uint2 cur = qn;
+                qn = t.sym[nbyte];
+                uint32_t code, nbits;
+                x = enc_step(t, p.L, cur, x, &code, &nbits, bad);
+                bits += nbits;
+                ++i;
+            }
+            s = *(qb < last ? qb : last);           // the block four ahead
+            ++qb;
+            return false;
+        };
+        for (uint32_t m = 0;; m += 2u) {            // i = 32 m here
+            if (m && i < n && ckE == x) {
+                mq = m;
+                wb = bits;
+                break;
+            }
+            ckE = w.ckx[q0 + min(m + 2u, kECK - 1u)];
+            if (blk(s0, s1) || blk(s1, s2)) break;
+            if (i < n && ckO == x) {                // i = 32 (m + 1)
+                mq = m + 1u;
+                wb = bits;
+                break;
+            }
+            ckO = w.ckx[q0 + min(m + 3u, kECK - 1u)];
+            if (blk(s2, s3) || blk(s3, s0)) break;
+        }
     } else {
         SymReader rd;
         rd.init(p.sym, b, end);
@@ -370,7 +430,7 @@ __global__ __launch_bounds__(kEWG) void k_enc_sync(EncWork w) {
     __syncthreads();
     if (valid && !st) {
         if (c > c0) {
-            enc_walk(w, p, t, c, s_exit[threadIdx.x - 1u], &bad);
+            enc_walk<false>(w, p, t, c, s_exit[threadIdx.x - 1u], &bad);
         } else {
             // chunk 0 starts on the true path; a range's first chunk (a seam) gets an
             // entry that matches no exit, so k_enc_fix walks it
@@ -427,7 +487,7 @@ __global__ __launch_bounds__(kEWG) void k_enc_fix(EncWork w) {
             w.ff[p.chunk_base + list[i]] = 1u;      // listed this round
         }
         __syncthreads();                    // entries captured before any walk writes
-        for (uint32_t i = threadIdx.x; i < n; i += kEWG) enc_walk(w, p, t, list[i], next[i], &bad);
+        for (uint32_t i = threadIdx.x; i < n; i += kEWG) enc_walk<true>(w, p, t, list[i], next[i], &bad);
         __threadfence_block();
         __syncthreads();
         if (threadIdx.x == 0) s_n2 = 0u;
