@@ -274,7 +274,9 @@ __device__ __forceinline__ uint32_t enc_bits(const EncWork& w, const EncPlan& p,
 // checkpoint; records entry, and mq / wbits or (no meeting) wexit / wtot.  Every lane
 // stores all five fields once, at the end: with the stores in the two exits, the device
 // compiler merged them into stores whose base pointers it selected per lane from a
-// scratch copy of the work struct, and records came out mixed up.
+// scratch copy of the work struct, and records came out mixed up.  kFast (k_enc_fix,
+// whose last rounds run few walks on a nearly idle device) takes aligned symbols 64 at a
+// time with the loads kept in flight; its registers do not fit k_enc_sync.
 template <bool kFast>
 __device__ void enc_walk(const EncWork& w, const EncPlan& p, const EncTab& t, uint32_t c,
                          uint32_t e, uint32_t* bad) {
@@ -285,9 +287,7 @@ __device__ void enc_walk(const EncWork& w, const EncPlan& p, const EncTab& t, ui
     if (w.ckx[q0] == e) {
         mq = 0u;
     } else if (kFast && (((uintptr_t)p.sym) & 15u) == 0u) {
-        // (k_enc_fix, whose last rounds run few walks on a nearly idle device; the extra
-        // registers do not fit k_enc_sync.)  Symbols 16-byte aligned (chunk starts are
-        // multiples of 16 too): 64 symbols per
+        // Symbols 16-byte aligned (chunk starts are multiples of 16 too): 64 symbols per
         // iteration from four block slots, each reloaded in place with the block four
         // ahead once taken, and the states of the next even and odd checkpoints read two
         // checkpoints ahead.  Every load is unconditional straight-line code, so the
