@@ -9,8 +9,9 @@
 // into chunks that encode in parallel:
 //   k_enc_sync   per 256-chunk range: each lane encodes its chunk from the start state
 //                L (a guess for chunks >= 1), storing the state and the bits emitted at
-//                every checkpoint (kEK symbols); then each lane encodes its chunk again
-//                from its predecessor's exit until the state meets the stored one.
+//                every checkpoint (kEK symbols).
+//   k_enc_walk   per range: each lane encodes its chunk again from its predecessor's
+//                spec exit until the state meets the stored one.
 //   k_enc_fix    per stream: rounds that re-walk every chunk whose recorded start is not
 //                its predecessor's exit (range seams, rare failed walks), then the
 //                exclusive scan of bits per chunk, the stream's results, and zeroed
@@ -274,10 +275,9 @@ __device__ __forceinline__ uint32_t enc_bits(const EncWork& w, const EncPlan& p,
 // checkpoint; records entry, and mq / wbits or (no meeting) wexit / wtot.  Every lane
 // stores all five fields once, at the end: with the stores in the two exits, the device
 // compiler merged them into stores whose base pointers it selected per lane from a
-// scratch copy of the work struct, and records came out mixed up.  kFast (k_enc_fix,
-// whose last rounds run few walks on a nearly idle device) takes aligned symbols 64 at a
-// time with the loads kept in flight; its registers do not fit k_enc_sync.
-template <bool kFast>
+// scratch copy of the work struct, and records came out mixed up.  Aligned symbols are
+// taken 64 at a time with the loads kept in flight: a walk is a serial chain, and the
+// last fixup rounds run few walks on a nearly idle device.
 __device__ void enc_walk(const EncWork& w, const EncPlan& p, const EncTab& t, uint32_t c,
                          uint32_t e, uint32_t* bad) {
     const uint32_t g = p.chunk_base + c;
@@ -286,7 +286,7 @@ __device__ void enc_walk(const EncWork& w, const EncPlan& p, const EncTab& t, ui
     uint32_t mq = kNoneE, wb = 0u, x = e, bits = 0u;
     if (w.ckx[q0] == e) {
         mq = 0u;
-    } else if (kFast && (((uintptr_t)p.sym) & 15u) == 0u) {
+    } else if ((((uintptr_t)p.sym) & 15u) == 0u) {
         // Symbols 16-byte aligned (chunk starts are multiples of 16 too): 64 symbols per
         // iteration from four block slots, each reloaded in place with the block four
         // ahead once taken, and the states of the next even and odd checkpoints read two
@@ -370,7 +370,6 @@ __device__ void enc_walk(const EncWork& w, const EncPlan& p, const EncTab& t, ui
 
 __global__ __launch_bounds__(kEWG) void k_enc_sync(EncWork w) {
     extern __shared__ uint32_t lds[];
-    __shared__ uint32_t s_exit[kEWG];
     __shared__ uint32_t s_job, s_warm;
     if (threadIdx.x == 0) s_job = job_of_task(w, blockIdx.x);
     __syncthreads();
@@ -426,23 +425,35 @@ __global__ __launch_bounds__(kEWG) void k_enc_sync(EncWork w) {
             w.ff[g] = 0u;
         }
     }
-    s_exit[threadIdx.x] = x;
-    __syncthreads();
-    if (valid && !st) {
-        if (c > c0) {
-            enc_walk<false>(w, p, t, c, s_exit[threadIdx.x - 1u], &bad);
-        } else {
-            // chunk 0 starts on the true path; a range's first chunk (a seam) gets an
-            // entry that matches no exit, so k_enc_fix walks it
-            const uint32_t g = p.chunk_base + c;
-            w.entry[g] = c == 0u ? p.L : kNoneE;
-            w.mq[g] = 0u;
-            w.wbits[g] = 0u;
-            w.wexit[g] = 0u;
-            w.wtot[g] = 0u;
-        }
+    if (valid && !st && c == c0) {
+        // chunk 0 starts on the true path; a range's first chunk (a seam) gets an entry
+        // that matches no exit, so k_enc_fix walks it (k_enc_walk walks the others)
+        const uint32_t g = p.chunk_base + c;
+        w.entry[g] = c == 0u ? p.L : kNoneE;
+        w.mq[g] = 0u;
+        w.wbits[g] = 0u;
+        w.wexit[g] = 0u;
+        w.wtot[g] = 0u;
     }
     if (bad || (st && threadIdx.x == 0)) atomicOr(&w.res[s_job].status, bad ? kEncNoCode : st);
+}
+
+// Per range (grid as k_enc_sync): each chunk but the range's first encodes again from its
+// predecessor's spec exit until it meets its own spec at a checkpoint.  A kernel of its
+// own, so that the walks get the registers that keep their loads in flight (enc_walk).
+__global__ __launch_bounds__(kEWG) void k_enc_walk(EncWork w) {
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t s_job;
+    if (threadIdx.x == 0) s_job = job_of_task(w, blockIdx.x);
+    __syncthreads();
+    const EncPlan p = w.plans[__builtin_amdgcn_readfirstlane(s_job)];
+    EncTab t;
+    if (stage_enc(p, lds, t)) return;          // uniform; k_enc_sync reported it
+    const uint32_t c0 = (blockIdx.x - p.task_base) * kEWG;
+    const uint32_t c = c0 + threadIdx.x;
+    uint32_t bad = 0u;
+    if (c > c0 && c < p.n_chunks) enc_walk(w, p, t, c, w.sexit[p.chunk_base + c - 1u], &bad);
+    if (bad) atomicOr(&w.res[s_job].status, kEncNoCode);
 }
 
 // Per stream: consistency rounds, scan, results, zeroed shared units.
@@ -487,7 +498,7 @@ __global__ __launch_bounds__(kEWG) void k_enc_fix(EncWork w) {
             w.ff[p.chunk_base + list[i]] = 1u;      // listed this round
         }
         __syncthreads();                    // entries captured before any walk writes
-        for (uint32_t i = threadIdx.x; i < n; i += kEWG) enc_walk<true>(w, p, t, list[i], next[i], &bad);
+        for (uint32_t i = threadIdx.x; i < n; i += kEWG) enc_walk(w, p, t, list[i], next[i], &bad);
         __threadfence_block();
         __syncthreads();
         if (threadIdx.x == 0) s_n2 = 0u;
@@ -776,6 +787,7 @@ extern "C" int mans_encode_device(const mans_encode_job* jobs, size_t n_jobs, vo
     const size_t lds = enc_lds_bytes(ml);
     const size_t lds_staged = ((lds + 15u) & ~(size_t)15u) + kStageBytes;
     hipLaunchKernelGGL(k_enc_sync, dim3(tasks), dim3(kEWG), lds_staged, st, w);
+    hipLaunchKernelGGL(k_enc_walk, dim3(tasks), dim3(kEWG), lds, st, w);
     hipLaunchKernelGGL(k_enc_fix, dim3((uint32_t)n_jobs), dim3(kEWG), lds, st, w);
     hipLaunchKernelGGL(k_enc_write, dim3(tasks), dim3(kEWG), lds_staged, st, w);
     e = hipGetLastError();
