@@ -330,7 +330,9 @@ int mans_table_status(const void* workspace, size_t n_jobs, void* stream, uint32
  * at their exact bit offsets.  Enqueued on `stream`; the plans are copied in first
  * (synchronously).  No allocation. */
 typedef struct mans_encode_job {
-    const uint8_t* symbols;     /* device, n bytes */
+    const uint8_t* symbols;     /* device, n bytes (when 16-byte aligned, the walks may also
+                                   load the rest of the aligned 16-byte block holding the
+                                   last symbol; those bytes are never used) */
     uint64_t n;
     const uint32_t* table;      /* device, L entries (decoder table) */
     uint32_t L;                 /* power of two in [2, 16384] */
