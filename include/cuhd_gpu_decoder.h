@@ -7,9 +7,11 @@
 //   number_of_states           L (table rows)
 //   max_codeword_length        ignored (the reference ignores it too)
 //   preferred_subsequence_size, threads_per_block
-//                              accepted for compatibility; this decoder picks its own
-//                              sync-chunk and checkpoint lengths (results do not
-//                              depend on them)
+//                              must equal the ones `aux` was built with (as the reference
+//                              needs); they shape only the reference sync state that
+//                              aux's accessors return.  This decoder picks its own
+//                              sync-chunk and checkpoint lengths (results do not depend
+//                              on them).
 // Like the reference, decode() may return before the last kernel completes; the next
 // blocking copy (cpy_device_to_host) or CUHDUtil::time synchronises.  Integrity faults
 // found on the device (rejected table) are reported at the next

@@ -131,7 +131,33 @@ typedef struct mans_decode_args {
      * written by the write pass whatever the sync-chunk length; what
      * MANS_F_TRUST_SYNC_POINTS reads back as index_in with chunk_units 32. */
     mans_sync_point* items_out;
+    /* Optional: the reference decoder's own sync state, filled for a caller that inspects
+     * it (cuhd::CUHDGPUDecoderMemory's accessors); NULL = none.  Read at enqueue. */
+    const struct mans_ref_sync* ref_sync;
 } mans_decode_args;
+
+/* The arrays the reference decoder leaves in CUHDGPUDecoderMemory after a decode
+ * (src/multians_gpu_decoder_memory.cc:27-44; written at src/multians_gpu_decoder.cu:
+ * 335-353, 593-596, 635, 659-670), for subsequences of `subsequence_units` units
+ * (n_sub = ceil(n_units / subsequence_units)) and blocks of `threads` subsequences:
+ *   sync_points[4j .. 4j+3]  CUHDSubsequenceSyncPoint {unit, bit, output_size, state} of
+ *                            subsequence j: the unit (relative to the subsequence's
+ *                            first unit) and bit where its last step starts, the number
+ *                            of symbols before its first step (the exclusive scan phase 3
+ *                            leaves in output_size; low 32 bits, the reference's type), and
+ *                            the state x (in [L, 2L)) before its last step.  A step
+ *                            belongs to the subsequence holding the unit it starts in;
+ *                            zero-bit steps after the last unit belong to the last one.
+ *   output_sizes[j]          the same exclusive scan (or NULL)
+ *   synced_flags[b]          1 per block when the sync phase converged, else 0 (or NULL)
+ * All device memory; only the first n_out steps are described. */
+typedef struct mans_ref_sync {
+    uint32_t* sync_points;
+    uint32_t* output_sizes;
+    uint8_t* synced_flags;
+    uint32_t subsequence_units;
+    uint32_t threads;
+} mans_ref_sync;
 
 /* kernel_events[i] is recorded before kernel i (0 init, 1 sync, 2 check, 3 fixup,
  * 4 scan, 5 write); kernel_events[6] after the last one. */
@@ -155,6 +181,13 @@ int mans_workspace_bytes_geometry(uint64_t n_units, uint32_t chunk_units,
  * MANS_F_SYNC_ERRORS is set.  Integrity faults found on the device are reported
  * by mans_decode_status(). */
 int mans_decode(const mans_decode_args* args);
+/* Fills args->ref_sync from the workspace of the last decode enqueued with these same
+ * arguments (same stream, sizes, geometry and workspace, not reused since), enqueued on
+ * args->stream, without decoding again: for a caller that inspects the reference decoder's
+ * sync state only sometimes (cuhd::CUHDGPUDecoderMemory's accessors).  Replaces the
+ * reading of sync_info / output_sizes / sequence_synced after the reference's decode
+ * (src/multians_gpu_decoder.cu:591-596). */
+int mans_ref_sync_fill(const mans_decode_args* args);
 /* Device-side status word of the last decode that used `workspace`:
  * 0 = ok, else a MANS_E* code (synchronises `stream`). */
 int mans_decode_status(void* workspace, void* stream, int* status);
@@ -189,6 +222,11 @@ typedef struct mans_batch_opts {
     uint32_t flags;             /* MANS_BATCH_* */
     uint32_t reserved;
     void** kernel_events;       /* optional, as in mans_decode_args */
+    /* Optional (device, n u32, caller order): each stream's own status word, zeroed at the
+     * start of the decode; 0 = ok, bit 0 = its table was rejected (nothing of it is
+     * written), bit 1 = its sync rounds did not converge (mans_set_sync_round_limit).
+     * mans_decode_status reports the OR over the batch. */
+    uint32_t* stream_status;
 } mans_batch_opts;
 
 typedef struct mans_batch_stats {
@@ -245,6 +283,14 @@ int mans_set_geometry_rules(const mans_geometry_rule* rules, size_t n);
  * (its phase-2 kernel re-launches from the host instead of waiting). */
 #define MANS_WAIT_POLLS_DEFAULT (1u << 20)
 int mans_set_wait_polls(uint32_t polls);
+/* Fixup rounds per stream (decoder k_fixup, encoder k_enc_fix) before a stream that is
+ * still inconsistent is reported (decoder: mans_decode_status MANS_ECORRUPT; encoder:
+ * result status 8) instead of looping on.  0 (the default) = n_chunks + 1, a bound the
+ * rounds never reach (each round proves its leftmost listed chunk).  A smaller limit
+ * is for tests that force the report.  Applies to work enqueued afterwards.  No
+ * counterpart in the reference (its host loop re-launches phase 2 until every flag is
+ * set, src/multians_gpu_decoder.cu:617-656, with no bound). */
+int mans_set_sync_round_limit(uint32_t rounds);
 /* The chunk length the decoder picks for a stream (rules, else built-in). */
 int mans_chunk_units(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t* chunk_units);
 /* Sync points of a stream of n_units units with this chunk length (a power of two in
@@ -272,6 +318,10 @@ int mans_stream_range(const mans_stream_desc* full, const mans_sync_point* index
 /* --------------------------------------------------------- host-side codec */
 /* Symbols s = min(255, floor(X)), X ~ Exp(lambda); element i depends only on (seed, i). */
 int mans_gen_symbols(uint64_t seed, double lambda, uint8_t* out, uint64_t n, int threads);
+/* The same symbols, written on the device (out: device, n bytes), enqueued on `stream`.
+ * For large test and benchmark streams (generation, tables, encoding and decoding all on
+ * the device).  No allocation. */
+int mans_gen_symbols_device(uint64_t seed, double lambda, uint8_t* out, uint64_t n, void* stream);
 /* hist[256] = occurrences of each byte value in in[0..n) (threads > 1: parallel).
  * Replaces the counting step of ANSTableGenerator::generate_distribution_from_buffer
  * (src/ans_table_generator.cc:61-95). */
@@ -350,7 +400,8 @@ typedef struct mans_encode_result {
     uint64_t n_units;
     uint32_t first_bit;
     uint32_t first_state;
-    uint32_t status;            /* 0 ok; 1 bad table, 2 symbol without a state, 4 capacity */
+    uint32_t status;            /* 0 ok; bits: 1 bad table, 2 symbol without a state,
+                                   4 capacity, 8 fixup rounds did not converge */
     uint32_t reserved;
 } mans_encode_result;
 

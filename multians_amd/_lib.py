@@ -35,7 +35,15 @@ class DecodeArgs(C.Structure):
                 ("flags", C.c_uint32), ("workspace", C.c_void_p),
                 ("workspace_bytes", C.c_uint64), ("stream", C.c_void_p),
                 ("kernel_events", C.c_void_p), ("index_in", C.c_void_p),
-                ("index_out", C.c_void_p), ("items_out", C.c_void_p)]
+                ("index_out", C.c_void_p), ("items_out", C.c_void_p),
+                ("ref_sync", C.c_void_p)]
+
+
+class RefSync(C.Structure):
+    """mans_ref_sync: the reference decoder's sync state arrays (device pointers)."""
+    _fields_ = [("sync_points", C.c_void_p), ("output_sizes", C.c_void_p),
+                ("synced_flags", C.c_void_p), ("subsequence_units", C.c_uint32),
+                ("threads", C.c_uint32)]
 
 
 N_KERNEL_EVENTS = 7
@@ -61,7 +69,7 @@ class SyncPoint(C.Structure):
 class BatchOpts(C.Structure):
     _fields_ = [("workspace", C.c_void_p), ("workspace_bytes", C.c_uint64),
                 ("stream", C.c_void_p), ("flags", C.c_uint32), ("reserved", C.c_uint32),
-                ("kernel_events", C.c_void_p)]
+                ("kernel_events", C.c_void_p), ("stream_status", C.c_void_p)]
 
 
 MANS_BATCH_ONE_PIPELINE = 1
@@ -107,9 +115,11 @@ EXPORTS = [
     "mans_normalize_counts", "mans_build_tables", "mans_max_encoded_units", "mans_encode",
     "mans_make_stream", "mans_batch_graph_create", "mans_graph_launch", "mans_graph_destroy",
     "mans_set_geometry_rules", "mans_chunk_units", "mans_histogram", "mans_set_wait_polls",
+    "mans_set_sync_round_limit",
     "mans_table_workspace_bytes", "mans_build_tables_device", "mans_table_status",
     "mans_sync_point_count", "mans_stream_range", "mans_sync_points_check",
     "mans_encode_workspace_bytes", "mans_encode_device", "mans_encode_results",
+    "mans_gen_symbols_device", "mans_ref_sync_fill",
 ]
 
 _lib = None
@@ -173,6 +183,9 @@ def lib():
         "mans_chunk_units": ([u64, u64, u32, C.POINTER(u32)], i32),
         "mans_histogram": ([vp, u64, vp, i32], i32),
         "mans_set_wait_polls": ([u32], i32),
+        "mans_gen_symbols_device": ([u64, C.c_double, C.c_void_p, u64, C.c_void_p], i32),
+        "mans_ref_sync_fill": ([C.POINTER(DecodeArgs)], i32),
+        "mans_set_sync_round_limit": ([u32], i32),
         "mans_table_workspace_bytes": ([C.c_size_t, C.POINTER(u64)], i32),
         "mans_build_tables_device": ([C.POINTER(TableJob), C.c_size_t, vp, u64, vp], i32),
         "mans_table_status": ([vp, C.c_size_t, vp, C.POINTER(u32)], i32),
@@ -368,6 +381,20 @@ WAIT_POLLS_DEFAULT = 1 << 20
 def set_wait_polls(polls=WAIT_POLLS_DEFAULT):
     """Polls of the sync kernel's wait for the previous range's exit (see mans_c_api.h)."""
     check(lib().mans_set_wait_polls(int(polls)))
+
+
+def gen_symbols_device(seed, lam, out, stream=None):
+    """mans_gen_symbols_device into the u8 device tensor `out` (its whole length)."""
+    import torch
+    st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    check(lib().mans_gen_symbols_device(int(seed), float(lam), C.c_void_p(out.data_ptr()),
+                                        out.numel(), C.c_void_p(st)))
+
+
+def set_sync_round_limit(rounds=0):
+    """Fixup rounds per stream before non-convergence is reported (0 = the proven
+    bound; see mans_c_api.h)."""
+    check(lib().mans_set_sync_round_limit(int(rounds)))
 
 
 def chunk_units(n_units, n_out, L):

@@ -65,51 +65,97 @@ CUHDGPUOutputBuffer::CUHDGPUOutputBuffer(std::shared_ptr<CUHDOutputBuffer> outpu
 
 // ------------------------------------------------------------------ decoder memory
 
-static constexpr size_t kStatusBytes = 64 * 4;   // status words at workspace offset 0
+static_assert(sizeof(mans_decode_args) <= 256, "decode args fit the memory's copy");
 
 CUHDGPUDecoderMemory::CUHDGPUDecoderMemory(size_t num_units, size_t subsequence_size,
                                            size_t num_threads)
     : num_units_(num_units),
-      num_subseq_(subsequence_size ? SDIV(num_units, subsequence_size) : 0) {
-    (void)num_threads;
-}
+      subsequence_size_(subsequence_size),
+      num_threads_(num_threads),
+      num_subseq_(subsequence_size ? SDIV(num_units, subsequence_size) : 0),
+      num_blocks_(num_threads ? SDIV(num_subseq_, num_threads) : 0) {}
 
 CUHDGPUDecoderMemory::~CUHDGPUDecoderMemory() { free(); }
 
 void CUHDGPUDecoderMemory::allocate() {
     if (is_allocated_) return;
+    if (!subsequence_size_ || !num_threads_) {
+        std::printf("MANS error: decoder memory needs a subsequence size and a thread count\n");
+        std::exit(1);
+    }
     MANS_CHECK(mans_workspace_bytes(num_units_, 0, &bytes_));
     MANS_CHECK(mans_malloc(&workspace_, bytes_));
-    void* h = nullptr;
-    MANS_CHECK(mans_host_alloc_pinned(&h, kStatusBytes));
-    status_host_ = static_cast<std::uint8_t*>(h);
-    std::memset(status_host_, 0, kStatusBytes);
+    // the reference's arrays (mans_malloc zero-fills, like its cudaMemset)
+    void* p = nullptr;
+    MANS_CHECK(mans_malloc(&p, std::max<size_t>(num_subseq_, 1) * sizeof(CUHDSubsequenceSyncPoint)));
+    sync_info_ = static_cast<CUHDSubsequenceSyncPoint*>(p);
+    MANS_CHECK(mans_malloc(&p, std::max<size_t>(num_subseq_, 1) * sizeof(std::uint32_t)));
+    output_sizes_ = static_cast<std::uint32_t*>(p);
+    MANS_CHECK(mans_malloc(&p, std::max<size_t>(num_blocks_, 1)));
+    synced_device_ = static_cast<std::uint8_t*>(p);
+    MANS_CHECK(mans_host_alloc_pinned(&p, std::max<size_t>(num_blocks_, 1)));
+    synced_host_ = static_cast<std::uint8_t*>(p);
+    std::memset(synced_host_, 0, std::max<size_t>(num_blocks_, 1));
     is_allocated_ = true;
 }
 
+void CUHDGPUDecoderMemory::note_decode(const void* decode_args) {
+    std::memcpy(last_, decode_args, sizeof(mans_decode_args));
+    have_last_ = true;
+    filled_ = false;
+}
+
+void CUHDGPUDecoderMemory::fill() {
+    if (!is_allocated_ || !have_last_ || filled_) return;
+    mans_ref_sync r;
+    std::memset(&r, 0, sizeof(r));
+    r.sync_points = reinterpret_cast<std::uint32_t*>(sync_info_);
+    r.output_sizes = output_sizes_;
+    r.synced_flags = synced_device_;
+    r.subsequence_units = (std::uint32_t)subsequence_size_;
+    r.threads = (std::uint32_t)num_threads_;
+    mans_decode_args a;
+    std::memcpy(&a, last_, sizeof(a));
+    a.ref_sync = &r;
+    MANS_CHECK(mans_ref_sync_fill(&a));
+    filled_ = true;
+}
+
 void CUHDGPUDecoderMemory::retrieve_sync_data() {
-    MANS_CHECK(mans_memcpy_d2h(status_host_, workspace_, kStatusBytes, nullptr));
+    fill();
+    MANS_CHECK(mans_memcpy_d2h(synced_host_, synced_device_, std::max<size_t>(num_blocks_, 1),
+                               nullptr));
 }
 
 void CUHDGPUDecoderMemory::free() {
     if (!is_allocated_) return;
     MANS_CHECK(mans_free(workspace_));
-    MANS_CHECK(mans_host_free_pinned(status_host_));
+    MANS_CHECK(mans_free(sync_info_));
+    MANS_CHECK(mans_free(output_sizes_));
+    MANS_CHECK(mans_free(synced_device_));
+    MANS_CHECK(mans_host_free_pinned(synced_host_));
     workspace_ = nullptr;
-    status_host_ = nullptr;
+    sync_info_ = nullptr;
+    output_sizes_ = nullptr;
+    synced_device_ = nullptr;
+    synced_host_ = nullptr;
+    have_last_ = false;
     is_allocated_ = false;
 }
 
 CUHDSubsequenceSyncPoint* CUHDGPUDecoderMemory::get_sync_info() {
-    return static_cast<CUHDSubsequenceSyncPoint*>(workspace_);
+    fill();
+    return sync_info_;
 }
 std::uint32_t* CUHDGPUDecoderMemory::get_output_sizes() {
-    return reinterpret_cast<std::uint32_t*>(static_cast<std::uint8_t*>(workspace_) + kStatusBytes);
+    fill();
+    return output_sizes_;
 }
 std::uint8_t* CUHDGPUDecoderMemory::get_sequence_synced_device() {
-    return static_cast<std::uint8_t*>(workspace_);
+    fill();
+    return synced_device_;
 }
-std::uint8_t* CUHDGPUDecoderMemory::get_sequence_synced_host() { return status_host_; }
+std::uint8_t* CUHDGPUDecoderMemory::get_sequence_synced_host() { return synced_host_; }
 
 // ------------------------------------------------------------------ util
 

@@ -16,8 +16,16 @@ void CUHDGPUDecoder::decode(std::shared_ptr<CUHDGPUInputBuffer> input, size_t in
                             size_t max_codeword_length, size_t preferred_subsequence_size,
                             size_t threads_per_block) {
     (void)max_codeword_length;
-    (void)preferred_subsequence_size;
-    (void)threads_per_block;
+    // the subsequence size and block size only shape the reference's sync state arrays
+    // (CUHDGPUDecoderMemory's accessors), which must have been sized for them, as in the
+    // reference; this decoder picks its own sync-chunk lengths
+    if (preferred_subsequence_size != aux->subsequence_size() ||
+        threads_per_block != aux->num_threads()) {
+        std::printf("MANS error: decoder memory built for subsequence size %zu and %zu threads, "
+                    "decode called with %zu and %zu\n", aux->subsequence_size(),
+                    aux->num_threads(), preferred_subsequence_size, threads_per_block);
+        std::exit(1);
+    }
     if (input_size > aux->num_units()) {
         std::printf("MANS error: decoder memory sized for %zu units, stream has %zu\n",
                     aux->num_units(), input_size);
@@ -37,6 +45,7 @@ void CUHDGPUDecoder::decode(std::shared_ptr<CUHDGPUInputBuffer> input, size_t in
     a.workspace_bytes = aux->workspace_bytes();
     a.stream = nullptr;
     MANS_CHECK(mans_decode(&a));
+    aux->note_decode(&a);
 }
 
 int CUHDGPUDecoder::decode_status(std::shared_ptr<CUHDGPUDecoderMemory> aux) {

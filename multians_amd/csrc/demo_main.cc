@@ -1,19 +1,24 @@
 // demo_main.cc -- the reference demo's flow (src/main.cc:34-207) on the drop-in classes.
 //
-//   mans_demo <device> <size in MB> [--no-gpu] [--checksums]
+//   mans_demo <device> <size in MB> [--no-gpu] [--checksums] [--dump-sync DIR]
 //
 // For lambda = 0.1, 0.26, ..., 2.34: build the exponential distribution and test data,
 // the tables and the encoded stream with ANSTableGenerator/ANSEncoder, decode on the GPU
 // with CUHDGPUDecoder (S = 4, 128 threads, as the demo passes them), copy back, reverse
 // and compare with the input.  --checksums prints FNV-1a hashes of the counts, decoder
 // table, data and units per lambda so the host codec can be compared with the
-// reference's; --no-gpu skips the decode.
+// reference's; --no-gpu skips the decode; --dump-sync DIR writes, for the first two
+// lambdas, the stream and the decoder memory's reference sync state after the decode
+// (get_sync_info, get_output_sizes, retrieve_sync_data + get_sequence_synced_host) to
+// DIR/sync_<k>.bin, for a test to check against the oracle.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <iomanip>
 #include <iostream>
+#include <string>
+#include <vector>
 
 #include "multians.h"
 
@@ -30,8 +35,12 @@ static std::uint64_t fnv(const void* p, size_t n) {
     return h;
 }
 
-static int run(long input_size, bool gpu, bool checksums) {
+template <typename T>
+static void put(std::FILE* f, const T* p, size_t n) { std::fwrite(p, sizeof(T), n, f); }
+
+static int run(long input_size, bool gpu, bool checksums, const char* dump) {
     int failures = 0;
+    int k = 0;
     std::cout << "lambda | compressed size (bytes) | time [gpu decode] (us)" << std::endl;
     for (float lambda = 0.1f; lambda < 2.5f; lambda += 0.16) {
         auto dist = ANSTableGenerator::generate_distribution(
@@ -89,9 +98,33 @@ static int run(long input_size, bool gpu, bool checksums) {
                 std::cout << "status error" << std::endl;
                 ++failures;
             }
+            if (dump && k < 2) {
+                const size_t nu = input_buffer->get_compressed_size();
+                const size_t nsub = SDIV(nu, SUBSEQUENCE_SIZE);
+                const size_t nblk = SDIV(nsub, THREADS_PER_BLOCK);
+                std::vector<std::uint32_t> sync(4 * nsub), sizes(nsub);
+                MANS_CHECK(mans_memcpy_d2h(sync.data(), mem->get_sync_info(), 16 * nsub, nullptr));
+                MANS_CHECK(mans_memcpy_d2h(sizes.data(), mem->get_output_sizes(), 4 * nsub, nullptr));
+                mem->retrieve_sync_data();
+                const std::string path = std::string(dump) + "/sync_" + std::to_string(k) + ".bin";
+                std::FILE* f = std::fopen(path.c_str(), "wb");
+                if (!f) { std::cout << "error: cannot write " << path << std::endl; return 1; }
+                const std::uint64_t hdr[9] = {nu, input_buffer->get_first_bit(),
+                                              input_buffer->get_first_state(), NUM_STATES,
+                                              (std::uint64_t)input_size, SUBSEQUENCE_SIZE,
+                                              THREADS_PER_BLOCK, nsub, nblk};
+                put(f, hdr, 9);
+                put(f, input_buffer->get_compressed_data(), nu);
+                put(f, reinterpret_cast<const std::uint32_t*>(decoder_table->get()), NUM_STATES);
+                put(f, sync.data(), 4 * nsub);
+                put(f, sizes.data(), nsub);
+                put(f, mem->get_sequence_synced_host(), nblk);
+                std::fclose(f);
+            }
         }
         std::cout << std::left << std::setw(7) << lambda << std::setw(12)
                   << input_buffer->get_compressed_size() * sizeof(UNIT_TYPE) << us << std::endl;
+        ++k;
     }
     return failures;
 }
@@ -105,12 +138,14 @@ int main(int argc, char** argv) {
     int dev = std::atoi(argv[1]);
     long mb = std::atol(argv[2]);
     bool gpu = true, checksums = false;
+    const char* dump = nullptr;
     for (int i = 3; i < argc; ++i) {
         if (!std::strcmp(argv[i], "--no-gpu")) gpu = false;
         if (!std::strcmp(argv[i], "--checksums")) checksums = true;
+        if (!std::strcmp(argv[i], "--dump-sync") && i + 1 < argc) dump = argv[++i];
     }
     long size = mb > 0 ? mb * 1024 * 1024 : 100000;
     if (gpu) MANS_CHECK(mans_set_device(dev));
-    int f = run(size, gpu, checksums);
+    int f = run(size, gpu, checksums, dump);
     return f ? 2 : 0;
 }

@@ -189,6 +189,64 @@ int mans_build_tables(const uint32_t* counts, uint32_t nsym, uint32_t L, uint32_
 
 uint64_t mans_max_encoded_units(uint64_t n) { return n / 2 + 2; }
 
+}  // extern "C"
+
+namespace {
+
+// Compact form of an encoder table (the full one is 3 x 256 x L words, 12 MiB at
+// L = 4096, and its rows are read at random): for a symbol s owning f_s states, a step
+// from state x emits nb = m_s - (x < thr_s) bits (the low bits of x, the code) and moves to
+// ys[base_s + (x >> nb) - f_s].  Built from the full table and checked against every one
+// of its entries; a table of another shape keeps the full-table path.
+struct CompactEnc {
+    std::vector<uint32_t> f, base, m, thr, ys;
+};
+
+bool compact_encoder(const uint32_t* enc, uint32_t L, CompactEnc& c) {
+    c.f.assign(256, 0);
+    c.base.assign(256, 0);
+    c.m.assign(256, 0);
+    c.thr.assign(256, 0);
+    c.ys.assign(L, 0);
+    uint32_t used = 0;
+    for (uint32_t s = 0; s < 256; ++s) {
+        const uint32_t* row = enc + 3ull * s * L;
+        if (!row[0]) {                       // symbol without states: its row is unset
+            for (uint32_t i = 1; i < L; ++i)
+                if (row[3ull * i]) return false;
+            continue;
+        }
+        uint32_t mx = 0, thr = 2u * L;
+        for (uint32_t i = 0; i < L; ++i) mx = std::max(mx, row[3ull * i + 2]);
+        for (uint32_t i = 0; i < L; ++i)
+            if (row[3ull * i + 2] == mx) { thr = L + i; break; }
+        // f: the number of distinct next states (k = x >> nb covers [f, 2f))
+        const uint32_t f = (thr >> mx);
+        if (f == 0 || used + f > L) return false;
+        c.f[s] = f;
+        c.m[s] = mx;
+        c.thr[s] = thr;
+        c.base[s] = used;
+        for (uint32_t i = 0; i < L; ++i) {
+            const uint32_t x = L + i;
+            const uint32_t* e = row + 3ull * i;
+            const uint32_t nb = mx - (x < thr ? 1u : 0u);
+            const uint32_t k = x >> nb;
+            if (!e[0] || e[2] != nb || e[1] != (x & ((1u << nb) - 1u)) || k < f || k >= 2u * f)
+                return false;
+            uint32_t& y = c.ys[used + k - f];
+            if (y && y != e[0]) return false;
+            y = e[0];
+        }
+        used += f;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
 int mans_encode(const uint8_t* in, uint64_t n, const uint32_t* enc, uint32_t L, uint32_t* units,
                 uint64_t* n_units, uint32_t* first_bit, uint32_t* first_state) {
     if ((!in && n) || !enc || !units || !n_units || !first_bit || !first_state)
@@ -198,16 +256,33 @@ int mans_encode(const uint8_t* in, uint64_t n, const uint32_t* enc, uint32_t L, 
     uint64_t acc = 0;
     uint32_t fill = 0, x = L;
     uint64_t nu = 0;
-    for (uint64_t i = 0; i < n; ++i) {
-        const uint32_t* e = enc + 3ull * ((uint64_t)in[i] * L + (x - L));
-        if (!e[0]) return set_error(MANS_EINVAL, "symbol %u has no code", in[i]);
-        acc = (acc << e[2]) | e[1];
-        fill += e[2];
-        x = e[0];
-        if (fill >= 32) {
-            fill -= 32;
-            units[nu++] = (uint32_t)(acc >> fill);
-            acc &= (1ull << fill) - 1;
+    CompactEnc c;
+    if (n >= 4096u && compact_encoder(enc, L, c)) {
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint32_t s = in[i];
+            if (!c.f[s]) return set_error(MANS_EINVAL, "symbol %u has no code", s);
+            const uint32_t nb = c.m[s] - (x < c.thr[s] ? 1u : 0u);
+            acc = (acc << nb) | (x & ((1u << nb) - 1u));
+            fill += nb;
+            x = c.ys[c.base[s] + (x >> nb) - c.f[s]];
+            if (fill >= 32) {
+                fill -= 32;
+                units[nu++] = (uint32_t)(acc >> fill);
+                acc &= (1ull << fill) - 1;
+            }
+        }
+    } else {
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint32_t* e = enc + 3ull * ((uint64_t)in[i] * L + (x - L));
+            if (!e[0]) return set_error(MANS_EINVAL, "symbol %u has no code", in[i]);
+            acc = (acc << e[2]) | e[1];
+            fill += e[2];
+            x = e[0];
+            if (fill >= 32) {
+                fill -= 32;
+                units[nu++] = (uint32_t)(acc >> fill);
+                acc &= (1ull << fill) - 1;
+            }
         }
     }
     units[nu++] = fill ? (uint32_t)(acc << (32 - fill)) : 0u;
@@ -230,7 +305,11 @@ extern "C" int mans_make_stream(uint64_t seed, double lambda, uint64_t n, uint32
     std::vector<uint8_t> tmp;
     uint8_t* sym = symbols_out;
     if (!sym) { tmp.resize(n); sym = tmp.data(); }
-    int rc = mans_gen_symbols(seed, lambda, sym, n, 1);
+    // large streams are generated on several threads (callers that make many streams at
+    // once run one per thread, and their streams are small)
+    const unsigned hc = std::thread::hardware_concurrency();
+    const int th = n >= (1ull << 26) ? (int)std::min(8u, hc ? hc : 1u) : 1;
+    int rc = mans_gen_symbols(seed, lambda, sym, n, th);
     if (rc) return rc;
     uint64_t hist[256] = {0};
     for (uint64_t i = 0; i < n; ++i) ++hist[sym[i]];

@@ -276,6 +276,8 @@ struct StreamPlan {
     const SyncPoint* index_in;   // optional: spec of chunk c >= 1 starts at index_in[c]
     SyncPoint* index_out;        // optional: true sync points written after the scan
     SyncPoint* items_out;        // optional: a record per write item (32 units), by k_write
+    uint32_t* status_out;        // optional (batch): this stream's status word, zeroed by
+                                 // k_init; bit 0 rejected table, bit 1 not converged
 };
 
 // Workspace arrays (global indices).

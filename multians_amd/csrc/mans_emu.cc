@@ -28,7 +28,7 @@ int emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint64_t n
                const uint32_t* table, uint32_t L, uint32_t first_state, uint32_t first_bit,
                uint32_t S, uint32_t K, uint32_t warm, uint64_t* stats, uint32_t* ck_entries_out,
                uint64_t* ck_offsets_out, const SyncPoint* index_in = nullptr,
-               SyncPoint* index_out = nullptr) {
+               SyncPoint* index_out = nullptr, uint32_t round_limit = 0u) {
     if (!n_out) return 0;
     std::vector<uint32_t> tab(L);
     int bad = 0, cyc = 0;
@@ -111,12 +111,15 @@ int emu_decode(const uint32_t* units, uint64_t n_units, uint8_t* out, uint64_t n
         if (c >= 1 && went[c] != chunk_exit(sp, w, c - 1)) fl[status[1]++] = c;
     }
     st_fail = status[1];
-    // k_fixup (same phases as the kernel)
+    // k_fixup (same phases and the same round bound as the kernel: n_chunks + 1 rounds,
+    // or round_limit; a stream still inconsistent then is reported, -5)
     uint32_t* list = fl.data();
     uint32_t* next = fl2.data();
-    for (;;) {
+    const uint32_t cap = round_limit && round_limit <= nc ? round_limit : nc + 1u;
+    for (uint32_t round = 0;; ++round) {
         uint32_t n = status[1];
         if (!n) break;
+        if (round >= cap) return -5;
         ++st_rounds;
         for (uint32_t i = 0; i < n; ++i) {          // A: capture entries, mark
             uint32_t t = list[i];
@@ -193,6 +196,16 @@ int mans_emu_decode_warm(const uint32_t* units, uint64_t n_units, uint8_t* out, 
                          uint64_t* stats) {
     return emu_decode(units, n_units, out, n_out, table, L, first_state, first_bit, S, K, warm,
                       stats, nullptr, nullptr);
+}
+
+// The same with at most round_limit fixup rounds (0 = n_chunks + 1): -5 when the stream
+// is still inconsistent after them (the kernel's status bit 1).
+int mans_emu_decode_limited(const uint32_t* units, uint64_t n_units, uint8_t* out,
+                            uint64_t n_out, const uint32_t* table, uint32_t L,
+                            uint32_t first_state, uint32_t first_bit, uint32_t S, uint32_t K,
+                            uint32_t round_limit, uint64_t* stats) {
+    return emu_decode(units, n_units, out, n_out, table, L, first_state, first_bit, S, K, 0u,
+                      stats, nullptr, nullptr, nullptr, nullptr, round_limit);
 }
 
 // The same from recorded sync points (index_in, n_chunks records, or NULL) and/or

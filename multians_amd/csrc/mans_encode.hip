@@ -44,6 +44,7 @@ constexpr uint32_t kNoneE = 0xFFFFFFFFu;
 constexpr uint32_t kEncBadTable = 1u;   // a `next` field outside its symbol's range
 constexpr uint32_t kEncNoCode = 2u;     // a symbol with no state in the table
 constexpr uint32_t kEncCapacity = 4u;   // more units than units_capacity
+constexpr uint32_t kEncNoConverge = 8u; // fixup rounds stopped at their bound, inconsistent
 
 struct EncPlan {
     const uint8_t* sym;
@@ -68,6 +69,7 @@ struct EncWork {
     uint32_t *ckx, *ckb;
     // per job
     mans_encode_result* res;
+    uint32_t round_limit;       // k_enc_fix rounds per stream (0 = n_chunks + 1, the bound)
 };
 
 __device__ uint32_t job_of_task(const EncWork& w, uint32_t t) {
@@ -491,7 +493,16 @@ __global__ __launch_bounds__(kEWG) void k_enc_fix(EncWork w) {
     for (uint32_t c = 1u + threadIdx.x; c < p.n_chunks; c += kEWG)
         if (w.entry[p.chunk_base + c] != enc_exit(w, p, c - 1u)) list[atomicAdd(&s_n, 1u)] = c;
     __syncthreads();
-    while (s_n) {
+    // The leftmost listed chunk starts from a proven exit and is consistent after its
+    // round, so at most n_chunks rounds run; stop after n_chunks + 1 (or w.round_limit)
+    // and report a stream still inconsistent then.
+    const uint32_t cap = w.round_limit && w.round_limit <= p.n_chunks ? w.round_limit
+                                                                       : p.n_chunks + 1u;
+    for (uint32_t round = 0; s_n; ++round) {
+        if (round >= cap) {                 // uniform (s_n read after a barrier)
+            if (threadIdx.x == 0) r->status |= kEncNoConverge;
+            return;
+        }
         const uint32_t n = s_n;
         for (uint32_t i = threadIdx.x; i < n; i += kEWG) {
             next[i] = enc_exit(w, p, list[i] - 1u);
@@ -747,6 +758,7 @@ EncWork enc_work(void* ws, const EncLayout& l, size_t n_jobs, uint32_t tasks) {
     w.ckx = (uint32_t*)(b + l.ckx);
     w.ckb = (uint32_t*)(b + l.ckb);
     w.res = (mans_encode_result*)(b + l.res);
+    w.round_limit = 0u;
     return w;
 }
 
@@ -776,7 +788,8 @@ extern "C" int mans_encode_device(const mans_encode_job* jobs, size_t n_jobs, vo
         return set_error(MANS_ENOMEM, "workspace of %llu bytes < %llu needed",
                          (unsigned long long)workspace_bytes, (unsigned long long)l.bytes);
     hipStream_t st = (hipStream_t)stream;
-    const EncWork w = enc_work(workspace, l, n_jobs, tasks);
+    EncWork w = enc_work(workspace, l, n_jobs, tasks);
+    w.round_limit = sync_round_limit();
     // plans, and zeroed results (status words are ORed into)
     std::vector<char> img(l.chunk[0] - l.plans, 0);
     std::memcpy(img.data(), plans.data(), sizeof(EncPlan) * n_jobs);

@@ -71,9 +71,15 @@ struct KArgs {
     uint32_t n_streams;
     uint32_t n_tasks;
     uint32_t wait_polls;             // k_sync: polls of a range's wait for its predecessor
+    uint32_t round_limit;            // k_fixup: rounds per stream (0 = n_chunks + 1, the bound)
     uint32_t any_index_out;          // some stream asks for its sync points (k_index runs)
     uint32_t direct;                 // trusted item records: the write pass alone
     Work w;
+    // optional: the reference decoder's sync state (mans_ref_sync), single streams only
+    uint32_t* ref_sync;
+    uint32_t* ref_sizes;
+    uint8_t* ref_flags;
+    uint32_t ref_S, ref_nsub, ref_nblocks;
 };
 
 // Chunk task of workgroup b: batches run their tasks longest-expected first, so the
@@ -126,12 +132,16 @@ __device__ uint32_t stage_table(const StreamPlan& sp, uint32_t* lds, uint32_t* s
     }
     if (bad) {
         atomicOr(&status[0], 1u);
+        if (sp.status_out) atomicOr(sp.status_out, 1u);
         atomicOr(&s_flags, 2u);
     }
     if (cyc) atomicOr(&s_flags, 1u);
     __syncthreads();
     const uint32_t f = s_flags;
-    if ((f & 1u) && sp.n_ck > 1u && threadIdx.x == 0) atomicOr(&status[0], 1u);
+    if ((f & 1u) && sp.n_ck > 1u && threadIdx.x == 0) {
+        atomicOr(&status[0], 1u);
+        if (sp.status_out) atomicOr(sp.status_out, 1u);
+    }
     return __builtin_amdgcn_readfirstlane(f);
 }
 
@@ -142,7 +152,11 @@ __global__ void k_init(KArgs a, uint32_t n_status) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_status) w.status[i] = 0u;
     if (i < 4u) w.ctick[i] = 0u;
-    if (i < a.n_streams) w.fcnt[i] = 0u;
+    if (i < a.n_streams) {
+        w.fcnt[i] = 0u;
+        uint32_t* so = plan_of(a, i)->status_out;
+        if (so) *so = 0u;
+    }
     if (i < a.n_tasks) {
         const StreamPlan& sp = *plan_of(a, stream_of_task<false>(a, i));
         const uint32_t c0 = (i - sp.task_chunk) * kWG;
@@ -672,7 +686,10 @@ __global__ __launch_bounds__(kWG) void k_check(KArgs a) {
 // start (wentry) may differ from the current exit of their predecessor.  Each round
 // walks them again from that exit and lists any chunk (itself, or its successor)
 // still inconsistent afterwards.  The leftmost listed chunk always starts from a
-// proven exit, so the rounds end.  Streams are independent, so no grid-wide sync.
+// proven exit and is consistent after its round, so at most n_chunks rounds run; the
+// loop stops after n_chunks + 1 (or a.round_limit) rounds and reports a stream that is
+// still inconsistent then in status[0] bit 1 instead of looping on.  Streams are
+// independent, so no grid-wide sync.
 __global__ __launch_bounds__(kWG) void k_fixup(KArgs a) {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_n, s_n2;
@@ -694,11 +711,20 @@ __global__ __launch_bounds__(kWG) void k_fixup(KArgs a) {
             atomicAdd(&w.status[5], 1u);
             atomicMax(&w.status[6], s_n);
         }
+        const uint32_t cap = a.round_limit && a.round_limit <= sp.n_chunks ? a.round_limit
+                                                                             : sp.n_chunks + 1u;
         for (uint32_t round = 0;; ++round) {
             __syncthreads();
             const uint32_t n = s_n;
             if (n == 0u) {
                 if (threadIdx.x == 0) atomicMax(&w.status[4], round);
+                break;
+            }
+            if (round >= cap) {                    // not converged (uniform): report, stop
+                if (threadIdx.x == 0) {
+                    atomicOr(&w.status[0], 2u);
+                    if (sp.status_out) atomicOr(sp.status_out, 2u);
+                }
                 break;
             }
             for (uint32_t i = threadIdx.x; i < n; i += kWG) {        // capture + mark
@@ -832,6 +858,72 @@ __global__ __launch_bounds__(kWG) void k_index(KArgs a) {
     if (!sp.index_out) return;
     const uint32_t c = (t - sp.task_chunk) * kWG + threadIdx.x;
     if (c < sp.n_chunks) sp.index_out[c] = chunk_sync_point(sp, a.w, c);
+}
+
+// Derived table words read from the reference table in HBM (derive_entry per lookup).
+struct GlobalTab {
+    const uint32_t* t;
+    uint32_t L;
+    __device__ uint32_t operator[](uint32_t i) const {
+        int bad = 0, cyc = 0;
+        return derive_entry(load_unit(t, L, i), L, i, &bad, &cyc);
+    }
+};
+
+// The reference decoder's sync state (mans_ref_sync) of a single stream, from the proven
+// checkpoints: lane q decodes checkpoint interval q (its true entry, its scanned offset and
+// step count) from HBM and, for each subsequence of ref_S units, writes the count of
+// symbols before its first step (where that step lies in the interval) and the unit, bit
+// and state of its last step (where that lies in the interval).  Every unit below the
+// last holds at least one step start (codewords are at most 15 bits), so whether an
+// interval's first (last) step is its subsequence's first (last) follows from where the
+// subsequence starts (ends) relative to the interval.  Lanes below ref_nblocks also write
+// the per-block flags: 1, or 0 when the fixup rounds did not converge (status bit 1).
+// Compatibility output, not on the decode path (no LDS staging).
+__global__ __launch_bounds__(kWG) void k_refsync(KArgs a) {
+    const StreamPlan& sp = a.one;
+    const Work& w = a.w;
+    const uint32_t q = blockIdx.x * kWG + threadIdx.x;
+    if (q < a.ref_nblocks && a.ref_flags) a.ref_flags[q] = (w.status[0] & 2u) ? 0u : 1u;
+    if (q >= sp.n_ck) return;
+    uint32_t entry, e2;
+    const uint64_t off = checkpoint_offset(sp, w, q, &entry);
+    uint64_t end = q + 1u < sp.n_ck ? checkpoint_offset(sp, w, q + 1u, &e2) : sp.n_out;
+    if (end > sp.n_out) end = sp.n_out;
+    if (off >= end) return;
+    const bool last_interval = q + 1u == sp.n_ck;
+    const uint32_t S = a.ref_S, nsub = a.ref_nsub;
+    const uint64_t k0 = (uint64_t)q * sp.K, k1 = k0 + sp.K;      // interval units [k0, k1)
+    GlobalTab tab{sp.table, sp.L};
+    BCursor cur;
+    bcursor_init(cur, sp.units, sp.n_units, (uint32_t)k0, entry_off(entry));
+    uint32_t xi = entry_state(entry);
+    uint32_t j = min(cur.u / S, nsub - 1u);
+    for (uint64_t i = off; i < end; ++i) {
+        const bool first = i == off ? (uint64_t)j * S >= k0 : false;
+        if (first) {
+            a.ref_sync[4ull * j + 2u] = (uint32_t)i;
+            if (a.ref_sizes) a.ref_sizes[j] = (uint32_t)i;
+        }
+        const uint32_t unit_rel = cur.u - j * S, bit = cur.o, x = xi + sp.L;
+        uint32_t crossed;
+        bstep(tab, cur, xi, crossed);
+        const uint32_t jn = min(cur.u / S, nsub - 1u);
+        bool last;
+        if (i + 1u == sp.n_out) last = true;
+        else if (i + 1u == end) last = last_interval || (uint64_t)(j + 1u) * S <= k1;
+        else last = jn != j;
+        if (last) {
+            a.ref_sync[4ull * j + 0u] = unit_rel;
+            a.ref_sync[4ull * j + 1u] = bit;
+            a.ref_sync[4ull * j + 3u] = x;
+        }
+        if (jn != j && i + 1u < end) {          // the next step opens subsequence jn
+            a.ref_sync[4ull * jn + 2u] = (uint32_t)(i + 1u);
+            if (a.ref_sizes) a.ref_sizes[jn] = (uint32_t)(i + 1u);
+        }
+        j = jn;
+    }
 }
 
 // ---------------------------------------------------------------- write pass
@@ -1069,6 +1161,7 @@ static uint32_t ilog2(uint32_t v) { uint32_t r = 0; while ((1u << (r + 1)) <= v)
 // path (tens to hundreds of units at L = 4096), so the spec and walk passes stay
 // close to one decode of the stream.
 static std::atomic<uint32_t> g_wait_polls{MANS_WAIT_POLLS_DEFAULT};
+static std::atomic<uint32_t> g_round_limit{0u};
 static std::mutex g_rules_mu;
 static std::vector<mans_geometry_rule> g_rules;
 
@@ -1239,6 +1332,7 @@ static int launch_pipeline(KArgs a, uint32_t n_chunk_tasks, uint32_t n_item_task
     size_t lds_s = lds + (size_t)kWG * kLS * 4u;   // table + round buffer
     auto mark = [&](int i) { if (ev) (void)hipEventRecord((hipEvent_t)ev[i], st); };
     a.wait_polls = g_wait_polls.load();
+    a.round_limit = g_round_limit.load();
     mark(0);
     hipLaunchKernelGGL(k_init, dim3((std::max({a.n_streams, a.n_tasks, 64u}) + 255u) / 256u),
                        dim3(256), 0, st, a, 64u);
@@ -1268,6 +1362,10 @@ static int launch_pipeline(KArgs a, uint32_t n_chunk_tasks, uint32_t n_item_task
         hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kWG), 0, st, a, total_chunks, bsum);
     }
     if (a.any_index_out) hipLaunchKernelGGL(k_index, dim3(n_chunk_tasks), dim3(kWG), 0, st, a);
+    if (a.ref_sync) {
+        const uint32_t lanes = std::max(a.one.n_ck, a.ref_nblocks);
+        hipLaunchKernelGGL(k_refsync, dim3((lanes + kWG - 1u) / kWG), dim3(kWG), 0, st, a);
+    }
     mark(5);
     {
         // persistent write pass: workgroups take blocks of 4 passes from a device counter
@@ -1330,12 +1428,12 @@ extern "C" int mans_workspace_bytes_geometry(uint64_t n_units, uint32_t chunk_un
     return MANS_OK;
 }
 
-extern "C" int mans_decode(const mans_decode_args* d) {
-    if (!d) return set_error(MANS_EINVAL, "args is NULL");
+// Plan of a single-stream decode (shared by mans_decode and mans_ref_sync_fill): the
+// kernel arguments over the caller's workspace and the task counts.
+static int single_plan(const mans_decode_args* d, KArgs* out, uint32_t* nct, uint32_t* nit) {
     int rc = check_stream_args(d->units, d->n_units, d->out, d->n_out, d->table, d->L,
                                d->first_state, d->first_bit);
     if (rc) return rc;
-    if (d->n_out == 0) return MANS_OK;
     uint32_t S, K;
     rc = stream_geometry(d->n_units, d->n_out, d->L, d->chunk_units, d->checkpoint_units,
                          d->index_in, d->index_out, &S, &K);
@@ -1374,7 +1472,7 @@ extern "C" int mans_decode(const mans_decode_args* d) {
                                       "%u (and no index_out)", kWI);
     if (((uintptr_t)d->items_out) & 7u)
         return set_error(MANS_EINVAL, "items_out must be 8-byte aligned");
-    KArgs a;
+    KArgs& a = *out;
     std::memset(&a, 0, sizeof(a));
     a.one = sp;
     a.plans = nullptr;
@@ -1382,10 +1480,37 @@ extern "C" int mans_decode(const mans_decode_args* d) {
     a.any_index_out = d->index_out ? 1u : 0u;
     a.direct = (d->flags & MANS_F_TRUST_SYNC_POINTS) ? 1u : 0u;
     a.w = make_work(d->workspace, l);
-    uint32_t nct = (sp.n_chunks + kWG - 1) / kWG;
-    uint32_t nit = (n_write_items(sp.n_ck) + kWG - 1) / kWG;
-    a.n_tasks = nct;
-    rc = launch_pipeline(a, nct, nit, sp.n_chunks, sp.L, sp.K, (hipStream_t)d->stream, d->kernel_events);
+    if (d->ref_sync) {
+        const mans_ref_sync& r = *d->ref_sync;
+        if (!r.sync_points || !r.subsequence_units || !r.threads || a.direct)
+            return set_error(MANS_EINVAL, "ref_sync needs sync_points, subsequence_units and "
+                                          "threads (and a proving decode)");
+        if ((((uintptr_t)r.sync_points) | ((uintptr_t)r.output_sizes)) & 3u)
+            return set_error(MANS_EINVAL, "ref_sync arrays must be 4-byte aligned");
+        a.ref_sync = r.sync_points;
+        a.ref_sizes = r.output_sizes;
+        a.ref_flags = r.synced_flags;
+        a.ref_S = r.subsequence_units;
+        a.ref_nsub = (uint32_t)((d->n_units + r.subsequence_units - 1u) / r.subsequence_units);
+        a.ref_nblocks = (a.ref_nsub + r.threads - 1u) / r.threads;
+    }
+    *nct = (sp.n_chunks + kWG - 1) / kWG;
+    *nit = (n_write_items(sp.n_ck) + kWG - 1) / kWG;
+    a.n_tasks = *nct;
+    return MANS_OK;
+}
+
+extern "C" int mans_decode(const mans_decode_args* d) {
+    if (!d) return set_error(MANS_EINVAL, "args is NULL");
+    if (d->n_out == 0)
+        return check_stream_args(d->units, d->n_units, d->out, d->n_out, d->table, d->L,
+                                 d->first_state, d->first_bit);
+    KArgs a;
+    uint32_t nct, nit;
+    int rc = single_plan(d, &a, &nct, &nit);
+    if (rc) return rc;
+    rc = launch_pipeline(a, nct, nit, a.one.n_chunks, a.one.L, a.one.K, (hipStream_t)d->stream,
+                         d->kernel_events);
     if (rc) return rc;
     if (d->flags & MANS_F_SYNC_ERRORS) {
         int status = 0;
@@ -1393,6 +1518,21 @@ extern "C" int mans_decode(const mans_decode_args* d) {
         if (rc) return rc;
         if (status) return set_error(status, "device reported status %d", status);
     }
+    return MANS_OK;
+}
+
+extern "C" int mans_ref_sync_fill(const mans_decode_args* d) {
+    if (!d || !d->ref_sync) return set_error(MANS_EINVAL, "args or ref_sync is NULL");
+    if (d->n_out == 0) return MANS_OK;
+    KArgs a;
+    uint32_t nct, nit;
+    int rc = single_plan(d, &a, &nct, &nit);
+    if (rc) return rc;
+    const uint32_t lanes = std::max(a.one.n_ck, a.ref_nblocks);
+    hipLaunchKernelGGL(k_refsync, dim3((lanes + kWG - 1u) / kWG), dim3(kWG), 0,
+                       (hipStream_t)d->stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(MANS_EHIP, "kernel launch: %s", hipGetErrorString(e));
     return MANS_OK;
 }
 
@@ -1404,7 +1544,8 @@ extern "C" int mans_decode_status(void* workspace, void* stream, int* status) {
     hipError_t e = hipMemcpyAsync(&v, st, 4, hipMemcpyDeviceToHost, (hipStream_t)stream);
     if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
     if (e != hipSuccess) return set_error(MANS_EHIP, "status read: %s", hipGetErrorString(e));
-    *status = (v & 1u) ? MANS_ETABLE : 0;
+    // bit 0: a rejected table; bit 1: sync rounds that did not converge (k_fixup)
+    *status = (v & 1u) ? MANS_ETABLE : (v & 2u) ? MANS_ECORRUPT : 0;
     return MANS_OK;
 }
 
@@ -1529,6 +1670,7 @@ constexpr size_t kStreamsPerPart = 256;   // fewer streams per part: one pipelin
 
 struct BatchPart {
     std::vector<mans_stream_desc> descs;
+    std::vector<size_t> orig;            // caller index of each of descs
     std::vector<StreamPlan> plans;
     uint64_t chunks = 0, cks = 0, units = 0, syms = 0, off = 0;
     uint32_t ct = 0, it = 0, ml = 0, mk = 0;
@@ -1542,6 +1684,7 @@ static int plan_parts(const mans_stream_desc* d, size_t n, size_t k, std::vector
     parts.assign(k, BatchPart());
     if (k == 1) {
         parts[0].descs.assign(d, d + n);
+        for (size_t i = 0; i < n; ++i) parts[0].orig.push_back(i);
     } else {
         // longest processing time first: each stream to the part with the fewest symbols
         std::vector<size_t> idx(n);
@@ -1559,7 +1702,10 @@ static int plan_parts(const mans_stream_desc* d, size_t n, size_t k, std::vector
         }
         for (size_t p = 0; p < k; ++p) {
             std::sort(mine[p].begin(), mine[p].end());
-            for (size_t i : mine[p]) parts[p].descs.push_back(d[i]);
+            for (size_t i : mine[p]) {
+                parts[p].descs.push_back(d[i]);
+                parts[p].orig.push_back(i);
+            }
         }
     }
     uint64_t o = 0;
@@ -1587,6 +1733,15 @@ static int mark_direct(std::vector<BatchPart>& parts, const mans_stream_desc* d,
                                           "index_in at chunk_units %u (and no index_out)", i, kWI);
     for (BatchPart& bp : parts) bp.direct = 1u;
     return MANS_OK;
+}
+
+// opts->stream_status (optional, device, one u32 per caller stream): each stream's plan
+// points at its word (k_init zeroes it; the kernels OR the stream's errors into it).
+static void attach_status(std::vector<BatchPart>& parts, const mans_batch_opts* opts) {
+    if (!opts->stream_status) return;
+    for (BatchPart& bp : parts)
+        for (size_t i = 0; i < bp.plans.size(); ++i)
+            bp.plans[i].status_out = opts->stream_status + bp.orig[i];
 }
 
 static size_t n_parts(size_t n) {
@@ -1765,6 +1920,9 @@ extern "C" int mans_decode_batch(const mans_stream_desc* descs, size_t n,
     int rc = plan_parts(descs, n, k, parts, &need);
     if (!rc) rc = mark_direct(parts, descs, n, opts->flags);
     if (rc) return rc;
+    if (((uintptr_t)opts->stream_status) & 3u)
+        return set_error(MANS_EINVAL, "stream_status must be 4-byte aligned");
+    attach_status(parts, opts);
     if (!opts->workspace || opts->workspace_bytes < need)
         return set_error(MANS_ENOMEM, "batch workspace of %llu bytes < %llu needed",
                          (unsigned long long)opts->workspace_bytes, (unsigned long long)need);
@@ -1817,6 +1975,9 @@ extern "C" int mans_batch_graph_create(const mans_stream_desc* descs, size_t n,
     int rc = plan_parts(descs, n, n_parts(n), parts, &need);
     if (!rc) rc = mark_direct(parts, descs, n, opts->flags);
     if (rc) return rc;
+    if (((uintptr_t)opts->stream_status) & 3u)
+        return set_error(MANS_EINVAL, "stream_status must be 4-byte aligned");
+    attach_status(parts, opts);
     if (!opts->workspace || opts->workspace_bytes < need)
         return set_error(MANS_ENOMEM, "batch workspace of %llu bytes < %llu needed",
                          (unsigned long long)opts->workspace_bytes, (unsigned long long)need);
@@ -1890,6 +2051,13 @@ extern "C" int mans_graph_destroy(void* graph) {
 
 extern "C" int mans_set_wait_polls(uint32_t polls) {
     g_wait_polls.store(polls);
+    return MANS_OK;
+}
+
+uint32_t sync_round_limit() { return mans::g_round_limit.load(); }
+
+extern "C" int mans_set_sync_round_limit(uint32_t rounds) {
+    g_round_limit.store(rounds);
     return MANS_OK;
 }
 

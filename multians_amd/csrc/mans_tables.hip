@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "mans_c_api.h"
@@ -254,5 +255,96 @@ extern "C" int mans_table_status(const void* workspace, size_t n_jobs, void* str
     }
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) return set_error(MANS_EHIP, "status: %s", hipGetErrorString(e));
+    return MANS_OK;
+}
+
+// ------------------------------------------------------------ symbols on the device
+// mans_gen_symbols_device: the same symbols as mans_gen_symbols (mans_codec.cc), element
+// i from (seed, i) alone: u = splitmix64(seed + (i + 1) * golden) >> 11, uniform on
+// [0, 2^53), s = #{k in 1..255 : u < thr[k]} with thr[k] = floor(2^53 exp(-lambda k))
+// computed on the host (the same libm call as the host generator, so equal bytes).  The
+// generator of the demo data (src/main.cc:55-73 draws from std::exponential_distribution);
+// used to make large test and benchmark streams without a host round trip.
+namespace {
+
+struct GenThr {
+    uint64_t thr[256];
+};
+
+__device__ __forceinline__ uint64_t dev_mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// largest k in [lo, hi] with u < thr[k] (u < thr[lo] holds)
+__device__ __forceinline__ uint32_t dev_sym_of(const uint64_t* thr, uint64_t u, uint32_t lo,
+                                               uint32_t hi) {
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (u < thr[mid]) lo = mid; else hi = mid - 1u;
+    }
+    return lo;
+}
+
+constexpr uint32_t kGenWG = 256;
+
+// Grid-stride over 16-symbol blocks; per workgroup the thresholds and, per 2^41-wide
+// bucket of u (its top 12 bits), the symbol range [s(bucket end), s(bucket start)]
+// sit in LDS, so most symbols take one lookup.
+__global__ __launch_bounds__(kGenWG) void k_gen_symbols(GenThr t, uint64_t seed, uint8_t* out,
+                                                        uint64_t n) {
+    __shared__ uint64_t thr[256];
+    __shared__ uint8_t hi[4096], lo[4096];
+    thr[threadIdx.x] = t.thr[threadIdx.x];
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < 4096u; k += kGenWG) {
+        hi[k] = (uint8_t)dev_sym_of(thr, (uint64_t)k << 41, 0u, 255u);
+        lo[k] = (uint8_t)dev_sym_of(thr, (((uint64_t)k + 1u) << 41) - 1u, 0u, 255u);
+    }
+    __syncthreads();
+    const uint64_t nb = (n + 15u) / 16u;
+    for (uint64_t b = (uint64_t)blockIdx.x * kGenWG + threadIdx.x; b < nb;
+         b += (uint64_t)gridDim.x * kGenWG) {
+        uint32_t w[4];
+        for (uint32_t q = 0; q < 4u; ++q) {
+            uint32_t v = 0u;
+            for (uint32_t r = 0; r < 4u; ++r) {
+                const uint64_t i = b * 16u + q * 4u + r;
+                const uint64_t u = dev_mix64(seed + (i + 1u) * 0x9E3779B97F4A7C15ull) >> 11;
+                const uint32_t k = (uint32_t)(u >> 41);
+                const uint32_t h = hi[k], l = lo[k];
+                const uint32_t s = h == l ? h : dev_sym_of(thr, u, l, h);
+                v |= s << (8u * r);
+            }
+            w[q] = v;
+        }
+        const uint64_t i0 = b * 16u;
+        uint8_t* dst = out + i0;
+        if (i0 + 16u <= n && (((uintptr_t)dst) & 15u) == 0u) {
+            *reinterpret_cast<uint4*>(dst) = make_uint4(w[0], w[1], w[2], w[3]);
+        } else {
+            for (uint32_t j = 0; j < 16u && i0 + j < n; ++j)
+                dst[j] = (uint8_t)(w[j >> 2] >> (8u * (j & 3u)));
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" int mans_gen_symbols_device(uint64_t seed, double lambda, uint8_t* out, uint64_t n,
+                                       void* stream) {
+    if (!out && n) return set_error(MANS_EINVAL, "out is NULL");
+    if (!(lambda > 0.0)) return set_error(MANS_EINVAL, "lambda must be > 0");
+    if (!n) return MANS_OK;
+    GenThr t;
+    for (int k = 0; k < 256; ++k)
+        t.thr[k] = (uint64_t)std::floor(std::ldexp(std::exp(-lambda * k), 53));
+    const uint64_t nb = (n + 15u) / 16u;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((nb + kGenWG - 1u) / kGenWG, 4096u);
+    hipLaunchKernelGGL(k_gen_symbols, dim3(grid), dim3(kGenWG), 0, (hipStream_t)stream, t, seed,
+                       out, n);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(MANS_EHIP, "gen launch: %s", hipGetErrorString(e));
     return MANS_OK;
 }

@@ -426,3 +426,107 @@ class BatchGraph:
             self.close()
         except Exception:
             pass
+
+
+class GeneratedBatch:
+    """Streams made on the device alone, for workloads too large to generate and encode on
+    the host quickly: symbols (mans_gen_symbols_device, the same bytes as the host
+    generator), decoder tables from their histograms (mans_build_tables_device, the host
+    codec's bytes) and units (mans_encode_device, unit-exact against the host codec).
+    specs: [(seed, lambda, n_symbols, L)].  Buffers are packed, each stream 16-byte
+    aligned; `symbols(i)` is stream i in encoder order (the decoder emits it reversed)."""
+
+    def __init__(self, specs, device="cuda"):
+        torch = _torch()
+        self.specs = list(specs)
+        n = len(self.specs)
+        self.n = n
+        L = np.array([sp[3] for sp in self.specs], np.int64)
+        nsym = np.array([sp[2] for sp in self.specs], np.int64)
+        cap = np.array([(_lib.lib().mans_max_encoded_units(int(k)) + 3) // 4 * 4 + 4
+                        for k in nsym], np.int64)
+        self.soff = np.concatenate([[0], np.cumsum((nsym + 15) // 16 * 16)])
+        self.toff = np.concatenate([[0], np.cumsum(L)])
+        self.uoff = np.concatenate([[0], np.cumsum(cap)])
+        self.L, self.n_symbols = L, nsym
+        self.sym = torch.empty(max(int(self.soff[-1]), 16), dtype=torch.uint8, device=device)
+        self.tables = torch.zeros(max(int(self.toff[-1]), 1), dtype=torch.int32, device=device)
+        self.units = torch.zeros(max(int(self.uoff[-1]), 4), dtype=torch.int32, device=device)
+        for i, (seed, lam, k, _) in enumerate(self.specs):
+            if k:
+                _lib.gen_symbols_device(seed, lam, self.symbols(i))
+        jobs = [(self.symbols(i), int(L[i]), self.table(i), None) for i in range(n)]
+        st = _lib.build_tables_device(jobs)
+        if any(st):
+            raise RuntimeError("device tables: status %s" % sorted(set(st)))
+        ej = [(self.symbols(i), self.table(i), int(L[i]),
+               self.units[int(self.uoff[i]):int(self.uoff[i + 1])]) for i in range(n)]
+        res = _lib.encode_device(ej)
+        if any(r[3] for r in res):
+            raise RuntimeError("device encode: status %s" % sorted(set(r[3] for r in res)))
+        self.n_units = np.array([r[0] for r in res], np.int64)
+        self.first_bit = np.array([r[1] for r in res], np.int64)
+        self.first_state = np.array([r[2] for r in res], np.int64)
+        self.device = device
+
+    def symbols(self, i):
+        return self.sym[int(self.soff[i]):int(self.soff[i]) + int(self.n_symbols[i])]
+
+    def table(self, i):
+        return self.tables[int(self.toff[i]):int(self.toff[i + 1])]
+
+    def stream_units(self, i):
+        return self.units[int(self.uoff[i]):int(self.uoff[i]) + int(self.n_units[i])]
+
+    def encoded(self, i):
+        """Host copy of stream i as an EncodedStream (for the oracle)."""
+        return EncodedStream(self.stream_units(i).cpu().numpy().view(np.uint32).copy(),
+                             int(self.first_bit[i]), int(self.first_state[i]),
+                             self.table(i).cpu().numpy().view(np.uint32).copy(),
+                             int(self.L[i]), int(self.n_symbols[i]))
+
+    def descs(self, out, ooff, index=None):
+        """StreamDesc array decoding stream i into out[ooff[i]:ooff[i + 1]]."""
+        d = (_lib.StreamDesc * self.n)()
+        ob = out.data_ptr()
+        for i in range(self.n):
+            x = d[i]
+            x.units = self.units.data_ptr() + 4 * int(self.uoff[i])
+            x.n_units = int(self.n_units[i])
+            x.out = ob + int(ooff[i])
+            x.n_out = int(self.n_symbols[i])
+            x.table = self.tables.data_ptr() + 4 * int(self.toff[i])
+            x.L = int(self.L[i])
+            x.first_state = int(self.first_state[i])
+            x.first_bit = int(self.first_bit[i])
+        return d
+
+    def decode_one(self, i, out, ws, stream=None, **kw):
+        """mans_decode of stream i into the u8 tensor out with workspace tensor ws."""
+        torch = _torch()
+        a = _lib.DecodeArgs()
+        a.units = self.units.data_ptr() + 4 * int(self.uoff[i])
+        a.n_units = int(self.n_units[i])
+        a.out = out.data_ptr()
+        a.n_out = int(self.n_symbols[i])
+        a.table = self.tables.data_ptr() + 4 * int(self.toff[i])
+        a.L = int(self.L[i])
+        a.first_state = int(self.first_state[i])
+        a.first_bit = int(self.first_bit[i])
+        a.workspace = ws.data_ptr()
+        a.workspace_bytes = ws.numel()
+        a.stream = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        a.kernel_events = kw.get("kernel_events")
+        a.chunk_units = kw.get("chunk_units", 0)
+        idx = kw.get("index_out")
+        a.index_out = idx.data_ptr() if idx is not None else None
+        _lib.check(_lib.lib().mans_decode(C.byref(a)))
+        return out
+
+    def status(self, ws, stream=None):
+        torch = _torch()
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        s = C.c_int()
+        _lib.check(_lib.lib().mans_decode_status(C.c_void_p(ws.data_ptr()), C.c_void_p(st),
+                                                 C.byref(s)))
+        return s.value

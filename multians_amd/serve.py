@@ -143,6 +143,7 @@ class Request:
     t_admit: float = 0.0
     t_done: float = 0.0
     out_off: int = 0      # arena offset of the output
+    status: int = -1      # the decoder's status word for this request (0 = ok)
 
 
 @dataclass
@@ -153,8 +154,10 @@ class _Batch:
     hip_stream: object
     descs: object
     t_launch: float
-    check: object = None
-    fresh: list = None        # pool streams whose item records this batch writes
+    status: object = None     # pinned host copy of every request's status word
+    check: object = None      # pinned host copy of the checked requests' mismatch flags
+    check_idx: list = None    # positions in reqs of the checked requests
+    fresh: list = None        # (position in reqs, pool stream) whose item records it writes
 
 
 @dataclass
@@ -167,6 +170,7 @@ class ServeStats:
     peak_bytes: int = 0
     checked: int = 0
     mismatches: int = -1
+    failed: int = 0           # requests whose decode reported a nonzero status
 
     def summary(self):
         lat = self.latencies_ms
@@ -181,6 +185,7 @@ class ServeStats:
             "peak_hbm_GB": round(self.peak_bytes / 1e9, 3),
             "checked": self.checked,
             "mismatches": self.mismatches,
+            "failed": self.failed,
         }
 
 
@@ -249,15 +254,20 @@ class Server:
         return launches, out_bytes, ws
 
     def _launch(self, reqs, t_now, hs, check_every):
+        torch = self.torch
         launches, out_bytes, ws_bytes = self._batch_block(reqs)
-        blk = self.arena.alloc(out_bytes + ws_bytes)
+        # the block also holds one status word per request (the decoder writes them)
+        st_bytes = _up(4 * len(reqs))
+        blk = self.arena.alloc(out_bytes + ws_bytes + st_bytes)
         if blk is None:
             return None
         off, nb = blk
         base = self.arena.base + off
+        st_off = off + out_bytes + ws_bytes
         for r in reqs:
             r.out_off += off
         fresh = []
+        g0 = 0                                  # first status word of the group
         for group, trusted, descs in launches:
             for j, r in enumerate(group):
                 descs[j].out = self.arena.base + r.out_off
@@ -265,28 +275,37 @@ class Server:
             o.workspace = base + out_bytes
             o.workspace_bytes = ws_bytes
             o.stream = hs.cuda_stream
+            o.stream_status = self.arena.base + st_off + 4 * g0
             # one pipeline per batch: the server already keeps max_inflight batches in
             # flight on its own streams
             o.flags = _lib.MANS_BATCH_ONE_PIPELINE | \
                 (_lib.MANS_BATCH_TRUST_SYNC_POINTS if trusted else 0)
             _lib.check(self.lib.mans_decode_batch(descs, len(group), C.byref(o), None))
             if self.use_records and not trusted:
-                fresh.extend(r.stream for r in group)
+                fresh.extend((g0 + j, r.stream) for j, r in enumerate(group))
+            g0 += len(group)
         descs = [d for _, _, d in launches]
-        check = None
-        if check_every and self.pool.sym is not None:
-            torch = self.torch
-            with torch.cuda.stream(hs):
-                for r in reqs:
+        order = [r for g, _, _ in launches for r in g]       # status word order
+        status = torch.empty(len(reqs), dtype=torch.int32, pin_memory=True)
+        check, check_idx = None, None
+        with torch.cuda.stream(hs):
+            words = self.arena.buf[st_off:st_off + 4 * len(reqs)].view(torch.int32)
+            status.copy_(words, non_blocking=True)
+            if check_every and self.pool.sym is not None:
+                flags, check_idx = [], []
+                for k, r in enumerate(order):
                     if r.rid % check_every:
                         continue
                     n = int(self.pool.n_sym[r.stream])
                     got = self.arena.buf[r.out_off:r.out_off + n]
-                    bad = (got != self.pool.expected(r.stream)).any().to(torch.int32)
-                    check = bad if check is None else check + bad
-        ev = self.torch.cuda.Event()
+                    flags.append((got != self.pool.expected(r.stream)).any())
+                    check_idx.append(k)
+                if flags:
+                    check = torch.empty(len(flags), dtype=torch.bool, pin_memory=True)
+                    check.copy_(torch.stack(flags), non_blocking=True)
+        ev = torch.cuda.Event()
         ev.record(hs)
-        return _Batch(reqs, blk, ev, hs, descs, t_now, check, fresh)
+        return _Batch(order, blk, ev, hs, descs, t_now, status, check, check_idx, fresh)
 
     def run(self, arrivals, stream_ids, check_every=0, poll_sleep=20e-6):
         """arrivals: sorted arrival times (s); stream_ids: pool index per request.
@@ -301,7 +320,8 @@ class Server:
         sizes = []
         n_batches = 0
         checked = 0
-        bad_total = torch.zeros((), dtype=torch.int32, device=self.arena.buf.device)
+        failed = 0
+        bad_total = 0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         while nxt < len(reqs) or pending or inflight:
@@ -314,16 +334,23 @@ class Server:
             for b in inflight:
                 if b.event.query():
                     t = time.perf_counter() - t0
-                    for r in b.reqs:
+                    status = b.status.numpy()
+                    for r, stw in zip(b.reqs, status):
                         r.t_done = t
+                        r.status = int(stw)
                     done.extend(b.reqs)
+                    failed += int(np.count_nonzero(status))
                     if b.check is not None:
-                        bad_total += b.check
-                        checked += sum(1 for r in b.reqs if r.rid % max(check_every, 1) == 0)
+                        flags = b.check.numpy()
+                        for k, bad in zip(b.check_idx, flags):
+                            if status[k] == 0:      # a failed request's output is undefined
+                                checked += 1
+                                bad_total += int(bad)
                     self.arena.free(*b.block)
                     free_streams.append(b.hip_stream)
-                    if b.fresh:                # their records are complete now
-                        self.pool.have_rec[b.fresh] = True
+                    for k, sid in (b.fresh or ()):
+                        if status[k] == 0:      # its records are complete and proven now
+                            self.pool.have_rec[sid] = True
                 else:
                     still.append(b)
             inflight = still
@@ -336,9 +363,10 @@ class Server:
                     r = pending[0]
                     n = int(self.pool.n_sym[r.stream])
                     # outputs plus a workspace bound (about 1.5 B per unit at K = 32,
-                    # about 2.5 B per unit for a decode from 32-unit item records)
+                    # about 2.5 B per unit for a decode from 32-unit item records) and
+                    # the request's status word
                     est = _up(n) + (3 if self.use_records else 2) * \
-                        int(self.pool.n_units[r.stream]) + 4096
+                        int(self.pool.n_units[r.stream]) + 4096 + 4
                     if take and need + est > room:
                         break
                     take.append(pending.popleft())
@@ -368,9 +396,9 @@ class Server:
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         lat = np.array([(r.t_done - r.t_arrival) * 1e3 for r in done])
-        syms = int(sum(int(self.pool.n_sym[r.stream]) for r in done))
+        syms = int(sum(int(self.pool.n_sym[r.stream]) for r in done if r.status == 0))
         st = ServeStats(lat, syms, wall, n_batches, sizes, self.arena.peak, checked,
-                        int(bad_total.item()) if check_every else -1)
+                        bad_total if check_every else -1, failed)
         return st
 
 

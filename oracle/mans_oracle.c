@@ -240,6 +240,30 @@ int mo_decode_seq(const uint32_t* units, size_t n_units, const uint32_t* table, 
     return 0;
 }
 
+/* Last step owned by each subsequence of S units (the step the reference's phase 1
+ * records per subsequence: multians_gpu_decoder.cu:160-168 keeps last_state, last_word_unit
+ * and last_word_bit of the last codeword that starts in the subsequence), over the first
+ * n_out steps: its state before the step and its absolute start bit.  A step belongs to
+ * the subsequence holding the unit it starts in; steps at or beyond unit n_sub*S belong to
+ * the last one.  Subsequences owning no step keep last_state 0. */
+int mo_subseq_last(const uint32_t* units, size_t n_units, const uint32_t* table, uint32_t L,
+                   uint32_t first_state, uint32_t first_bit, size_t n_out, uint32_t S,
+                   uint32_t* last_state, uint64_t* last_bit) {
+    uint64_t p = 32u - first_bit;
+    uint32_t x = first_state;
+    size_t n_sub = (n_units + S - 1) / S;
+    for (size_t j = 0; j < n_sub; ++j) { last_state[j] = 0; last_bit[j] = 0; }
+    for (size_t k = 0; k < n_out; ++k) {
+        size_t j = (size_t)(p / (32ull * S));
+        if (j >= n_sub) j = n_sub - 1;
+        last_state[j] = x;
+        last_bit[j] = p;
+        uint8_t s;
+        if (mo_step(units, n_units, table, L, &x, &p, &s)) return -1;
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------- multicore decoder */
 /* Restatement of MulticoreDecoder (multicore_decoder.cc:17-333): T threads, each
  * owning an interval of subsequences (:275-303); a speculative pass from the

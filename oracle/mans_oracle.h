@@ -22,6 +22,8 @@
  *                        invariant (decoder ends at state L, bit 32*n_units).
  *   mo_decode_seq        round trip against the original symbols; brute-force
  *                        re-derivation of entries/counts on tiny streams.
+ *   mo_subseq_last       brute-force re-derivation on tiny streams; one step from each
+ *                        record lands on the next subsequence's entry (mo_decode_seq).
  *   mo_decode_multicore  vs the compiled reference MulticoreDecoder (same S, T),
  *                        including the streams where the reference drops trailing
  *                        zero-bit symbols.
@@ -78,6 +80,15 @@ int mo_decode_seq(const uint32_t* units, size_t n_units, const uint32_t* table, 
                   uint32_t first_state, uint32_t first_bit, uint8_t* out, size_t n_out,
                   uint32_t S, uint32_t* entries, uint32_t* counts,
                   uint32_t* end_state, uint64_t* end_bit);
+
+/* Last step owned by each of the ceil(n_units / S) subsequences within the first n_out
+ * steps: last_state[j] = state before it, last_bit[j] = its absolute start bit (the step
+ * whose position the reference's phase 1 records per subsequence,
+ * multians_gpu_decoder.cu:160-168).  Steps beyond the last subsequence's units count as
+ * its own.  Returns 0 or -1. */
+int mo_subseq_last(const uint32_t* units, size_t n_units, const uint32_t* table, uint32_t L,
+                   uint32_t first_state, uint32_t first_bit, size_t n_out, uint32_t S,
+                   uint32_t* last_state, uint64_t* last_bit);
 
 /* Restatement of MulticoreDecoder::decode (multicore_decoder.cc:17-90) with pthreads.
  * units must be padded by >= S+2 zero units after n_units (the reference pads 4 and

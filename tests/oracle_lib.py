@@ -35,6 +35,8 @@ def _load_oracle():
     lib.mo_decode_seq.argtypes = [_u32p, C.c_size_t, _u32p, C.c_uint32, C.c_uint32, C.c_uint32,
                                   _u8p, C.c_size_t, C.c_uint32, C.c_void_p, C.c_void_p,
                                   C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]
+    lib.mo_subseq_last.argtypes = [_u32p, C.c_size_t, _u32p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                   C.c_size_t, C.c_uint32, _u32p, _u64p]
     lib.mo_decode_multicore.argtypes = [C.c_size_t, C.c_size_t, _u32p, C.c_size_t, _u32p,
                                         C.c_uint32, C.c_uint32, C.c_uint32, _u8p, C.c_size_t]
     return lib
@@ -147,6 +149,19 @@ def decode_seq(st, n_out=None, S=0):
     if rc:
         raise ValueError("decode_seq failed")
     return out, entries[:nsub], counts[:nsub], es.value, eb.value
+
+
+def subseq_last(st, S, n_out=None):
+    """(state before, absolute start bit) of the last step of each subsequence of S units."""
+    n_out = st.n if n_out is None else n_out
+    nsub = (len(st.units) + S - 1) // S
+    ls = np.zeros(max(nsub, 1), np.uint32)
+    lb = np.zeros(max(nsub, 1), np.uint64)
+    rc = oracle().mo_subseq_last(st.units, len(st.units), st.table, st.L, st.first_state,
+                                 st.first_bit, n_out, S, ls, lb)
+    if rc:
+        raise ValueError("subseq_last failed")
+    return ls[:nsub], lb[:nsub]
 
 
 def decode_multicore(st, S=4, T=4, n_out=None, pad=None):

@@ -130,3 +130,32 @@ def test_spec_warmup(emu, lam, S, warm):
                                   st.first_state, st.first_bit, S, 16, warm, stats)
     assert rc == 0
     assert np.array_equal(out[::-1], sym)
+
+
+def test_fixup_round_bound(emu):
+    """The fixup rounds stop after n_chunks + 1 rounds (or a smaller limit) and report a
+    stream still inconsistent then (-5, the kernel's status bit 1) instead of looping on.
+    16-unit chunks at lambda = 1 need many rounds (the merge distance is hundreds of
+    units): one round is too few, the default bound is never reached, and the number of
+    rounds the default run takes is a limit that suffices exactly."""
+    emu.mans_emu_decode_limited.argtypes = [_u32p, C.c_uint64, _u8p, C.c_uint64, _u32p,
+                                            C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                            C.c_uint32, C.c_uint32, _u64p]
+    sym = ol.gen_symbols(17, 1.0, 400_000)
+    st, _ = ol.encode(sym, L=4096)
+
+    def run(limit):
+        out = np.zeros(len(sym), np.uint8)
+        stats = np.zeros(8, np.uint64)
+        rc = emu.mans_emu_decode_limited(st.units, len(st.units), out, len(sym), st.table, st.L,
+                                         st.first_state, st.first_bit, 16, 16, limit, stats)
+        return rc, out, stats
+
+    rc, out, stats = run(0)
+    assert rc == 0 and np.array_equal(out[::-1], sym)
+    rounds = int(stats[3])
+    assert 2 <= rounds <= int(stats[0])
+    assert run(1)[0] == -5
+    assert run(rounds - 1)[0] == -5
+    rc, out, _ = run(rounds)
+    assert rc == 0 and np.array_equal(out[::-1], sym)

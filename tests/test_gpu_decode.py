@@ -89,6 +89,33 @@ def test_range_wait_runs_out(polls):
     assert np.array_equal(got2, ref2)
 
 
+def test_fixup_round_limit_reports():
+    """The fixup rounds stop at their bound and report a stream that is still
+    inconsistent (status MANS_ECORRUPT) instead of looping on.  Forced from the host by
+    a one-round limit on a stream with 16-unit chunks, whose walks fail for many chunks
+    in a row (the merge distance is hundreds of units); with the default bound the same
+    stream decodes bit-exact."""
+    import torch
+    from multians_amd import _lib
+    from multians_amd.decoder import DeviceStream, encode_symbols
+    print("case round limit", flush=True)
+    sym = ol.gen_symbols(17, 1.0, 400_000)
+    es = encode_symbols(sym, 4096)
+    ds = DeviceStream(es, chunk_units=16, checkpoint_units=16)
+    _lib.set_sync_round_limit(1)
+    try:
+        ds.decode(chunk_units=16, checkpoint_units=16)
+        torch.cuda.synchronize()
+        limited = ds.status()
+    finally:
+        _lib.set_sync_round_limit()
+    assert limited == _lib.MANS_ECORRUPT
+    out = ds.decode(chunk_units=16, checkpoint_units=16)
+    torch.cuda.synchronize()
+    assert ds.status() == 0
+    assert np.array_equal(out[:len(sym)].cpu().numpy()[::-1], sym)
+
+
 @pytest.mark.parametrize("spec", [(2026181085674, 0.25656187489628735, 1_105_978),
                                   (2026181087171, 0.17296609080379266, 1_971_802)])
 def test_walk_enters_chunk_as_another_leaves(spec):

@@ -180,3 +180,24 @@ def test_encode_writes_decoder_item_records(lam, L, n):
         assert np.array_equal(out[:n].cpu().numpy()[::-1], sym)
     else:
         assert (nrec, int(enc_rec["state"][0]), int(enc_rec["bit"][0])) == (1, 0, 32)
+
+
+def test_encode_round_limit_reports():
+    """The encoder's fixup rounds stop at their bound and report a stream still
+    inconsistent then (result status 8) instead of looping on.  Forced from the host by
+    a one-round limit on low-entropy streams of 4,096 chunks (encoding from a guessed
+    state meets the true path only after 1,000+ symbols there, so failed walks chain over
+    consecutive chunks); with the default bound every stream is unit-exact again."""
+    from multians_amd import _lib
+    print("case encode round limit", flush=True)
+    syms = [ol.gen_symbols(60 + i, lam, 1 << 24) for i, lam in enumerate([1.5, 1.9, 2.5, 3.0])]
+    jobs = [_job(s, _host(s, 4096)[3], 4096) for s in syms]
+    _lib.set_sync_round_limit(1)
+    try:
+        limited = _lib.encode_device(jobs)
+    finally:
+        _lib.set_sync_round_limit()
+    assert any(r[3] == 8 for r in limited)
+    assert all(r[3] in (0, 8) for r in limited)
+    for s, res, job in zip(syms, _lib.encode_device(jobs), jobs):
+        _check(s, 4096, res, job)

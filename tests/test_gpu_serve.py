@@ -80,3 +80,31 @@ def test_served_from_item_records():
         n = _lib.sync_point_count(es.n_units, 32)
         a = int((pool.rptr[i] - pool.rec.data_ptr()) // 16)
         assert torch.equal(pool.rec[16 * a:16 * (a + n)], rec[:16 * n])
+
+
+@pytest.mark.gpu
+def test_served_failed_stream_is_reported():
+    """A request whose stream has a rejected table (an entry with 16 bits) is reported as
+    failed from its own status word, and never gets item records; the requests batched
+    with it still succeed and are checked."""
+    import torch
+    from multians_amd import serve
+    import bench
+
+    specs = serve.request_mix(20, seed=12, max_bytes=1 << 20)
+    items = bench.make_streams(specs)
+    pool = serve.StreamPool([es for es, _ in items], [sym for _, sym in items], records=True)
+    bad = 5
+    t0 = int((pool.tptr[bad] - pool.tables.data_ptr()) // 4)
+    pool.tables[t0 + 3] = (16 << 24) | (int(pool.tables[t0 + 3]) & 0xFFFFFF)
+    torch.cuda.synchronize()
+    srv = serve.Server(pool, 1 << 30, max_inflight=2, use_records=True)
+    ids = np.arange(60) % pool.n
+    st = srv.run(np.zeros(len(ids)), ids, check_every=1)
+    s = st.summary()
+    print(s, flush=True)
+    n_bad = int(np.count_nonzero(ids == bad))
+    assert s["failed"] == n_bad
+    assert st.checked == len(ids) - n_bad and st.mismatches == 0
+    assert not pool.have_rec[bad]
+    assert pool.have_rec.sum() == pool.n - 1

@@ -110,3 +110,21 @@ def test_tables_large_input_histogram():
     hc, hd = _host(sym, 4096)
     assert np.array_equal(counts.cpu().numpy().view(np.uint32), hc)
     assert np.array_equal(table.cpu().numpy().view(np.uint32), hd)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lam", [0.002, 0.01, 0.3, 1.0, 3.0])
+def test_device_symbols_equal_host(lam):
+    """mans_gen_symbols_device writes the host generator's bytes (whole blocks, a ragged
+    tail, and an output that is not 16-byte aligned)."""
+    import torch
+    from multians_amd import _lib
+    print(f"case gen lam={lam}", flush=True)
+    for n, skew in [(1, 0), (17, 0), (1_000_003, 0), (5_000_000, 3)]:
+        host = _lib.gen_symbols(77, lam, n, threads=4)
+        buf = torch.zeros(n + 32, dtype=torch.uint8, device="cuda")
+        _lib.gen_symbols_device(77, lam, buf[skew:skew + n])
+        torch.cuda.synchronize()
+        got = buf.cpu().numpy()
+        assert np.array_equal(got[skew:skew + n], host)
+        assert not got[:skew].any() and not got[skew + n:].any()

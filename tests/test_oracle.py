@@ -171,6 +171,43 @@ def test_entries_counts_bruteforce():
         assert list(ent) == e_bf and list(cnt) == c_bf
 
 
+def test_subseq_last_bruteforce():
+    """Last step of each subsequence re-derived by an independent pure-Python walk, and
+    one decoding step from each record (but the last) lands exactly on the next
+    subsequence's entry as mo_decode_seq reports it."""
+    L = 64
+    sym = ol.gen_symbols(5, 0.7, 900)
+    st, _ = ol.encode(sym, L=L)
+    bits = []
+    for u in st.units:
+        bits += [(int(u) >> i) & 1 for i in range(32)]
+    bits += [0] * 64
+
+    def step(x, p):
+        e = int(st.table[x - L])
+        nb = e >> 24
+        v = sum(bits[p + i] << i for i in range(nb))
+        return ((e & 0xFFFF) << nb) | v, p + nb
+
+    for S in (1, 2, 3, 4, 7):
+        ls, lb = ol.subseq_last(st, S)
+        nsub = -(-len(st.units) // S)
+        p, x = 32 - st.first_bit, st.first_state
+        s_bf, b_bf = [0] * nsub, [0] * nsub
+        for _ in range(len(sym)):
+            j = min(p // (32 * S), nsub - 1)
+            s_bf[j], b_bf[j] = x, p
+            x, p = step(x, p)
+        assert list(ls) == s_bf and list(lb) == b_bf
+        _, ent, cnt, _, _ = ol.decode_seq(st, S=S)
+        for j in range(nsub - 1):
+            if cnt[j] == 0 or cnt[j + 1] == 0:
+                continue
+            x2, p2 = step(int(ls[j]), int(lb[j]))
+            assert ent[j + 1] == (x2 - L) | ((p2 - 32 * S * (j + 1)) << 16)
+            assert lb[j] < 32 * S * (j + 1) <= p2
+
+
 @pytest.mark.parametrize("lam", [0.05, 0.5, 2.0])
 def test_symbol_generator_closed_form(lam):
     n = 2_000_000
