@@ -64,13 +64,15 @@ def test_decoder_memory_holds_reference_sync_state(tmp_path):
         st = ol.Stream(units, fb, fs, table, L, n)
         _, _, cnt, _, _ = ol.decode_seq(st, S=S)
         ls, lb = ol.subseq_last(st, S)
-        scan = (np.concatenate([[0], np.cumsum(cnt.astype(np.uint64))[:-1]]) & 0xFFFFFFFF)
+        scan = np.concatenate([np.zeros(1, np.uint64),
+                               np.cumsum(cnt.astype(np.uint64))[:-1]]) & np.uint64(0xFFFFFFFF)
         own = cnt > 0
         assert own.sum() >= nsub - 1
         assert np.array_equal(sizes, scan.astype(np.uint32))
         assert np.array_equal(sync[own, 2], scan[own].astype(np.uint32))
         j = np.nonzero(own)[0]
         assert np.array_equal(sync[own, 3], ls[own])
-        assert np.array_equal(sync[own, 0].astype(np.uint64), (lb[own] >> 5) - j.astype(np.uint64) * S)
-        assert np.array_equal(sync[own, 1].astype(np.uint64), lb[own] & 31)
+        assert np.array_equal(sync[own, 0].astype(np.uint64),
+                              (lb[own] >> np.uint64(5)) - j.astype(np.uint64) * np.uint64(S))
+        assert np.array_equal(sync[own, 1].astype(np.uint64), lb[own] & np.uint64(31))
         assert flags.all()
