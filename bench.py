@@ -5,9 +5,12 @@ sizes log-uniform in [4 KiB, 16 MiB], symbols s = min(255, floor(X)), X ~ Exp(la
 with lambda ~ LogUniform(0.005, 2.0) per stream, each with its own 4096-state table
 normalised from its histogram -- decoded by one batched launch sequence
 (mans_decode_batch).  Beside it: the same streams decoded by per-stream plain launches
-(mans_decode once per stream, in order, one HIP stream), and the single-stream at-scale
-configuration (2^30 symbols for lambda in {0.01, 0.05, 0.2, 1.0}, 4096 states), and the
-served configuration (continuous batching under Poisson arrivals; p50/p99 latency).
+(mans_decode once per stream, in order, one HIP stream), the single-stream at-scale
+configuration (2^30 symbols for lambda in {0.01, 0.05, 0.2, 1.0}, 4096 states), the
+stress configuration (2^32 symbols at lambda 0.002 and 3.0), the CPU-runnable
+configuration (4 MiB, 1024 states, lambda 0.1: GPU beside the reference multicore
+decoder at S = 4 and a tuned S), and the served configuration (continuous batching under
+Poisson arrivals; p50/p99 latency).
 
 Every decoded stream is checked bit-exact against its original symbols before timing.
 Inputs are resident in HBM; timing is HIP events on the decode stream.  A step is one
@@ -18,7 +21,7 @@ on this host.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--streams N] [--no-single]
                     [--no-plain] [--no-cpu-baseline] [--calibrate-copy MIB]
-                    [--serve-seconds S] [--serve-pool N] [--stress]
+                    [--serve-seconds S] [--serve-pool N] [--no-stress] [--no-c1]
 """
 import argparse
 import ctypes as C
@@ -411,37 +414,115 @@ def bench_encode(args, torch, items):
     return out
 
 
-def bench_single(args, torch):
-    """Single-stream at-scale breakdown: 2^log2n symbols per lambda, L = 4096."""
+def _time_single(torch, gb, i, reps):
+    """Decode stream i of a GeneratedBatch: bit-exact check on the device, then the median
+    of `reps` HIP-event timings and a per-kernel breakdown."""
     from multians_amd import _lib
-    from multians_amd.decoder import DeviceStream
+    from multians_amd.decoder import workspace_bytes
+    n = int(gb.n_symbols[i])
+    out = torch.empty(n, dtype=torch.uint8, device="cuda")
+    ws = torch.zeros(workspace_bytes(int(gb.n_units[i]), int(gb.L[i])), dtype=torch.uint8,
+                     device="cuda")
+    gb.decode_one(i, out, ws)
+    torch.cuda.synchronize()
+    ok = gb.status(ws) == 0 and bool(torch.equal(out, gb.symbols(i).flip(0)))
+    if not ok:
+        raise SystemExit("single-stream decode mismatch (stream %s)" % (gb.specs[i],))
+    for _ in range(2):
+        gb.decode_one(i, out, ws)
+    ms = sorted(time_events(torch, lambda: gb.decode_one(i, out, ws), 1) for _ in range(reps))
+    kt = _lib.KernelTimer()
+    gb.decode_one(i, out, ws, kernel_events=kt.ptr)
+    torch.cuda.synchronize()
+    kms = kt.times_ms()
+    kt.close()
+    del out, ws
+    return ms[len(ms) // 2], kms
+
+
+def bench_single(args, torch):
+    """Single-stream at-scale configuration: 2^log2n symbols per lambda, L = 4096; streams
+    made on the device (the host generator's symbols and the host codec's tables and units),
+    median of 10 decodes each."""
+    from multians_amd.decoder import GeneratedBatch
     lams = [0.01, 0.05, 0.2, 1.0]
     n = 1 << args.log2n
-    items = make_streams([(1000 + i, lam, n, 4096) for i, lam in enumerate(lams)])
     per = []
-    for (es, sym), lam in zip(items, lams):
-        ds = DeviceStream(es)
-        out = ds.decode()
-        ref = torch.from_numpy(sym).cuda()
-        ok = bool(torch.equal(out[:n].flip(0), ref)) and ds.status() == 0
-        del ref
-        if not ok:
-            raise SystemExit("single-stream decode mismatch for lambda=%g" % lam)
-        ms = time_events(torch, ds.decode, 5)
-        kt = _lib.KernelTimer()
-        ds.decode(kernel_events=kt.ptr)
-        torch.cuda.synchronize()
-        kms = kt.times_ms()
-        kt.close()
-        algo = 4.0 * es.n_units + n
+    for i, lam in enumerate(lams):
+        gb = GeneratedBatch([(1000 + i, lam, n, 4096)])
+        ms, kms = _time_single(torch, gb, 0, 10)
+        nu = int(gb.n_units[0])
+        algo = 4.0 * nu + n
         per.append({"lambda": lam, "ms": round(ms, 4), "decoded_GBps": round(n / ms / 1e6, 2),
-                    "ratio": round(n / (4.0 * es.n_units), 3),
+                    "ratio": round(n / (4.0 * nu), 3),
                     "hbm_frac": round(algo / ms / 1e6 / HBM_PEAK_GBS, 4), "bit_exact": True,
                     "kernels_ms": {k: round(v, 4) for k, v in kms.items()}})
         log("single", json.dumps(per[-1]))
-        del ds
+        del gb
         torch.cuda.empty_cache()
-    return per, items
+    return per, None
+
+
+def bench_plain_c1(args, torch):
+    """The CPU-runnable configuration: 4 MiB of uint8 symbols, L = 1024, lambda = 0.1, one
+    stream.  GPU decode (median of 20, HIP events) beside the reference MulticoreDecoder
+    (oracle/_ref, else the oracle's restatement) on this host at S = 4 (the demo's) and at
+    the best S of a short sweep, every output checked against the input and against each
+    other (src/main.cc:46-164 prints the same pair per lambda)."""
+    from multians_amd import _lib
+    from multians_amd.decoder import DeviceStream, EncodedStream
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    n, L, lam = 4 << 20, 1024, 0.1
+    units, fb, fs, dec, sym = _lib.make_stream(20261204, lam, n, L, True)
+    es = EncodedStream(units, fb, fs, dec, L, n)
+    ds = DeviceStream(es)
+    out = ds.decode()
+    torch.cuda.synchronize()
+    gpu = out[:n].cpu().numpy()
+    if ds.status() != 0 or not np.array_equal(gpu[::-1], sym):
+        raise SystemExit("C1 GPU decode mismatch")
+    for _ in range(3):
+        ds.decode()
+    gms = sorted(time_events(torch, ds.decode, 1) for _ in range(20))[10]
+    padded = np.concatenate([units, np.zeros(1040, np.uint32)])
+    use_ref = ol.ref_available()
+
+    def multicore(S):
+        o = np.zeros(n, np.uint8)
+        t0 = time.perf_counter()
+        if use_ref:
+            us = ol.ref().ref_multicore(S, THREADS, padded, len(units), dec, L, fs, fb, o, n)
+            dt = us / 1e3
+        else:
+            ol.oracle().mo_decode_multicore(S, THREADS, padded, len(units), dec, L, fs, fb, o, n)
+            dt = (time.perf_counter() - t0) * 1e3
+        return dt, o
+
+    cpu = {}
+    for S in (4, 16, 64, 256, 1024):
+        ts = []
+        for _ in range(5):
+            dt, o = multicore(S)
+            ts.append(dt)
+        diff = np.nonzero(o != gpu)[0]
+        # the reference drops trailing zero-bit symbols when n_units % S == 0 (the last
+        # ones in decode order); any other difference is an error
+        if len(diff) and diff.min() < n - 64:
+            raise SystemExit("C1 multicore (S=%d) differs from the GPU decode" % S)
+        cpu[S] = {"ms": round(sorted(ts)[2], 3), "bit_exact_vs_gpu": len(diff) == 0}
+    best = min(cpu, key=lambda k: cpu[k]["ms"])
+    out = {"symbols": n, "L": L, "lambda": lam, "units": len(units),
+           "ratio": round(n / (4.0 * len(units)), 3), "gpu_ms": round(gms, 4),
+           "gpu_decoded_GBps": round(n / gms / 1e6, 3), "bit_exact_vs_input": True,
+           "cpu_kind": "reference" if use_ref else "port", "cpu_threads": THREADS,
+           "cpu_S4_ms": cpu[4]["ms"], "cpu_best_S": best, "cpu_best_ms": cpu[best]["ms"],
+           "cpu_sweep": {str(k): v for k, v in cpu.items()},
+           "gpu_vs_cpu_S4": round(cpu[4]["ms"] / gms, 2),
+           "gpu_vs_cpu_best": round(cpu[best]["ms"] / gms, 2)}
+    log("C1", json.dumps(out))
+    del ds
+    return out
 
 
 def bench_tables(args, torch):
@@ -486,35 +567,24 @@ def bench_tables(args, torch):
 def bench_stress(args, torch):
     """Stress configuration: one stream of 2^32 uint8 symbols (4 GiB, 64-bit offsets) for
     lambda = 0.002 (symbol 255 holds 60% of the mass) and lambda = 3.0 (ratio > 7), 4096
-    states; bit-exact on the device, then timed."""
-    from multians_amd import _lib
-    from multians_amd.decoder import DeviceStream, EncodedStream
+    states, made on the device; bit-exact on the device, then timed (median of 5)."""
+    from multians_amd.decoder import GeneratedBatch
     n = 1 << 32
     out = []
     for i, lam in enumerate([0.002, 3.0]):
         t0 = time.time()
-        sym = _lib.gen_symbols(4000 + i, lam, n, threads=THREADS)
-        counts = _lib.normalize_counts(_lib.histogram(sym, threads=THREADS), 4096)
-        enc, dec = _lib.build_tables(counts, 4096)
-        units, fb, fs = _lib.encode(sym, enc, 4096)
-        del enc
-        es = EncodedStream(units, fb, fs, dec, 4096, n, counts)
-        log("stress lambda=%g: %d units (ratio %.2f), prepared in %.0fs" %
-            (lam, es.n_units, n / (4.0 * es.n_units), time.time() - t0))
-        ds = DeviceStream(es)
-        o = ds.decode()
+        gb = GeneratedBatch([(4000 + i, lam, n, 4096)])
         torch.cuda.synchronize()
-        ref = torch.from_numpy(sym).cuda()
-        del sym
-        ok = ds.status() == 0 and bool(torch.equal(o[:n], ref.flip(0)))
-        del ref
-        if not ok:
-            raise SystemExit("stress decode mismatch for lambda=%g" % lam)
-        ms = time_events(torch, ds.decode, 3)
-        out.append({"lambda": lam, "symbols": n, "ratio": round(n / (4.0 * es.n_units), 3),
-                    "ms": round(ms, 3), "decoded_GBps": round(n / ms / 1e6, 2), "bit_exact": True})
+        nu = int(gb.n_units[0])
+        log("stress lambda=%g: %d units (ratio %.2f), made on the device in %.1fs" %
+            (lam, nu, n / (4.0 * nu), time.time() - t0))
+        ms, kms = _time_single(torch, gb, 0, 5)
+        out.append({"lambda": lam, "symbols": n, "ratio": round(n / (4.0 * nu), 3),
+                    "ms": round(ms, 3), "decoded_GBps": round(n / ms / 1e6, 2),
+                    "hbm_frac": round((4.0 * nu + n) / ms / 1e6 / HBM_PEAK_GBS, 4),
+                    "bit_exact": True, "kernels_ms": {k: round(v, 4) for k, v in kms.items()}})
         log("stress", json.dumps(out[-1]))
-        del ds, o, es, units
+        del gb
         torch.cuda.empty_cache()
     return out
 
@@ -577,8 +647,9 @@ def bench_served(args, torch):
     return out
 
 
-def cpu_baseline(items, budget_s=20.0):
-    """Reference MulticoreDecoder on a bounded sample (streams of the benchmark)."""
+def cpu_baseline(items, budget_s=20.0, S=4):
+    """Reference MulticoreDecoder on a bounded sample (streams of the benchmark), with
+    subsequence size S (4 = the reference demo's)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
     use_ref = ol.ref_available()
@@ -590,15 +661,15 @@ def cpu_baseline(items, budget_s=20.0):
         # self-synchronisation distance: sample streams with >= 8192 units per thread
         if es.n_units < 8192 * THREADS:
             continue
-        units = np.concatenate([es.units, np.zeros(16, np.uint32)])
+        units = np.concatenate([es.units, np.zeros(S + 16, np.uint32)])
         out = np.zeros(es.n_symbols, np.uint8)
         t0 = time.perf_counter()
         if use_ref:
-            us = ol.ref().ref_multicore(4, THREADS, units, es.n_units, es.table, es.L,
+            us = ol.ref().ref_multicore(S, THREADS, units, es.n_units, es.table, es.L,
                                         es.first_state, es.first_bit, out, es.n_symbols)
             dt = us / 1e6
         else:
-            ol.oracle().mo_decode_multicore(4, THREADS, units, es.n_units, es.table, es.L,
+            ol.oracle().mo_decode_multicore(S, THREADS, units, es.n_units, es.table, es.L,
                                             es.first_state, es.first_bit, out, es.n_symbols)
             dt = time.perf_counter() - t0
         if not np.array_equal(out[::-1], sym):
@@ -616,9 +687,11 @@ def cpu_baseline(items, budget_s=20.0):
             break
     return {"value": round(total_bytes / total_s / 1e9, 4), "unit": "GB/s", "cores": THREADS,
             "kind": "reference" if use_ref else "port",
-            "sample": "MulticoreDecoder with S=4 (the reference demo's) and %d threads on the "
+            "sample": "MulticoreDecoder with S=%d%s and %d threads on the "
                       "first %d streams with >= 8192 units per thread (%.2f GB of "
-                      "symbols) of the benchmark batch, one after another" % (THREADS, nstreams, total_bytes / 1e9),
+                      "symbols) of the benchmark batch, one after another" %
+                      (S, " (the reference demo's)" if S == 4 else "", THREADS, nstreams,
+                       total_bytes / 1e9),
             "host_logical_cpus": os.cpu_count(),
             "streams_with_dropped_zero_bit_tail": dropped_tail}
 
@@ -656,8 +729,10 @@ def main():
                          "per-kernel timing decode does; for per-dispatch profiles")
     ap.add_argument("--tables", type=int, default=0, metavar="N",
                     help="also build decoder tables on the device for the first N streams")
-    ap.add_argument("--stress", action="store_true",
-                    help="also decode two 2^32-symbol streams (lambda 0.002 and 3.0)")
+    ap.add_argument("--no-stress", action="store_true",
+                    help="skip the two 2^32-symbol streams (lambda 0.002 and 3.0)")
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the 4 MiB CPU-runnable configuration (GPU vs multicore)")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the per-stream plain-launch leg")
     ap.add_argument("--no-index", action="store_true",
@@ -731,8 +806,10 @@ def main():
         res["single_stream"] = per
     if args.tables:
         res["device_tables"] = bench_tables(args, torch)
-    if args.stress:
+    if not args.no_stress:
         res["stress"] = bench_stress(args, torch)
+    if not args.no_c1:
+        res["c1_plain"] = bench_plain_c1(args, torch)
     if args.serve_seconds > 0:
         sv = bench_served(args, torch)
         res["served"] = sv
@@ -742,6 +819,12 @@ def main():
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(bitems)
         log("cpu baseline", json.dumps(res["cpu_baseline"]))
+        best = res.get("c1_plain", {}).get("cpu_best_S", 64)
+        if best != 4:
+            tuned = cpu_baseline(bitems, budget_s=10.0, S=best)
+            res["cpu_baseline"]["tuned_S"] = {"S": best, "value": tuned["value"],
+                                              "sample": tuned["sample"]}
+            log("cpu baseline, tuned S", json.dumps(res["cpu_baseline"]["tuned_S"]))
     print(json.dumps(res), flush=True)
 
 
