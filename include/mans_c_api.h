@@ -181,6 +181,18 @@ int mans_workspace_bytes_geometry(uint64_t n_units, uint32_t chunk_units,
  * MANS_F_SYNC_ERRORS is set.  Integrity faults found on the device are reported
  * by mans_decode_status(). */
 int mans_decode(const mans_decode_args* args);
+/* mans_decode without a workspace argument, for callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  args->workspace must be NULL and
+ * args->workspace_bytes 0: the decode uses a static device array of 64 MiB held by the
+ * library on each GPU it runs on (until the process ends).  A decode at the automatic
+ * chunk lengths needs a little over 1 byte per unit, so streams of up to about 50 million
+ * units (200 MB compressed) fit; a stream whose workspace would exceed 64 MiB is rejected
+ * with MANS_ENOMEM before anything is enqueued.
+ * Allocates nothing.  Every call on one GPU shares that GPU's array, so calls that overlap
+ * on one GPU (other streams, other threads) race; serialise them.  Device-side errors
+ * are reported only with MANS_F_SYNC_ERRORS (synchronises args->stream). */
+int mans_decode_noalloc(const mans_decode_args* args);
 /* Fills args->ref_sync from the workspace of the last decode enqueued with these same
  * arguments (same stream, sizes, geometry and workspace, not reused since), enqueued on
  * args->stream, without decoding again: for a caller that inspects the reference decoder's

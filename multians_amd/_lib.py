@@ -119,7 +119,7 @@ EXPORTS = [
     "mans_table_workspace_bytes", "mans_build_tables_device", "mans_table_status",
     "mans_sync_point_count", "mans_stream_range", "mans_sync_points_check",
     "mans_encode_workspace_bytes", "mans_encode_device", "mans_encode_results",
-    "mans_gen_symbols_device", "mans_ref_sync_fill",
+    "mans_gen_symbols_device", "mans_ref_sync_fill", "mans_decode_noalloc",
 ]
 
 _lib = None
@@ -185,6 +185,7 @@ def lib():
         "mans_set_wait_polls": ([u32], i32),
         "mans_gen_symbols_device": ([u64, C.c_double, C.c_void_p, u64, C.c_void_p], i32),
         "mans_ref_sync_fill": ([C.POINTER(DecodeArgs)], i32),
+        "mans_decode_noalloc": ([C.POINTER(DecodeArgs)], i32),
         "mans_set_sync_round_limit": ([u32], i32),
         "mans_table_workspace_bytes": ([C.c_size_t, C.POINTER(u64)], i32),
         "mans_build_tables_device": ([C.POINTER(TableJob), C.c_size_t, vp, u64, vp], i32),

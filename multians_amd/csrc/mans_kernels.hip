@@ -1521,6 +1521,82 @@ extern "C" int mans_decode(const mans_decode_args* d) {
     return MANS_OK;
 }
 
+// ------------------------------------------------------------ no-allocation decode
+// mans_decode_noalloc: mans_decode with the workspace taken from a static device array
+// instead of the caller's, for callers that cannot allocate.  The array (`scratch`, one
+// copy per GPU, held until the process ends) is addressed from the host only through the
+// pointer hipGetSymbolAddress returns for the current device (kept per device); the
+// symbol itself is a host-side stand-in and is never passed to a kernel.  Every call on
+// a GPU shares that GPU's array: calls that overlap on one GPU race.  k_init clears what
+// the kernels expect cleared (status words, tickets, fixup counters) at the start of each
+// call, as it does for a caller's workspace.
+constexpr uint64_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::vector<void*> g_scratch_ptr;    // per device, nullptr until looked up
+
+static int scratch_ptr(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return set_error(MANS_EHIP, "get device: %s", hipGetErrorString(e));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((size_t)dev >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(MANS_EHIP, "scratch address on device %d: %s", dev,
+                             hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return MANS_OK;
+}
+
+extern "C" int mans_decode_noalloc(const mans_decode_args* d) {
+    if (!d) return set_error(MANS_EINVAL, "args is NULL");
+    if (d->workspace || d->workspace_bytes)
+        return set_error(MANS_EINVAL, "mans_decode_noalloc uses its own scratch: pass no "
+                                      "workspace");
+    if (d->n_out == 0)
+        return check_stream_args(d->units, d->n_units, d->out, d->n_out, d->table, d->L,
+                                 d->first_state, d->first_bit);
+    mans_decode_args a = *d;
+    // the size this stream's decode needs (the same layout mans_decode checks)
+    a.workspace = (void*)(uintptr_t)256u;
+    a.workspace_bytes = ~0ull;
+    KArgs k;
+    uint32_t nct, nit;
+    int rc = single_plan(&a, &k, &nct, &nit);
+    if (rc) return rc;
+    uint64_t need = 0;
+    {
+        const StreamPlan& sp = k.one;
+        need = make_layout(sp.n_chunks, ck_slots(sp.n_chunks, sp.S, sp.K), 1).bytes;
+    }
+    if (need > kScratchBytes)
+        return set_error(MANS_ENOMEM, "mans_decode_noalloc: this stream needs %llu bytes of "
+                                      "scratch, the static scratch holds %llu (use mans_decode "
+                                      "with a workspace)",
+                         (unsigned long long)need, (unsigned long long)kScratchBytes);
+    rc = scratch_ptr(&a.workspace);
+    if (rc) return rc;
+    a.workspace_bytes = kScratchBytes;
+    rc = single_plan(&a, &k, &nct, &nit);
+    if (rc) return rc;
+    rc = launch_pipeline(k, nct, nit, k.one.n_chunks, k.one.L, k.one.K, (hipStream_t)d->stream,
+                         d->kernel_events);
+    if (rc) return rc;
+    if (d->flags & MANS_F_SYNC_ERRORS) {
+        int status = 0;
+        rc = mans_decode_status(a.workspace, d->stream, &status);
+        if (rc) return rc;
+        if (status) return set_error(status, "device reported status %d", status);
+    }
+    return MANS_OK;
+}
+
 extern "C" int mans_ref_sync_fill(const mans_decode_args* d) {
     if (!d || !d->ref_sync) return set_error(MANS_EINVAL, "args or ref_sync is NULL");
     if (d->n_out == 0) return MANS_OK;
