@@ -4,6 +4,7 @@
 // order the kernels of mans_kernels.hip run them, so the algorithm (spec, chained
 // walks, consistency check, fixup rounds, scan, count-driven write) can be checked
 // against the oracle without a GPU.  The product never loads this library.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -206,6 +207,109 @@ int mans_emu_decode_limited(const uint32_t* units, uint64_t n_units, uint8_t* ou
                             uint32_t round_limit, uint64_t* stats) {
     return emu_decode(units, n_units, out, n_out, table, L, first_state, first_bit, S, K, 0u,
                       stats, nullptr, nullptr, nullptr, nullptr, round_limit);
+}
+
+// Zero-bit run folding of the sync kernels' tables (fold_zero_bit_runs in
+// mans_kernels.hip), checked on the CPU: the in-place pointer jumping the kernel runs
+// (here sequentially, in thread-stride order) must give the words that following each
+// zero-bit run one step at a time gives, and decoding with the folded words (one lookup =
+// 1 + z steps at one bit position) must meet every unit boundary in the same state, at
+// the same offset, after the same number of steps as decoding one step at a time.
+// Returns 0, or a negative code naming the first difference; stats (4 u64): zero-bit
+// states, longest run, lookups of the folded decode, steps.
+int mans_emu_fold_check(const uint32_t* units, uint64_t n_units, const uint32_t* table,
+                        uint32_t L, uint32_t first_state, uint32_t first_bit, uint64_t n_out,
+                        uint64_t* stats) {
+    const uint32_t kMax = 255u;
+    std::vector<uint32_t> w(L), P(L), ref(L), fold(L);
+    int bad = 0, cyc = 0;
+    for (uint32_t i = 0; i < L; ++i) w[i] = derive_entry(table[i], L, i, &bad, &cyc) & ~0xFF00u;
+    if (bad || cyc) return -4;
+    // pointer jumping (next | z << 16), thread-stride order, in place, until no change
+    uint64_t zb = 0;
+    for (uint32_t i = 0; i < L; ++i) {
+        const bool z = (w[i] & 31u) == 0u;
+        P[i] = z ? ((w[i] >> 16) | (1u << 16)) : i;
+        zb += z;
+    }
+    for (bool changed = true; changed;) {
+        changed = false;
+        for (uint32_t t = 0; t < 256u; ++t)
+            for (uint32_t i = t; i < L; i += 256u) {
+                const uint32_t p = P[i], z = p >> 16;
+                if (!z) continue;
+                const uint32_t q = P[p & 0xFFFFu], zq = q >> 16;
+                if (!zq || z + zq > kMax) continue;
+                P[i] = (q & 0xFFFFu) | ((z + zq) << 16);
+                changed = true;
+            }
+    }
+    uint64_t longest = 0;
+    for (uint32_t i = 0; i < L; ++i) {
+        const uint32_t z = P[i] >> 16;
+        fold[i] = z ? (w[P[i] & 0xFFFFu] | (z << 8)) : w[i];     // symbol bytes are 0
+        // reference: follow the run one step at a time (at most kMax steps)
+        uint32_t j = i, n = 0;
+        while ((w[j] & 31u) == 0u && n < kMax) { j = w[j] >> 16; ++n; }
+        if (n == kMax && (w[j] & 31u) == 0u) {
+            // capped: pointer jumping stops at some run length <= kMax, which is valid
+            // too; check that its word describes a real position on this run
+            uint32_t k = i, m = 0;
+            while (m < z && (w[k] & 31u) == 0u) { k = w[k] >> 16; ++m; }
+            if (m != z || (fold[i] & ~0xFF00u) != w[k]) return -10;
+        } else {
+            ref[i] = n ? (w[j] | (n << 8)) : w[i];
+            if (fold[i] != ref[i]) return -11;
+        }
+        longest = std::max<uint64_t>(longest, z);
+    }
+    // decode both ways and compare at every unit boundary
+    auto bit = [&](uint64_t q) -> uint32_t {
+        const uint64_t u = q >> 5;
+        return u < n_units ? (units[u] >> (q & 31u)) & 1u : 0u;
+    };
+    auto bits = [&](uint64_t q, uint32_t k) {
+        uint32_t v = 0;
+        for (uint32_t i = 0; i < k; ++i) v |= bit(q + i) << i;
+        return v;
+    };
+    std::vector<uint64_t> e1, c1, e2, c2;
+    {   // plain
+        uint64_t p = 32u - first_bit, cnt = 0, next_u = 0;
+        uint32_t xi = first_state - L;
+        while (cnt < n_out) {
+            while (next_u <= n_units && 32ull * next_u <= p) {
+                e1.push_back(xi | ((p - 32ull * next_u) << 16));
+                c1.push_back(cnt);
+                ++next_u;
+            }
+            const uint32_t d = w[xi];
+            xi = (d >> 16) + bits(p, d & 31u);
+            p += d & 31u;
+            ++cnt;
+        }
+    }
+    uint64_t lookups = 0;
+    {   // folded
+        uint64_t p = 32u - first_bit, cnt = 0, next_u = 0;
+        uint32_t xi = first_state - L;
+        while (cnt < n_out) {
+            while (next_u <= n_units && 32ull * next_u <= p) {
+                e2.push_back(xi | ((p - 32ull * next_u) << 16));
+                c2.push_back(cnt);
+                ++next_u;
+            }
+            const uint32_t d = fold[xi];
+            xi = (d >> 16) + bits(p, d & 31u);
+            p += d & 31u;
+            cnt += 1u + ((d >> 8) & 0xFFu);
+            ++lookups;
+        }
+    }
+    if (e1 != e2) return -12;
+    if (c1 != c2) return -13;
+    if (stats) { stats[0] = zb; stats[1] = longest; stats[2] = lookups; stats[3] = n_out; }
+    return 0;
 }
 
 // The same from recorded sync points (index_in, n_chunks records, or NULL) and/or

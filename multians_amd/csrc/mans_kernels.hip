@@ -119,7 +119,10 @@ __device__ uint32_t stream_of_chunk(const KArgs& a, uint32_t gc) {
 // the table has zero-bit cycles (see derive_entry), bit 1 when it has rejected entries
 // (they stage as zero, a zero-bit self-loop a round would never leave, so nothing of
 // that stream is decoded); rejected tables set status[0] bit 0.
-__device__ uint32_t stage_table(const StreamPlan& sp, uint32_t* lds, uint32_t* status) {
+// With count_only, the symbol byte (bits 8-15) is staged as zero: the sync kernels keep
+// the length of a folded zero-bit run there instead (fold_zero_bit_runs).
+__device__ uint32_t stage_table(const StreamPlan& sp, uint32_t* lds, uint32_t* status,
+                                bool count_only = false) {
     __shared__ uint32_t s_flags;
     if (threadIdx.x == 0) s_flags = 0u;
     __syncthreads();
@@ -128,7 +131,8 @@ __device__ uint32_t stage_table(const StreamPlan& sp, uint32_t* lds, uint32_t* s
     const uint32_t tb = lds_addr(lds) >> 2;
     for (uint32_t i = threadIdx.x; i < sp.L; i += blockDim.x) {
         uint32_t e = load_unit(sp.table, sp.L, i);
-        lds[i] = derive_entry(e, sp.L, i, &bad, &cyc) + (tb << 16);
+        const uint32_t d = derive_entry(e, sp.L, i, &bad, &cyc);
+        lds[i] = (count_only ? (d & ~0xFF00u) : d) + (tb << 16);
     }
     if (bad) {
         atomicOr(&status[0], 1u);
@@ -143,6 +147,69 @@ __device__ uint32_t stage_table(const StreamPlan& sp, uint32_t* lds, uint32_t* s
         if (sp.status_out) atomicOr(sp.status_out, 1u);
     }
     return __builtin_amdgcn_readfirstlane(f);
+}
+
+// Zero-bit runs folded into the staged table, for the kernels that count steps without
+// emitting symbols (k_sync, k_fixup).  A zero-bit step reads no bits, so from a state
+// whose step reads none the next states are fixed until one whose step reads bits: the
+// word of state i becomes that state j's word, with the number z of zero-bit steps from i
+// to j (at most kMaxRun) in bits 8-15, where the emitted symbol was.  One lookup then
+// takes 1 + z steps, all starting at the same bit; a lane's state is still the state
+// before its next step, so checkpoints (the entry of a unit boundary) are unchanged.
+// The runs come from pointer jumping in LDS over `scratch` (L words: next | z << 16;
+// each word is read and written whole, so jumps taken in any order stay consistent):
+// about log2(run length) passes.  Tables with zero-bit cycles (flag bit 0) or rejected
+// entries keep their plain words; so do tables larger than `cap_words` (the scratch) or
+// 4096 states.  The table must be staged count_only (symbol bytes zero).
+constexpr uint32_t kMaxRun = 255;
+
+__device__ void fold_zero_bit_runs(uint32_t* lds, uint32_t L, uint32_t* scratch,
+                                   uint32_t cap_words) {
+    if (L > cap_words || L > 4096u) return;         // uniform
+    const uint32_t tb = lds_addr(lds) >> 2;
+    uint32_t any = 0u;
+    for (uint32_t i = threadIdx.x; i < L; i += blockDim.x) {
+        const uint32_t w = lds[i];
+        const uint32_t zb = (w & 31u) == 0u ? 1u : 0u;
+        scratch[i] = zb ? (((w >> 16) - tb) | (1u << 16)) : i;
+        any |= zb;
+    }
+    if (!__syncthreads_or(any)) return;
+    for (;;) {
+        uint32_t changed = 0u;
+        for (uint32_t i = threadIdx.x; i < L; i += blockDim.x) {
+            const uint32_t p = scratch[i];
+            const uint32_t z = p >> 16;
+            if (!z) continue;                       // a state whose step reads bits
+            const uint32_t q = scratch[p & 0xFFFFu];
+            const uint32_t zq = q >> 16;
+            if (!zq || z + zq > kMaxRun) continue;  // resolved, or at the run cap
+            scratch[i] = (q & 0xFFFFu) | ((z + zq) << 16);
+            changed = 1u;
+        }
+        if (!__syncthreads_or(changed)) break;
+    }
+    // the folded words (read all, barrier, write all: lds[] is read by other threads);
+    // L <= 4096 here, so each of the kWG threads holds at most kPer of them
+    constexpr uint32_t kPer = 4096u / kWG;
+    uint32_t nw[kPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        const uint32_t i = threadIdx.x + k * kWG;
+        nw[k] = 0u;
+        if (i < L) {
+            const uint32_t p = scratch[i];
+            const uint32_t z = p >> 16;
+            nw[k] = z ? (lds[p & 0xFFFFu] | (z << 8)) : lds[i];   // symbol bytes are 0
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        const uint32_t i = threadIdx.x + k * kWG;
+        if (i < L) lds[i] = nw[k];
+    }
+    __syncthreads();
 }
 
 // Clears the status words, tickets and fixup counters, and marks the last exit of each
@@ -248,22 +315,25 @@ __device__ __forceinline__ void round_decode(const LdsTab& tab, const uint32_t* 
     lds_window_load(p, lo, hi);
     uint32_t i = L.xi + tab.tb;
     uint32_t w = lds_word(i << 2);
-    uint32_t cnt = L.cnt, i1, p1;
+    uint32_t cnt = L.cnt, i1, p1, z2;
     do {
         const uint32_t win = funnel_shr(hi, lo, p);
         i1 = (w >> 16) + low_bits(win, w);
         const uint32_t w1 = lds_word(i1 << 2);
         p1 = p + (w & 0xFFu);
         i = (w1 >> 16) + __builtin_amdgcn_ubfe(win, w, w1);   // offset nb1, width nb2
+        // each lookup takes 1 + z steps (z: folded zero-bit run, bits 8-15)
+        cnt += 2u + __builtin_amdgcn_ubfe(w, 8u, 8u);
+        z2 = __builtin_amdgcn_ubfe(w1, 8u, 8u);
+        cnt += z2;
         w = lds_word(i << 2);
         p = p1 + (w1 & 0xFFu);
         lds_window_load(p, lo, hi);
-        cnt += 2u;
     } while (p < end);
-    if (p1 >= end) {                        // the round ended after the first step
+    if (p1 >= end) {                        // the round ended after the first lookup
         i = i1;
         p = p1;
-        --cnt;
+        cnt -= 1u + z2;
     }
     p -= pb0;
     L.u = rs + (p >> 5); L.o = p & 31u; L.xi = i - tab.tb; L.cnt = cnt;
@@ -536,8 +606,9 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
     __syncthreads();
     const uint32_t t = s_task;
     const StreamPlan sp = *plan_of(a, __builtin_amdgcn_readfirstlane(s_stream));
-    const uint32_t sf = stage_table(sp, lds, a.w.status);
+    const uint32_t sf = stage_table(sp, lds, a.w.status, true);
     const uint32_t cyc = sf & 1u;
+    if (!sf) fold_zero_bit_runs(lds, sp.L, lds + table_words(sp.L), kWG * kLS);
     LdsTab tab{lds_addr(lds) >> 2};
     RoundIO io{lds + table_words(sp.L), s_rbase, sp.units, sp.n_units};
     const Work& w = a.w;
@@ -702,7 +773,8 @@ __global__ __launch_bounds__(kWG) void k_fixup(KArgs a) {
         __syncthreads();
         if (s_n == 0u) continue;
         const StreamPlan sp = *plan_of(a, sid);
-        if (stage_table(sp, lds, w.status)) continue;
+        if (stage_table(sp, lds, w.status, true)) continue;
+        fold_zero_bit_runs(lds, sp.L, lds + table_words(sp.L), kWG * kLS);
         LdsTab tab{lds_addr(lds) >> 2};
         RoundIO io{lds + table_words(sp.L), s_rbase, sp.units, sp.n_units};
         uint32_t* list = w.flist + sp.chunk_base;

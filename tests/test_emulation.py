@@ -159,3 +159,22 @@ def test_fixup_round_bound(emu):
     assert run(rounds - 1)[0] == -5
     rc, out, _ = run(rounds)
     assert rc == 0 and np.array_equal(out[::-1], sym)
+
+
+@pytest.mark.parametrize("lam,L", [(0.5, 4096), (1.0, 4096), (2.0, 1024), (3.0, 4096),
+                                   (6.0, 4096), (1.0, 64)])
+def test_zero_bit_run_folding(emu, lam, L):
+    """The sync kernels fold zero-bit runs into their staged tables (one lookup = 1 + z
+    steps).  The kernel's in-place pointer jumping, run here sequentially, equals following
+    each run step by step, and decoding with the folded words meets every unit boundary in
+    the same state, at the same bit, after the same number of steps."""
+    emu.mans_emu_fold_check.argtypes = [_u32p, C.c_uint64, _u32p, C.c_uint32, C.c_uint32,
+                                        C.c_uint32, C.c_uint64, _u64p]
+    sym = ol.gen_symbols(8, lam, 300_000)
+    st, _ = ol.encode(sym, L=L)
+    stats = np.zeros(4, np.uint64)
+    rc = emu.mans_emu_fold_check(st.units, len(st.units), st.table, st.L, st.first_state,
+                                 st.first_bit, len(sym), stats)
+    assert rc == 0
+    if lam >= 1.0:
+        assert stats[0] > 0 and stats[2] < stats[3]     # runs exist and are folded
