@@ -276,16 +276,25 @@ int mans_graph_launch(void* graph, void* stream);
 int mans_graph_destroy(void* graph);
 
 /* Chunk-length rules (the autotuner's output; no reference counterpart: the reference
- * takes one subsequence size from its caller).  A stream uses the first rule with
- * L <= L_max, symbols per unit (n_out / n_units) <= spu_max and n_units <= units_max;
+ * takes one subsequence size from its caller).  A stream uses the first rule of its scope
+ * with L <= L_max, symbols per unit (n_out / n_units) <= spu_max and n_units <= units_max;
  * streams no rule matches use the built-in choice.  chunk_units is rounded down to a
- * power of two and must be >= 128.  n == 0 restores the built-in choice.  The rules
- * are process-wide; set them before decoding. */
+ * power of two and must be >= 32.  warm_units: units each chunk's speculative decode
+ * starts early (rounded down to the 16-unit checkpoint stride), or MANS_WARM_AUTO for the
+ * built-in choice.  scope: MANS_RULE_SINGLE (mans_decode), MANS_RULE_BATCH
+ * (mans_decode_batch and the chunk length mans_chunk_units reports), or both (0).
+ * n == 0 restores the built-in choice.  The rules are process-wide; set them before
+ * decoding. */
+#define MANS_WARM_AUTO 0xFFFFFFFFu
+#define MANS_RULE_SINGLE 1u
+#define MANS_RULE_BATCH 2u
 typedef struct mans_geometry_rule {
     uint32_t L_max;
     uint32_t chunk_units;
     double spu_max;
     uint64_t units_max;
+    uint32_t warm_units;
+    uint32_t scope;
 } mans_geometry_rule;
 int mans_set_geometry_rules(const mans_geometry_rule* rules, size_t n);
 /* Polls of the sync kernel's wait for the previous range's last exit (a range of 256

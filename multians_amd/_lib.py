@@ -94,7 +94,12 @@ class EncodeResult(C.Structure):
 
 class GeometryRule(C.Structure):
     _fields_ = [("L_max", C.c_uint32), ("chunk_units", C.c_uint32), ("spu_max", C.c_double),
-                ("units_max", C.c_uint64)]
+                ("units_max", C.c_uint64), ("warm_units", C.c_uint32), ("scope", C.c_uint32)]
+
+
+WARM_AUTO = 0xFFFFFFFF
+RULE_SINGLE = 1
+RULE_BATCH = 2
 
 
 class BatchStats(C.Structure):
@@ -346,12 +351,16 @@ class KernelTimer:
 
 
 def set_geometry_rules(rules):
-    """rules: iterable of (L_max, spu_max, units_max, chunk_units); [] = built-in choice."""
+    """rules: iterable of (L_max, spu_max, units_max, chunk_units[, warm_units[, scope]]);
+    warm_units defaults to WARM_AUTO, scope to 0 (single and batch); [] = built-in."""
     rules = list(rules)
     arr = (GeometryRule * max(len(rules), 1))()
-    for i, (L_max, spu_max, units_max, chunk) in enumerate(rules):
+    for i, r in enumerate(rules):
+        L_max, spu_max, units_max, chunk = r[:4]
         arr[i].L_max, arr[i].spu_max = int(L_max), float(spu_max)
         arr[i].units_max, arr[i].chunk_units = int(units_max), int(chunk)
+        arr[i].warm_units = int(r[4]) if len(r) > 4 else WARM_AUTO
+        arr[i].scope = int(r[5]) if len(r) > 5 else 0
     check(lib().mans_set_geometry_rules(arr, len(rules)))
 
 

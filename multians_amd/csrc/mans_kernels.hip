@@ -1241,32 +1241,43 @@ static std::vector<mans_geometry_rule> g_rules;
 // a tuning run has filled it.
 static const mans_geometry_rule kTunedRules[] = {
 #include "tuned_rules.inc"
-    {0u, 0u, 0.0, 0ull}};
+    {0u, 0u, 0.0, 0ull, 0u, 0u}};
 
-static uint32_t rule_chunk(uint64_t n_units, uint64_t n_out, uint32_t L) {
+// Smallest chunk length the rules may choose (the workspace bound of mans_workspace_bytes
+// assumes it).
+constexpr uint32_t kMinRuleChunk = 32;
+
+struct RuleHit {
+    uint32_t S;     // 0 = no rule
+    uint32_t W;     // MANS_WARM_AUTO = built-in warm-up
+};
+
+static RuleHit rule_geometry(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t scope) {
     const double spu = n_units ? (double)n_out / (double)n_units : 4.0;
+    auto match = [&](const mans_geometry_rule& r) {
+        return r.chunk_units && L <= r.L_max && spu <= r.spu_max && n_units <= r.units_max &&
+               (r.scope == 0u || (r.scope & scope));
+    };
     {
         std::lock_guard<std::mutex> lk(g_rules_mu);
         for (const mans_geometry_rule& r : g_rules)
-            if (L <= r.L_max && spu <= r.spu_max && n_units <= r.units_max) return r.chunk_units;
-        if (!g_rules.empty()) return 0u;
+            if (match(r)) return {r.chunk_units, r.warm_units};
+        if (!g_rules.empty()) return {0u, MANS_WARM_AUTO};
     }
     for (const mans_geometry_rule& r : kTunedRules)
-        if (r.chunk_units && L <= r.L_max && spu <= r.spu_max && n_units <= r.units_max)
-            return r.chunk_units;
-    return 0u;
+        if (match(r)) return {r.chunk_units, r.warm_units};
+    return {0u, MANS_WARM_AUTO};
 }
 
-// Spec warm-up (units decoded before each chunk, a multiple of K).  A stream of one
-// 256-chunk range gets 512 units: its walks then mostly merge at the chunk start, and the
-// range's walk phase (which lasts as long as its longest walk) shrinks more than the
-// spec pass grows (10,000-stream mix: 43.4 -> 40.6 ms; 128 / 256 / 384 / 768 units and
-// S/2 .. S/8 measured worse).  Larger streams keep none: on 2^30-symbol streams a fixed
-// warm-up measured neutral to 10% slower.
-static uint32_t choose_warm(uint64_t n_units, uint64_t n_out, uint32_t S, uint32_t K,
-                            bool index_in) {
-    (void)n_out;
+// Spec warm-up (units decoded before each chunk, a multiple of K).  Built-in choice (no
+// rule gives one): a stream of one 256-chunk range gets 512 units, larger ones none.  In
+// a stream of one range the walks then mostly merge at the chunk start, and the range's
+// walk phase (which lasts as long as its longest walk) shrinks more than the spec pass
+// grows (10,000-stream mix: 43.4 -> 40.6 ms).  The autotuner's rules set it per bucket.
+static uint32_t choose_warm(uint64_t n_units, uint32_t S, uint32_t K, bool index_in,
+                            uint32_t rule_W) {
     if (index_in) return 0u;                  // the specs start at recorded sync points
+    if (rule_W != MANS_WARM_AUTO) return std::min<uint32_t>(rule_W, 1u << 16) / K * K;
     const uint64_t chunks = (n_units + S - 1u) / S;
     return chunks <= 256u ? (512u / K) * K : 0u;
 }
@@ -1275,15 +1286,24 @@ static uint32_t choose_warm(uint64_t n_units, uint64_t n_out, uint32_t S, uint32
 // length they were recorded with, as given (one record per chunk of that length).
 static int stream_geometry(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t chunk_units,
                            uint32_t checkpoint_units, const void* index_in,
-                           const void* index_out, uint32_t* S, uint32_t* K);
+                           const void* index_out, uint32_t scope, uint32_t* S, uint32_t* K,
+                           uint32_t* W);
 
+// scope: MANS_RULE_SINGLE or MANS_RULE_BATCH (which rules apply); *W: the rule's warm-up
+// (MANS_WARM_AUTO when none), may be NULL.
 void choose_geometry(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t req_S,
-                     uint32_t req_K, uint32_t* S, uint32_t* K) {
+                     uint32_t req_K, uint32_t* S, uint32_t* K, uint32_t scope = MANS_RULE_BATCH,
+                     uint32_t* W = nullptr) {
     // checkpoints sit at round boundaries of the sync kernels: K is the round length
     (void)req_K;
     const uint32_t k = kR;
     uint32_t s = req_S;
-    if (!s) s = rule_chunk(n_units, n_out, L);
+    uint32_t w = MANS_WARM_AUTO;
+    if (!s) {
+        const RuleHit h = rule_geometry(n_units, n_out, L, scope);
+        s = h.S;
+        if (s) w = h.W;
+    }
     if (!s) {
         // s_safe: long against the distance over which a guessed-state decode merges
         // with the true path, so that walks rarely fail.  Measured at L = 4096 (units,
@@ -1309,13 +1329,16 @@ void choose_geometry(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t req_
     s = 1u << ilog2(s < k ? k : s);
     *S = s;
     *K = k;
+    if (W) *W = w;
 }
 
 static int stream_geometry(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t chunk_units,
                            uint32_t checkpoint_units, const void* index_in,
-                           const void* index_out, uint32_t* S, uint32_t* K) {
+                           const void* index_out, uint32_t scope, uint32_t* S, uint32_t* K,
+                           uint32_t* W) {
+    *W = MANS_WARM_AUTO;
     if (!index_in && !index_out) {
-        choose_geometry(n_units, n_out, L, chunk_units, checkpoint_units, S, K);
+        choose_geometry(n_units, n_out, L, chunk_units, checkpoint_units, S, K, scope, W);
         return MANS_OK;
     }
     if (chunk_units < kR || chunk_units > (1u << 24) || (chunk_units & (chunk_units - 1u)))
@@ -1476,8 +1499,9 @@ static int check_stream_args(const uint32_t* units, uint64_t n_units, const uint
 extern "C" int mans_workspace_bytes(uint64_t n_units, uint32_t L, uint64_t* bytes) {
     (void)L;
     if (!bytes) return set_error(MANS_EINVAL, "bytes is NULL");
-    // smallest geometry the automatic choice uses: S = 128; K (the round length) >= 8
-    uint64_t chunks = n_units / 128u + 2u, cks = n_units / 8u + 2u;
+    // smallest geometry the automatic choice uses: S = kMinRuleChunk; K (the round
+    // length) >= 8
+    uint64_t chunks = n_units / kMinRuleChunk + 2u, cks = n_units / 8u + 2u;
     *bytes = make_layout(chunks, cks, 1).bytes;
     return MANS_OK;
 }
@@ -1506,9 +1530,9 @@ static int single_plan(const mans_decode_args* d, KArgs* out, uint32_t* nct, uin
     int rc = check_stream_args(d->units, d->n_units, d->out, d->n_out, d->table, d->L,
                                d->first_state, d->first_bit);
     if (rc) return rc;
-    uint32_t S, K;
+    uint32_t S, K, W;
     rc = stream_geometry(d->n_units, d->n_out, d->L, d->chunk_units, d->checkpoint_units,
-                         d->index_in, d->index_out, &S, &K);
+                         d->index_in, d->index_out, MANS_RULE_SINGLE, &S, &K, &W);
     if (rc) return rc;
     if (K < 8u && !d->checkpoint_units) K = 8u;
     StreamPlan sp;
@@ -1524,7 +1548,7 @@ static int single_plan(const mans_decode_args* d, KArgs* out, uint32_t* nct, uin
     sp.S = S;
     sp.K = K;
     sp.log2K = ilog2(K);
-    sp.warm = choose_warm(d->n_units, d->n_out, S, K, d->index_in != nullptr);
+    sp.warm = choose_warm(d->n_units, S, K, d->index_in != nullptr, W);
     sp.index_in = (const SyncPoint*)d->index_in;
     sp.index_out = (SyncPoint*)d->index_out;
     sp.items_out = (SyncPoint*)d->items_out;
@@ -1714,9 +1738,9 @@ static int build_plans(const mans_stream_desc* d, size_t n, std::vector<StreamPl
         if (rc) return set_error(rc, "stream %zu: %s", i, mans_last_error());
         StreamPlan& sp = plans[i];
         std::memset(&sp, 0, sizeof(sp));
-        uint32_t S, K;
+        uint32_t S, K, W;
         rc = stream_geometry(d[i].n_units, d[i].n_out, d[i].L, d[i].chunk_units, 0,
-                             d[i].index_in, d[i].index_out, &S, &K);
+                             d[i].index_in, d[i].index_out, MANS_RULE_BATCH, &S, &K, &W);
         if (rc) return set_error(rc, "stream %zu: %s", i, mans_last_error());
         if (K < 8u) K = 8u;
         sp.units = d[i].units;
@@ -1729,7 +1753,7 @@ static int build_plans(const mans_stream_desc* d, size_t n, std::vector<StreamPl
         sp.first_bit = d[i].first_bit;
         sp.S = S;
         sp.K = K;
-        sp.warm = choose_warm(d[i].n_units, d[i].n_out, S, K, d[i].index_in != nullptr);
+        sp.warm = choose_warm(d[i].n_units, S, K, d[i].index_in != nullptr, W);
         sp.index_in = (const SyncPoint*)d[i].index_in;
         sp.index_out = (SyncPoint*)d[i].index_out;
         sp.items_out = (SyncPoint*)d[i].items_out;
@@ -2214,10 +2238,12 @@ extern "C" int mans_set_geometry_rules(const mans_geometry_rule* rules, size_t n
     std::vector<mans_geometry_rule> v;
     for (size_t i = 0; i < n; ++i) {
         mans_geometry_rule r = rules[i];
-        if (r.chunk_units < 128u || r.chunk_units > (1u << 24))
-            return set_error(MANS_EINVAL, "rule %zu: chunk_units %u outside [128, 2^24]", i,
-                             r.chunk_units);
-        uint32_t p = 128u;
+        if (r.chunk_units < kMinRuleChunk || r.chunk_units > (1u << 24))
+            return set_error(MANS_EINVAL, "rule %zu: chunk_units %u outside [%u, 2^24]", i,
+                             r.chunk_units, kMinRuleChunk);
+        if (r.scope > (MANS_RULE_SINGLE | MANS_RULE_BATCH))
+            return set_error(MANS_EINVAL, "rule %zu: scope %u", i, r.scope);
+        uint32_t p = kMinRuleChunk;
         while (p * 2u <= r.chunk_units) p *= 2u;
         r.chunk_units = p;
         v.push_back(r);

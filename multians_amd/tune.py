@@ -1,30 +1,32 @@
-"""Autotuner for the sync-chunk length S (the decoder's one free geometry knob).
+"""Autotuner for the decoder's geometry: the sync-chunk length S and the speculative
+warm-up W (units each chunk's guessed-state decode starts early), per stream shape.
 
     python -m multians_amd.tune [--streams 10000] [--out DIR]
 
 S trades lanes against failed walks: a short chunk gives a stream more lanes (lower
-latency) but more walks that cross their chunk without merging (more chained and
-re-walked work).  The built-in choice is
+latency) but more walks that cross their chunk without merging.  W trades work against
+walk length: a warm-up longer than the merge distance puts most chunks' guessed paths on
+the true path at the chunk start, so their walks merge at once.  Two rule sets are fitted
+on the device, both keyed by symbols-per-unit bucket (n_symbols / n_units, the entropy
+proxy the decoder knows before decoding) and stream size:
 
-    S = clamp(pow2ceil(n_units / 256), floor_b, safe_b)
+  * single (mans_decode, one stream on the GPU: latency): for one stream per (bucket,
+    size) -- symbols 2^20 .. 2^30, lambda chosen to land in each bucket -- every (S, W) of
+    a grid is timed (median of 3 decodes) and the fastest kept;
+  * batch (mans_decode_batch: throughput of the whole batch): the built-in shape
+    S = clamp(pow2ceil(n_units / 256), floor_b, safe_b), with warm-ups W1_b for streams
+    of one 256-chunk range and W2_b for longer ones, fitted by coordinate descent on the
+    decode time of the benchmark's mix (uint8 sizes log-uniform 4 KiB-16 MiB, lambda ~
+    LogU(0.005, 2.0), 4096 states; other seeds than the benchmark's) as one batch.
 
-per symbols-per-unit bucket b (symbols per unit = n_symbols / n_units, the entropy
-proxy the decoder knows before decoding), with the chunk count of one full 256-lane
-workgroup as the fill target.  The tuner fits (floor_b, safe_b) by coordinate descent
-on measured decode time of a workload:
-
-  * mixed: the batched benchmark mix (uint8 sizes log-uniform 4 KiB-16 MiB,
-    lambda ~ LogU(0.005, 2.0), 4096 states; other seeds than the benchmark's) decoded
-    as one batch; the objective is the whole batch's decode time, so the slowest
-    streams' critical paths count, not per-stream throughput;
-  * single: one 2^30-symbol stream per lambda in {0.01, 0.05, 0.2, 1.0}, each tuned
-    alone over S for streams of more than 2^24 units in its bucket.
-
-Every table's output is checked against the symbols when it is adopted.  The result is
-a rule table (mans_geometry_rule) written as JSON and as multians_amd/csrc/tuned_rules.inc,
-which the library compiles in as its default.
+Streams are made on the device (mans_gen_symbols_device, mans_build_tables_device,
+mans_encode_device: the host generator's and host codec's bytes); every decode the tuner
+times is compared with its symbols.  The result is a rule table (mans_geometry_rule)
+written as JSON and as multians_amd/csrc/tuned_rules.inc, which the library compiles in as
+its default.
 """
 import argparse
+import ctypes as C
 import json
 import os
 import time
@@ -36,147 +38,202 @@ from . import _lib
 PKG = os.path.dirname(os.path.abspath(__file__))
 
 SPU_EDGES = [4.6, 6.5, 10.0, 20.0, 40.0, 1e30]
-# the built-in heuristic's constants (L = 4096; L <= 1024 halves safe)
+# single-stream buckets (finer: a stream alone on the GPU is latency-bound, and its
+# merge distance, which sets that latency, moves quickly with the entropy) and the lambda
+# of each bucket's tuning stream (its symbols per unit at L = 4096 in brackets):
+# 0.01 (4.3), 0.03 (5.0), 0.06 (5.8), 0.1 (6.6), 0.15 (7.6), 0.25 (9.4), 0.5 (13),
+# 1.0 (21), 1.6 (32), 3.0 (106)
+SINGLE_EDGES = [4.6, 5.2, 6.0, 7.0, 8.5, 11.0, 16.0, 25.0, 50.0, 1e30]
+BUCKET_LAMBDA = [0.01, 0.03, 0.06, 0.1, 0.15, 0.25, 0.5, 1.0, 1.6, 3.0]
+SINGLE_LOG2N = [20, 22, 24, 26, 28, 30]
+SINGLE_LOG2N_LARGE_L = SINGLE_LOG2N + [32]      # 4096 states: also the 4 GiB streams
+SINGLE_L = [1024, 4096]
+SINGLE_S = [32, 64, 128, 256, 512, 1024, 2048, 4096]
+SINGLE_W = [0, 256, 512, 1024, 2048, 4096]
+# the built-in heuristic's constants for batches (L = 4096; L <= 1024 halves safe)
 FLOOR0 = [128, 256, 512, 512, 512, 512]
 SAFE0 = [2048, 4096, 8192, 8192, 8192, 2048]
 FLOORS = [128, 256, 512, 1024, 2048]
 SAFES = [1024, 2048, 4096, 8192, 16384]
-BIG = 1 << 24            # streams above this many units use the single-stream rules
 
 
-def heuristic_rules(floor, safe, big=None):
-    """Rule table reproducing clamp(pow2ceil(n/256), floor_b, safe_b) per spu bucket, for
-    L = 1024 (safe halved) and L = 4096; `big` = {bucket: S} overrides streams > BIG."""
+def batch_rules(floor, safe, warm1=None, warm2=None):
+    """Batch rule table: clamp(pow2ceil(n/256), floor_b, safe_b) per spu bucket, for
+    L = 1024 (safe halved) and L = 4096, warm-up warm1_b for the size classes whose streams
+    all fit one 256-chunk range and warm2_b for the others (None: the built-in warm-up,
+    512 units for a stream of one range and none for longer ones)."""
     rules = []
     for L, div in ((1024, 2), (4096, 1)):
         for b, spu_max in enumerate(SPU_EDGES):
             prev = None
             for k in range(8, 41):           # units in (2^(k-1), 2^k]
                 units_max = 1 << k
-                if big is not None and b in big and units_max > BIG:
-                    S = big[b]
+                fill = max(1, units_max >> 8)
+                S = max(floor[b], min(max(safe[b] // div, 128), fill), 128)
+                if warm1 is None:
+                    W = _lib.WARM_AUTO
                 else:
-                    fill = max(1, units_max >> 8)
-                    S = max(floor[b], min(max(safe[b] // div, 128), fill), 128)
-                if prev is not None and prev[3] == S:
-                    rules[-1] = (L, spu_max, units_max, S)
+                    W = warm1[b] if units_max <= 256 * S else warm2[b]
+                r = (L, spu_max, units_max, S, W, _lib.RULE_BATCH)
+                if prev is not None and prev[3:5] == r[3:5]:
+                    rules[-1] = r
                 else:
-                    rules.append((L, spu_max, units_max, S))
+                    rules.append(r)
                 prev = rules[-1]
     return rules
 
 
-def _make(specs, threads=16):
-    from concurrent.futures import ThreadPoolExecutor
-    from .decoder import EncodedStream
-
-    def one(sp):
-        seed, lam, n, L = sp
-        units, fb, fs, dec, sym = _lib.make_stream(seed, lam, n, L, True)
-        return EncodedStream(units, fb, fs, dec, L, n), sym
-    with ThreadPoolExecutor(threads) as ex:
-        return list(ex.map(one, specs))
+def _bucket(spu):
+    return next(j for j, e in enumerate(SPU_EDGES) if spu <= e)
 
 
-def _evaluate(batch, items, rules, torch, reps=3, check=True):
-    _lib.set_geometry_rules(rules)
-    batch.decode()
-    torch.cuda.synchronize()
-    if batch.status() != 0:
-        raise RuntimeError("decode status %d" % batch.status())
-    if check:
-        step = max(1, len(items) // 16)
-        for i in range(0, len(items), step):
-            if not np.array_equal(batch.output(i).cpu().numpy()[::-1], items[i][1]):
-                raise RuntimeError("tuning decode mismatch, stream %d" % i)
-    a = torch.cuda.Event(enable_timing=True)
-    z = torch.cuda.Event(enable_timing=True)
+def _time(fn, torch, reps=3):
     ts = []
     for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        z = torch.cuda.Event(enable_timing=True)
         a.record()
-        batch.decode()
+        fn()
         z.record()
         torch.cuda.synchronize()
         ts.append(a.elapsed_time(z))
     return float(np.median(ts))
 
 
-def tune_mixed(n_streams, log, sweeps=2):
+def tune_single(log):
+    """Single-stream rules: per state count, bucket and size, the fastest (S, W) of the
+    grid (L <= 1024 from the 1024-state streams, larger L from the 4096-state ones)."""
     import torch
-    from .decoder import DeviceBatch
+    from .decoder import GeneratedBatch, workspace_bytes
+    rules, rec = [], []
+    for L in SINGLE_L:
+        for b, lam in enumerate(BUCKET_LAMBDA):
+            found = []
+            for log2n in (SINGLE_LOG2N if L < SINGLE_L[-1] else SINGLE_LOG2N_LARGE_L):
+                gb = GeneratedBatch([(9000 + 97 * b + log2n + L, lam, 1 << log2n, L)])
+                nu, n = int(gb.n_units[0]), int(gb.n_symbols[0])
+                out = torch.empty(n, dtype=torch.uint8, device="cuda")
+                ws = torch.zeros(workspace_bytes(nu, L), dtype=torch.uint8, device="cuda")
+                ref = gb.symbols(0).flip(0)
+                times = {}
+                for S in SINGLE_S:
+                    if S > max(32, nu // 4):
+                        continue
+                    for W in SINGLE_W:
+                        # the warm-up multiplies the spec work by (S + W) / S: bounded
+                        # where the stream alone fills the GPU
+                        if nu * (S + W) / S > max(1 << 26, 4 * nu):
+                            continue
+                        _lib.set_geometry_rules([(1 << 16, 1e30, 1 << 40, S, W,
+                                                  _lib.RULE_SINGLE)])
+                        gb.decode_one(0, out, ws)
+                        torch.cuda.synchronize()
+                        if gb.status(ws) != 0 or not torch.equal(out, ref):
+                            raise RuntimeError("tuning decode mismatch (S=%d, W=%d)" % (S, W))
+                        times[(S, W)] = _time(lambda: gb.decode_one(0, out, ws), torch)
+                best = min(times, key=times.get)
+                spu = n / nu
+                log("single L=%d bucket %d (lambda %g, %.1f symbols/unit), 2^%d symbols, %d "
+                    "units: S=%d W=%d %.3f ms" % (L, b, lam, spu, log2n, nu, best[0], best[1],
+                                                  times[best]))
+                found.append((nu, best))
+                rec.append({"L": L, "bucket": b, "lambda": lam, "log2n": log2n, "units": nu,
+                            "spu": spu, "S": best[0], "W": best[1], "ms": round(times[best], 4),
+                            "grid_ms": {"%d/%d" % k: round(v, 4) for k, v in times.items()}})
+                del gb, out, ws, ref
+                torch.cuda.empty_cache()
+            # size boundaries: geometric midpoints between the tuned sizes
+            for i, (nu, (S, W)) in enumerate(found):
+                hi = int(np.sqrt(nu * found[i + 1][0])) if i + 1 < len(found) else 1 << 40
+                rules.append((L if L < SINGLE_L[-1] else 1 << 14, SINGLE_EDGES[b], hi, S, W,
+                              _lib.RULE_SINGLE))
+    _lib.set_geometry_rules([])
+    return rules, rec
+
+
+def tune_mixed(n_streams, single_rules, log, sweeps=2):
+    """Batch rules by coordinate descent on the mixed batch's decode time."""
+    import torch
+    from .decoder import GeneratedBatch
     rng = np.random.default_rng(20261207)
     sizes = np.floor(np.exp(rng.uniform(np.log(4096), np.log(16 << 20), n_streams))).astype(np.int64)
     lams = np.exp(rng.uniform(np.log(0.005), np.log(2.0), n_streams))
     specs = [(777_000_000 + i, float(lams[i]), int(sizes[i]), 4096) for i in range(n_streams)]
-    items = _make(specs)
-    _lib.set_geometry_rules([(1 << 16, 1e30, 1 << 40, 128)])     # largest workspace
-    batch = DeviceBatch([es for es, _ in items])
-    floor, safe = list(FLOOR0), list(SAFE0)
-    best = _evaluate(batch, items, heuristic_rules(floor, safe), torch)
+    gb = GeneratedBatch(specs)
+    ooff = np.concatenate([[0], np.cumsum(gb.n_symbols)])
+    out = torch.empty(int(ooff[-1]), dtype=torch.uint8, device="cuda")
+    descs = gb.descs(out, ooff)
+    b = C.c_uint64()
+    _lib.set_geometry_rules([(1 << 16, 1e30, 1 << 40, 128, 4096)])     # largest workspace
+    _lib.check(_lib.lib().mans_batch_workspace_bytes(descs, gb.n, C.byref(b)))
+    ws = torch.zeros(b.value, dtype=torch.uint8, device="cuda")
+    o = _lib.BatchOpts()
+    o.workspace, o.workspace_bytes = ws.data_ptr(), ws.numel()
+    o.stream = torch.cuda.current_stream().cuda_stream
+
+    def run():
+        _lib.check(_lib.lib().mans_decode_batch(descs, gb.n, C.byref(o), None))
+
+    def evaluate(rules, check):
+        _lib.set_geometry_rules(single_rules + rules)
+        run()
+        torch.cuda.synchronize()
+        if check:
+            for i in range(gb.n):
+                if not torch.equal(out[int(ooff[i]):int(ooff[i + 1])], gb.symbols(i).flip(0)):
+                    raise RuntimeError("tuning decode mismatch, stream %d" % i)
+        return _time(run, torch)
+
+    # the warm-up of batched streams stays the built-in one: a search over warm1_b / warm2_b
+    # (0-2048 units) on this mix found no change worth 1% of the batch's time
+    floor, safe, w1, w2 = list(FLOOR0), list(SAFE0), None, None
+    best = evaluate(batch_rules(floor, safe, w1, w2), True)
     log("mixed: %d streams, start %.2f ms" % (n_streams, best))
     for sweep in range(sweeps):
         changed = False
-        for b in range(len(SPU_EDGES)):
+        for bk in range(len(SPU_EDGES)):
             for which, cands in (("floor", FLOORS), ("safe", SAFES)):
                 for v in cands:
-                    f2, s2 = list(floor), list(safe)
-                    (f2 if which == "floor" else s2)[b] = v
-                    if f2[b] > s2[b] or (f2 == floor and s2 == safe):
+                    cur = {"floor": floor, "safe": safe, "warm1": w1, "warm2": w2}
+                    new = {k: (list(x) if x is not None else None) for k, x in cur.items()}
+                    new[which][bk] = v
+                    if new["floor"][bk] > new["safe"][bk] or new == cur:
                         continue
-                    t = _evaluate(batch, items, heuristic_rules(f2, s2), torch, check=False)
+                    r = batch_rules(new["floor"], new["safe"], new["warm1"], new["warm2"])
+                    t = evaluate(r, False)
                     if t < best * 0.99:
-                        t = _evaluate(batch, items, heuristic_rules(f2, s2), torch)   # checked
+                        t = evaluate(r, True)                              # checked
                         if t < best * 0.99:
                             log("  sweep %d bucket %d %s=%d: %.2f ms (was %.2f)" %
-                                (sweep, b, which, v, t, best))
-                            best, floor, safe, changed = t, f2, s2, True
+                                (sweep, bk, which, v, t, best))
+                            best, changed = t, True
+                            floor, safe, w1, w2 = (new["floor"], new["safe"], new["warm1"],
+                                                   new["warm2"])
         if not changed:
             break
-    final = _evaluate(batch, items, heuristic_rules(floor, safe), torch)
+    final = evaluate(batch_rules(floor, safe, w1, w2), True)
     _lib.set_geometry_rules([])
-    total = sum(es.n_symbols for es, _ in items)
-    log("mixed: floor %s safe %s -> %.2f ms (%.1f GB/s)" % (floor, safe, final, total / final / 1e6))
-    del batch, items
+    total = int(ooff[-1])
+    log("mixed: floor %s safe %s warm1 %s warm2 %s -> %.2f ms (%.1f GB/s)" %
+        (floor, safe, w1, w2, final, total / final / 1e6))
+    del gb, out, ws
     torch.cuda.empty_cache()
-    return floor, safe, final
-
-
-def tune_single(log):
-    import torch
-    from .decoder import DeviceBatch
-    big = {}
-    out = []
-    for i, lam in enumerate([0.01, 0.05, 0.2, 1.0]):
-        items = _make([(5000 + i, lam, 1 << 30, 4096)])
-        es = items[0][0]
-        spu = es.n_symbols / es.n_units
-        b = next(j for j, e in enumerate(SPU_EDGES) if spu <= e)
-        _lib.set_geometry_rules([(1 << 16, 1e30, 1 << 40, 128)])
-        batch = DeviceBatch([es])
-        times = {}
-        for S in [512, 1024, 2048, 4096, 8192]:
-            times[S] = round(_evaluate(batch, items, [(1 << 16, 1e30, 1 << 40, S)], torch), 3)
-        S_best = min(times, key=times.get)
-        big[b] = S_best
-        log("single lambda=%g (%.1f symbols/unit, bucket %d): %s -> S=%d" % (lam, spu, b, times, S_best))
-        out.append({"lambda": lam, "spu": spu, "bucket": b, "ms": times, "best": S_best})
-        _lib.set_geometry_rules([])
-        del batch, items
-        torch.cuda.empty_cache()
-    return big, out
+    return (floor, safe, w1, w2), final
 
 
 def write_rules(rules, meta, out_json, out_inc):
     os.makedirs(os.path.dirname(out_json), exist_ok=True)
     with open(out_json, "w") as f:
         json.dump(dict(meta, rules=[{"L_max": r[0], "spu_max": r[1], "units_max": r[2],
-                                     "chunk_units": r[3]} for r in rules]), f, indent=1)
+                                     "chunk_units": r[3], "warm_units": r[4], "scope": r[5]}
+                                    for r in rules]), f, indent=1)
     with open(out_inc, "w") as f:
-        f.write("// Generated by multians_amd/tune.py on MI355X (gfx950): built-in chunk-length\n"
-                "// rules {L_max, chunk_units, spu_max, units_max}, matched first-hit.\n")
+        f.write("// Generated by multians_amd/tune.py on MI355X (gfx950): built-in geometry\n"
+                "// rules {L_max, chunk_units, spu_max, units_max, warm_units, scope},\n"
+                "// matched first-hit within their scope (1 single-stream decode, 2 batch).\n")
         for r in rules:
             spu = "1e30" if r[1] >= 1e29 else repr(float(r[1]))
-            f.write("{%du, %du, %s, %dull},\n" % (r[0], r[3], spu, r[2]))
+            f.write("{%du, %du, %s, %dull, %du, %du},\n" % (r[0], r[3], spu, r[2], r[4], r[5]))
 
 
 def main():
@@ -184,18 +241,38 @@ def main():
     ap.add_argument("--streams", type=int, default=10000)
     ap.add_argument("--out", default=os.path.join(PKG, "tuning"))
     ap.add_argument("--inc", default=os.path.join(PKG, "csrc", "tuned_rules.inc"))
+    ap.add_argument("--no-single", action="store_true",
+                    help="keep the single-stream rules of the existing JSON")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="keep the batch parameters of the existing JSON")
     args = ap.parse_args()
     t0 = time.time()
 
     def log(*a):
         print(*a, flush=True)
-    big, single = tune_single(log)
-    floor, safe, ms = tune_mixed(args.streams, log)
-    rules = heuristic_rules(floor, safe, big)
-    meta = {"device": "MI355X (gfx950)", "floor": floor, "safe": safe, "spu_edges": SPU_EDGES,
-            "mixed_ms": ms, "mixed_streams": args.streams, "single": single,
-            "single_big_units": BIG}
-    write_rules(rules, meta, os.path.join(args.out, "gfx950.json"), args.inc)
+    import torch
+    torch.cuda.set_device(0)
+    out_json = os.path.join(args.out, "gfx950.json")
+    if args.no_single:
+        meta0 = json.load(open(out_json))
+        single = [tuple(r[k] for k in ("L_max", "spu_max", "units_max", "chunk_units",
+                                       "warm_units", "scope"))
+                  for r in meta0["rules"] if r["scope"] == _lib.RULE_SINGLE]
+        single_rec = meta0.get("single", [])
+    else:
+        single, single_rec = tune_single(log)
+    if args.no_mixed:
+        meta0 = json.load(open(out_json))
+        floor, safe = meta0["floor"], meta0["safe"]
+        w1, w2 = meta0.get("warm1"), meta0.get("warm2")
+        ms = meta0["mixed_ms"]
+    else:
+        (floor, safe, w1, w2), ms = tune_mixed(args.streams, single, log)
+    rules = single + batch_rules(floor, safe, w1, w2)
+    meta = {"device": "MI355X (gfx950)", "floor": floor, "safe": safe, "warm1": w1,
+            "warm2": w2, "spu_edges": SPU_EDGES, "single_edges": SINGLE_EDGES,
+            "mixed_ms": ms, "mixed_streams": args.streams, "single": single_rec}
+    write_rules(rules, meta, out_json, args.inc)
     log("wrote %d rules in %.0fs" % (len(rules), time.time() - t0))
 
 

@@ -127,7 +127,7 @@ def test_geometry_rules_pick_chunk_length():
         assert _lib.chunk_units(1 << 20, 20 << 20, 4096) == 4096    # spu 20 -> third
         assert _lib.chunk_units((1 << 20) + 1, 4 << 20, 4096) == 4096
         with pytest.raises(_lib.MansError):
-            _lib.set_geometry_rules([(4096, 5.0, 1 << 20, 64)])
+            _lib.set_geometry_rules([(4096, 5.0, 1 << 20, 16)])
         assert _lib.chunk_units(1 << 20, 4 << 20, 4096) == 512      # unchanged on error
     finally:
         _lib.set_geometry_rules([])
@@ -135,23 +135,24 @@ def test_geometry_rules_pick_chunk_length():
 
 
 def test_tuner_rule_table_matches_builtin_choice():
-    """The library's built-in chunk lengths are the autotuner's table: rebuilding the
-    table from the recorded tuning result (multians_amd/tuning/gfx950.json; the
-    heuristic's own constants when untuned) picks the same length for every stream
-    shape, and shapes no rule covers (L > 4096) fall back to the same heuristic."""
+    """The library's built-in geometry is the autotuner's table: loading the rules recorded
+    in multians_amd/tuning/gfx950.json picks the same chunk length for every stream shape
+    (the batch scope, which mans_chunk_units reports), the batch rules are the tuner's
+    shape rebuilt from the recorded parameters, and shapes no rule covers fall back to the
+    same heuristic."""
     import json
     from multians_amd import _lib, tune
     path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                         "multians_amd", "tuning", "gfx950.json")
-    if os.path.exists(path):
-        meta = json.load(open(path))
-        big = {d["bucket"]: d["best"] for d in meta["single"]}
-        rules = tune.heuristic_rules(meta["floor"], meta["safe"], big)
-    else:
-        rules = tune.heuristic_rules(tune.FLOOR0, tune.SAFE0)
+    meta = json.load(open(path))
+    rules = [tuple(r[k] for k in ("L_max", "spu_max", "units_max", "chunk_units", "warm_units",
+                                  "scope")) for r in meta["rules"]]
+    batch = [r for r in rules if r[5] == _lib.RULE_BATCH]
+    assert batch == tune.batch_rules(meta["floor"], meta["safe"], meta["warm1"], meta["warm2"])
+    assert all(r[5] in (_lib.RULE_SINGLE, _lib.RULE_BATCH) for r in rules)
     rng = np.random.default_rng(4)
     shapes = [(int(np.exp(rng.uniform(0, np.log(1 << 32)))), float(np.exp(rng.uniform(0, 5))),
-               int(rng.choice([256, 1024, 2048, 4096]))) for _ in range(3000)]
+               int(rng.choice([256, 1024, 2048, 4096, 8192]))) for _ in range(3000)]
     builtin = [_lib.chunk_units(n, int(n * spu), L) for n, spu, L in shapes]
     try:
         _lib.set_geometry_rules(rules)
