@@ -7,7 +7,8 @@
 //                                     merge walks from each chunk's predecessor exit)
 //   phase2_synchronise_blocks+loop -> k_check + k_fixup (consistency check, fixup
 //                                     rounds; on device, no host round trip)
-//   phase3 copies + thrust scan    -> k_scan_reduce/top/down (u64 exclusive scan)
+//   phase3 copies + thrust scan    -> k_scan (u64 exclusive scan, single pass with
+//                                     decoupled look-back)
 //   phase4_decode_write_output     -> k_write  (count-driven decode + store)
 // The per-lane logic lives in mans_core.h.
 #include <hip/hip_runtime.h>
@@ -72,6 +73,7 @@ struct KArgs {
     uint32_t n_tasks;
     uint32_t wait_polls;             // k_sync: polls of a range's wait for its predecessor
     uint32_t round_limit;            // k_fixup: rounds per stream (0 = n_chunks + 1, the bound)
+    uint32_t n_scan_blocks;          // k_scan blocks (descriptors k_init clears)
     uint32_t any_index_out;          // some stream asks for its sync points (k_index runs)
     uint32_t direct;                 // trusted item records: the write pass alone
     Work w;
@@ -219,6 +221,7 @@ __global__ void k_init(KArgs a, uint32_t n_status) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_status) w.status[i] = 0u;
     if (i < 4u) w.ctick[i] = 0u;
+    if (i < a.n_scan_blocks) w.bsum[i] = 0ull;          // k_scan's look-back descriptors
     if (i < a.n_streams) {
         w.fcnt[i] = 0u;
         uint32_t* so = plan_of(a, i)->status_out;
@@ -863,56 +866,94 @@ __device__ uint64_t wg_incl_scan(uint64_t v, uint64_t* s_w, uint64_t* tot) {
     return v + add;
 }
 
-__device__ __forceinline__ uint64_t count_of_global(const KArgs& a, uint32_t gc, uint32_t total) {
-    if (gc >= total) return 0ull;
-    const StreamPlan& sp = *plan_of(a, stream_of_chunk(a, gc));
-    if (gc - sp.chunk_base >= sp.n_chunks) return 0ull;
-    return chunk_count(sp, a.w, gc - sp.chunk_base);
-}
 
-__global__ __launch_bounds__(kWG) void k_scan_reduce(KArgs a, uint32_t total, uint64_t* bsum) {
-    __shared__ uint64_t s_w[4];
-    const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * (kScanBlock / kWG);
-    uint64_t v = 0;
-    for (uint32_t i = 0; i < kScanBlock / kWG; ++i) {
-        uint64_t c = count_of_global(a, base + i, total);
-        if (base + i < total) a.w.coff[base + i] = c;
-        v += c;
-    }
-    uint64_t tot;
-    wg_incl_scan(v, s_w, &tot);
-    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
-}
+// Single-pass exclusive scan with decoupled look-back (replaces the reference's phase-3
+// copies + thrust::exclusive_scan).  Workgroups take blocks of kScanBlock chunks in
+// ticket order (ctick[2]), so every block a workgroup looks back at belongs to a
+// workgroup that started earlier.  Each block scans its counts in registers and LDS,
+// publishes its total (flag 1), then one wave walks back over the preceding blocks'
+// descriptors 64 at a time, adding totals down to the nearest inclusive prefix (flag 2),
+// publishes its own inclusive prefix and writes its chunk offsets.  A descriptor is one u64: flag in bits 62-63,
+// value below (offsets are far below 2^62).  k_init zeroes the descriptors.  A
+// predecessor publishes its total without waiting on anything, so every wait ends; the
+// wait is bounded anyway (kScanPolls polls per window) and a run-out sets status bit 2
+// (reported as MANS_ECORRUPT).
+constexpr uint64_t kScanAgg = 1ull << 62, kScanIncl = 2ull << 62, kScanVal = kScanAgg - 1ull;
+constexpr uint32_t kScanPolls = 1u << 22;   // bound of one look-back wait (never reached)
 
-__global__ __launch_bounds__(kWG) void k_scan_top(uint64_t* bsum, uint32_t nb) {
+__global__ __launch_bounds__(kWG) void k_scan(KArgs a, uint32_t total, uint64_t* desc) {
     __shared__ uint64_t s_w[4];
-    __shared__ uint64_t s_carry;
-    if (threadIdx.x == 0) s_carry = 0;
+    __shared__ uint32_t s_blk;
+    __shared__ uint64_t s_prefix;
+    if (threadIdx.x == 0) s_blk = atomicAdd(&a.w.ctick[2], 1u);
     __syncthreads();
-    for (uint32_t b0 = 0; b0 < nb; b0 += kWG) {
-        uint32_t i = b0 + threadIdx.x;
-        uint64_t v = i < nb ? bsum[i] : 0ull, tot;
-        uint64_t incl = wg_incl_scan(v, s_w, &tot);
-        uint64_t carry = s_carry;
-        if (i < nb) bsum[i] = carry + incl - v;   // exclusive
-        __syncthreads();
-        if (threadIdx.x == 0) s_carry = carry + tot;
-        __syncthreads();
-    }
-}
-
-__global__ __launch_bounds__(kWG) void k_scan_down(KArgs a, uint32_t total, const uint64_t* bsum) {
-    __shared__ uint64_t s_w[4];
+    const uint32_t blk = s_blk;
     const uint32_t per = kScanBlock / kWG;
-    const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * per;
+    const uint32_t base = blk * kScanBlock + threadIdx.x * per;
     uint64_t c[kScanBlock / kWG];
     uint64_t v = 0;
+    // the thread's chunks are consecutive: find the first one's stream, then step on
+    uint32_t sid = base < total ? stream_of_chunk(a, base) : 0u;
     for (uint32_t i = 0; i < per; ++i) {
-        c[i] = base + i < total ? a.w.coff[base + i] : 0ull;
+        const uint32_t gc = base + i;
+        c[i] = 0u;
+        if (gc < total) {
+            while (a.plans && sid + 1u < a.n_streams && a.plans[sid + 1u].chunk_base <= gc) ++sid;
+            const StreamPlan& sp = *plan_of(a, sid);
+            if (gc - sp.chunk_base < sp.n_chunks) c[i] = chunk_count(sp, a.w, gc - sp.chunk_base);
+        }
         v += c[i];
     }
     uint64_t tot;
-    uint64_t run = bsum[blockIdx.x] + wg_incl_scan(v, s_w, &tot) - v;
+    const uint64_t incl = wg_incl_scan(v, s_w, &tot);
+    if (threadIdx.x < 64u) {                   // wave 0: publish, then look back 64 at a time
+        const uint32_t lane = threadIdx.x;
+        uint64_t prefix = 0;
+        if (blk == 0u) {
+            if (lane == 0u)
+                __hip_atomic_store(&desc[0], kScanIncl | tot, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (lane == 0u)
+                __hip_atomic_store(&desc[blk], kScanAgg | tot, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            // window [lo, hi): descriptor of block hi - 1 - lane
+            uint32_t hi = blk;
+            bool done = false, ran_out = false;
+            while (!done && hi > 0u) {
+                const uint32_t j = hi - 1u - lane;        // wraps for lanes below the start
+                const bool in = lane < hi;
+                uint64_t d = 0;
+                for (uint32_t i = 0;; ++i) {
+                    d = in ? __hip_atomic_load(&desc[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                           : kScanIncl;                   // past block 0: nothing more
+                    const uint64_t missing = __ballot((d >> 62) == 0u);
+                    if (!missing) break;
+                    if (i >= kScanPolls) { ran_out = true; break; }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                __atomic_thread_fence(__ATOMIC_ACQUIRE);
+                if (ran_out) break;
+                // the nearest inclusive prefix in the window (lowest lane with flag 2)
+                const uint64_t inclm = __ballot((d >> 62) == 2u);
+                const uint32_t stop = inclm ? (uint32_t)__builtin_ctzll(inclm) : 64u;
+                uint64_t add = (lane <= stop && in) ? (d & kScanVal) : 0ull;
+                for (uint32_t o = 32; o > 0; o >>= 1) add += __shfl_xor(add, o, 64);
+                prefix += add;
+                if (inclm) done = true;
+                hi = hi > 64u ? hi - 64u : 0u;
+            }
+            if (ran_out) {
+                if (lane == 0u) atomicOr(&a.w.status[0], 4u);
+            } else if (lane == 0u) {
+                __hip_atomic_store(&desc[blk], kScanIncl | (prefix + tot), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (lane == 0u) s_prefix = prefix;
+    }
+    __syncthreads();
+    uint64_t run = s_prefix + incl - v;
     for (uint32_t i = 0; i < per; ++i) {
         if (base + i < total) a.w.coff[base + i] = run;
         run += c[i];
@@ -1429,7 +1470,9 @@ static int launch_pipeline(KArgs a, uint32_t n_chunk_tasks, uint32_t n_item_task
     a.wait_polls = g_wait_polls.load();
     a.round_limit = g_round_limit.load();
     mark(0);
-    hipLaunchKernelGGL(k_init, dim3((std::max({a.n_streams, a.n_tasks, 64u}) + 255u) / 256u),
+    a.n_scan_blocks = (total_chunks + kScanBlock - 1) / kScanBlock;
+    hipLaunchKernelGGL(k_init,
+                       dim3((std::max({a.n_streams, a.n_tasks, a.n_scan_blocks, 64u}) + 255u) / 256u),
                        dim3(256), 0, st, a, 64u);
     mark(1);
     if (a.direct) {                         // trusted item records: the write pass alone
@@ -1450,11 +1493,8 @@ static int launch_pipeline(KArgs a, uint32_t n_chunk_tasks, uint32_t n_item_task
     hipLaunchKernelGGL(k_fixup, dim3(a.n_streams), dim3(kWG), lds_s, st, a);
     mark(4);
     {
-        uint32_t nb = (total_chunks + kScanBlock - 1) / kScanBlock;
-        uint64_t* bsum = a.w.bsum;
-        hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kWG), 0, st, a, total_chunks, bsum);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kWG), 0, st, bsum, nb);
-        hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kWG), 0, st, a, total_chunks, bsum);
+        const uint32_t nb = (total_chunks + kScanBlock - 1) / kScanBlock;
+        hipLaunchKernelGGL(k_scan, dim3(nb), dim3(kWG), 0, st, a, total_chunks, a.w.bsum);
     }
     if (a.any_index_out) hipLaunchKernelGGL(k_index, dim3(n_chunk_tasks), dim3(kWG), 0, st, a);
     if (a.ref_sync) {
@@ -1716,8 +1756,9 @@ extern "C" int mans_decode_status(void* workspace, void* stream, int* status) {
     hipError_t e = hipMemcpyAsync(&v, st, 4, hipMemcpyDeviceToHost, (hipStream_t)stream);
     if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
     if (e != hipSuccess) return set_error(MANS_EHIP, "status read: %s", hipGetErrorString(e));
-    // bit 0: a rejected table; bit 1: sync rounds that did not converge (k_fixup)
-    *status = (v & 1u) ? MANS_ETABLE : (v & 2u) ? MANS_ECORRUPT : 0;
+    // bit 0: a rejected table; bit 1: sync rounds that did not converge (k_fixup); bit 2:
+    // a scan look-back wait that ran out
+    *status = (v & 1u) ? MANS_ETABLE : (v & 6u) ? MANS_ECORRUPT : 0;
     return MANS_OK;
 }
 
