@@ -590,59 +590,67 @@ def bench_stress(args, torch):
 
 
 def bench_served(args, torch):
-    """Served configuration: open-loop Poisson arrivals at 70% of the measured batched
+    """Served configuration: open-loop Poisson arrivals at 70% of the measured served
     capacity on the request mix (L 1024/4096 50/50, uint8 sizes log-uniform 1 KiB-64 MiB,
-    lambda ~ LogU(0.005, 2.0)), admitted under a 64 GB HBM budget, continuous batching."""
+    lambda ~ LogU(0.005, 2.0)), admitted under a 64 GB HBM budget, continuous batching.
+    Each request brings its compressed stream and table from pinned host memory (a pool of
+    client streams it is drawn from): copied to the device on admission, decoded, and
+    released; the budget counts inputs, outputs and workspace, and latency runs from
+    arrival to completion, copy included."""
     from multians_amd import serve
     t0 = time.time()
     specs = serve.request_mix(args.serve_pool)
     items = make_streams(specs)
-    pool = serve.StreamPool([es for es, _ in items], [sym for _, sym in items])
-    del items
-    log("serve pool: %d streams, %.2f GB symbols, built in %.1fs" %
-        (pool.n, pool.n_sym.sum() / 1e9, time.time() - t0))
+    pool = serve.StreamPool([es for es, _ in items], [sym for _, sym in items], host_inputs=True)
+    log("serve pool (host inputs): %d streams, %.2f GB symbols, %.2f GB compressed, built in "
+        "%.1fs" % (pool.n, pool.n_sym.sum() / 1e9, pool.in_bytes.sum() / 1e9, time.time() - t0))
     srv = serve.Server(pool, 64e9, max_inflight=3)
-    # capacity: every pool stream at t = 0 (one batch), every result checked
+    # capacity: every pool stream at t = 0 (one backlog), every result checked
     cap = srv.run(np.zeros(pool.n), np.arange(pool.n), check_every=1)
-    if cap.mismatches:
-        raise SystemExit("served capacity run: %d mismatching streams" % cap.mismatches)
+    if cap.mismatches or cap.failed:
+        raise SystemExit("served capacity run: %d mismatching, %d failed" % (cap.mismatches, cap.failed))
     cap = srv.run(np.zeros(pool.n), np.arange(pool.n))
     capacity = cap.symbols / cap.wall_s
+    in_rate = pool.in_bytes.sum() / cap.wall_s
     rate = 0.7 * capacity / float(pool.n_sym.mean())
     arr = serve.poisson_arrivals(rate, args.serve_seconds)
     ids = np.random.default_rng(11).integers(0, pool.n, len(arr))
-    log("serve: capacity %.1f GB/s, offered %.1f GB/s = %.0f requests/s for %gs" %
-        (capacity / 1e9, 0.7 * capacity / 1e9, rate, args.serve_seconds))
+    log("serve: capacity %.1f GB/s of symbols (%.1f GB/s of inputs over the host link), "
+        "offered %.1f GB/s = %.0f requests/s for %gs" %
+        (capacity / 1e9, in_rate / 1e9, 0.7 * capacity / 1e9, rate, args.serve_seconds))
     st = srv.run(arr, ids, check_every=16)
-    if st.mismatches:
-        raise SystemExit("served run: %d mismatching requests" % st.mismatches)
+    if st.mismatches or st.failed:
+        raise SystemExit("served run: %d mismatching, %d failed" % (st.mismatches, st.failed))
     out = st.summary()
     out.update({"capacity_GBps": round(capacity / 1e9, 3),
+                "capacity_input_GBps": round(in_rate / 1e9, 3),
                 "offered_GBps": round(0.7 * capacity / 1e9, 3),
                 "offered_requests_per_s": round(rate, 1), "duration_s": args.serve_seconds,
-                "pool_streams": pool.n, "budget_GB": 64, "max_inflight": 3,
-                "latency": "arrival to completion observed on the host; inputs resident"})
+                "client_pool_streams": pool.n, "budget_GB": 64, "max_inflight": 3,
+                "latency": "arrival to completion observed on the host; each request's "
+                           "compressed stream and table copied host-to-device on admission "
+                           "(included)"})
     del srv
-    if not args.no_index:
-        # the same pool and offered load, serving repeat requests from item records that
-        # each stream's first decode wrote (records warmed by one pass over the pool)
-        pool.attach_records()
-        rsrv = serve.Server(pool, 64e9, max_inflight=3, use_records=True)
-        warm = rsrv.run(np.zeros(pool.n), np.arange(pool.n), check_every=1)
+    if args.serve_records:
+        # resident streams (not the served configuration): repeat requests decoded from the
+        # item records each stream's first decode wrote
+        rpool = serve.StreamPool([es for es, _ in items], [sym for _, sym in items], records=True)
+        rsrv = serve.Server(rpool, 64e9, max_inflight=3, use_records=True)
+        warm = rsrv.run(np.zeros(rpool.n), np.arange(rpool.n), check_every=1)
         if warm.mismatches:
             raise SystemExit("served records warm-up: %d mismatching streams" % warm.mismatches)
-        rcap = rsrv.run(np.zeros(pool.n), np.arange(pool.n))
-        keep = arr < min(args.serve_seconds, 30.0)          # at most 30 s of the arrivals
+        rcap = rsrv.run(np.zeros(rpool.n), np.arange(rpool.n))
+        keep = arr < min(args.serve_seconds, 30.0)
         rst = rsrv.run(arr[keep], ids[keep], check_every=16)
         if rst.mismatches:
             raise SystemExit("served from records: %d mismatching requests" % rst.mismatches)
-        out["from_item_records"] = dict(rst.summary(),
-                                        capacity_GBps=round(rcap.symbols / rcap.wall_s / 1e9, 3),
-                                        offered_GBps=round(0.7 * capacity / 1e9, 3),
-                                        duration_s=min(args.serve_seconds, 30.0))
-        log("served from item records", json.dumps(out["from_item_records"]))
-        del rsrv
-    del pool
+        out["resident_from_item_records"] = dict(
+            rst.summary(), capacity_GBps=round(rcap.symbols / rcap.wall_s / 1e9, 3),
+            duration_s=min(args.serve_seconds, 30.0),
+            note="streams resident on the device, repeat requests from item records")
+        log("served from item records (resident)", json.dumps(out["resident_from_item_records"]))
+        del rsrv, rpool
+    del pool, items
     torch.cuda.empty_cache()
     return out
 
@@ -723,7 +731,10 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="duration of the served run (0 = skip)")
     ap.add_argument("--serve-pool", type=int, default=1000,
-                    help="distinct streams the served requests draw from")
+                    help="client streams (host memory) the served requests draw from")
+    ap.add_argument("--serve-records", action="store_true",
+                    help="also serve resident streams from item records (not the served "
+                         "configuration)")
     ap.add_argument("--one-pipeline", action="store_true",
                     help="decode the batch as one pipeline (no concurrent parts), as the "
                          "per-kernel timing decode does; for per-dispatch profiles")

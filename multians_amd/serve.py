@@ -1,15 +1,22 @@
 """Continuous-batching decode server over mans_decode_batch.
 
-Requests are decodes of device-resident tANS streams (each with its own table).  The
-loop admits arrived requests under an HBM budget, coalesces everything admitted since
-the last launch into the next batched launch, and keeps up to `max_inflight` batches
-on the device at once, each on its own HIP stream, so a batch that arrives while others
-run overlaps them.  A request's latency runs from its arrival to the moment its batch's
-completion event is observed on the host.
+Requests are decodes of tANS streams (each with its own table).  The loop admits arrived
+requests under an HBM budget, coalesces everything admitted since the last launch into the
+next batched launch, and keeps up to `max_inflight` batches on the device at once, each on
+its own HIP stream, so a batch that arrives while others run overlaps them.  A request's
+latency runs from its arrival to the moment its batch's completion event is observed on
+the host.
 
-Memory: compressed inputs and tables are resident (a stream pool); the budget covers
-what a request holds while it is admitted -- its output bytes and its share of the batch
-workspace -- carved from one device arena.
+Two input modes:
+  * host inputs (StreamPool(host_inputs=True), the served configuration): a request brings
+    its compressed stream and table from host memory (the pool plays the clients: pinned
+    host buffers, one per stream).  On admission they are copied to the device on a copy
+    stream, into the request's block of the device arena, and the batch's decode waits
+    for the copy; nothing of a stream stays on the device after its request.  The budget
+    covers inputs, outputs, the batch workspace and status words, and latency includes
+    the copy.
+  * resident inputs (the default): compressed streams and tables stay on the device; the
+    budget covers what a request holds while admitted (output, workspace share).
 """
 import bisect
 import ctypes as C
@@ -79,31 +86,50 @@ class StreamPool:
     aligned, padded to whole 16-byte blocks), packed tables, and optionally the
     original symbols (decode order) for checking results."""
 
-    def __init__(self, streams, symbols=None, device="cuda", records=False):
+    def __init__(self, streams, symbols=None, device="cuda", records=False, host_inputs=False):
         """records: keep a buffer of 32-unit item records per stream (written by the
-        stream's first decode, read by later ones; Server(use_records=True))."""
+        stream's first decode, read by later ones; Server(use_records=True)).
+        host_inputs: keep the streams in pinned host memory instead, each as one image
+        [table | units] (256-byte aligned), copied to the device per request."""
         import torch
         self.streams = list(streams)
         n = len(self.streams)
         self.n = n
-        uoff = np.zeros(n + 1, np.int64)
-        toff = np.zeros(n + 1, np.int64)
-        for i, es in enumerate(self.streams):
-            uoff[i + 1] = uoff[i] + ((es.n_units + 3) // 4 + 1) * 4
-            toff[i + 1] = toff[i] + es.L
-        units = np.zeros(uoff[-1], np.uint32)
-        tables = np.zeros(toff[-1], np.uint32)
-        for i, es in enumerate(self.streams):
-            units[uoff[i]:uoff[i] + es.n_units] = es.units
-            tables[toff[i]:toff[i] + es.L] = es.table
-        self.units = torch.from_numpy(units.view(np.int32)).to(device)
-        self.tables = torch.from_numpy(tables.view(np.int32)).to(device)
-        del units, tables
-        self.uptr = self.units.data_ptr() + 4 * uoff[:-1]
-        self.tptr = self.tables.data_ptr() + 4 * toff[:-1]
+        self.host_inputs = bool(host_inputs)
+        if self.host_inputs and records:
+            raise ValueError("item records need resident inputs")
         self.n_units = np.array([es.n_units for es in self.streams], np.int64)
-        self.n_sym = np.array([es.n_symbols for es in self.streams], np.int64)
         self.L = np.array([es.L for es in self.streams], np.int64)
+        if self.host_inputs:
+            # per stream: table (4L bytes), then units padded to whole 16-byte blocks
+            self.in_bytes = np.array([_up(4 * es.L + 16 * ((es.n_units + 3) // 4 + 1))
+                                      for es in self.streams], np.int64)
+            ioff = np.concatenate([[0], np.cumsum(self.in_bytes)])
+            self.inputs = torch.zeros(max(int(ioff[-1]), 1), dtype=torch.uint8, pin_memory=True)
+            host = self.inputs.numpy()
+            for i, es in enumerate(self.streams):
+                o = int(ioff[i])
+                host[o:o + 4 * es.L] = es.table.view(np.uint8)
+                host[o + 4 * es.L:o + 4 * es.L + 4 * es.n_units] = es.units.view(np.uint8)
+            self.in_ptr = self.inputs.data_ptr() + ioff[:-1]
+            self.uptr = self.tptr = None
+        else:
+            uoff = np.zeros(n + 1, np.int64)
+            toff = np.zeros(n + 1, np.int64)
+            for i, es in enumerate(self.streams):
+                uoff[i + 1] = uoff[i] + ((es.n_units + 3) // 4 + 1) * 4
+                toff[i + 1] = toff[i] + es.L
+            units = np.zeros(uoff[-1], np.uint32)
+            tables = np.zeros(toff[-1], np.uint32)
+            for i, es in enumerate(self.streams):
+                units[uoff[i]:uoff[i] + es.n_units] = es.units
+                tables[toff[i]:toff[i] + es.L] = es.table
+            self.units = torch.from_numpy(units.view(np.int32)).to(device)
+            self.tables = torch.from_numpy(tables.view(np.int32)).to(device)
+            del units, tables
+            self.uptr = self.units.data_ptr() + 4 * uoff[:-1]
+            self.tptr = self.tables.data_ptr() + 4 * toff[:-1]
+        self.n_sym = np.array([es.n_symbols for es in self.streams], np.int64)
         self.first_state = np.array([es.first_state for es in self.streams], np.int64)
         self.first_bit = np.array([es.first_bit for es in self.streams], np.int64)
         self.have_rec = np.zeros(n, bool)
@@ -143,6 +169,7 @@ class Request:
     t_admit: float = 0.0
     t_done: float = 0.0
     out_off: int = 0      # arena offset of the output
+    in_off: int = 0       # arena offset of the input image (host inputs)
     status: int = -1      # the decoder's status word for this request (0 = ok)
 
 
@@ -205,6 +232,9 @@ class Server:
         self.max_inflight = max_inflight
         self.max_batch = max_batch
         self.streams = [torch.cuda.Stream(device=device) for _ in range(max_inflight)]
+        self.host = pool.host_inputs
+        # host inputs: copies on their own stream (one link; batches queue there in order)
+        self.copy_stream = torch.cuda.Stream(device=device) if self.host else None
         self.lib = _lib.lib()
 
     def _groups(self, reqs):
@@ -224,6 +254,11 @@ class Server:
         for r in reqs:
             r.out_off = out_bytes
             out_bytes += _up(int(p.n_sym[r.stream]))
+        in_bytes = 0
+        if self.host:
+            for r in reqs:
+                r.in_off = out_bytes + in_bytes
+                in_bytes += int(p.in_bytes[r.stream])
         launches = []
         ws = 0
         for group, trusted in self._groups(reqs):
@@ -232,12 +267,13 @@ class Server:
             for j, r in enumerate(group):
                 i = r.stream
                 d = descs[j]
-                d.units = int(p.uptr[i])
+                # workspace sizing only reads sizes and pointers: out (and, with host
+                # inputs, units and table) are set at launch
+                d.units = 1 << 12 if self.host else int(p.uptr[i])
                 d.n_units = int(p.n_units[i])
-                # workspace sizing only reads sizes and pointers: out is set at launch
                 d.out = 1 << 12
                 d.n_out = int(p.n_sym[i])
-                d.table = int(p.tptr[i])
+                d.table = 1 << 12 if self.host else int(p.tptr[i])
                 d.L = int(p.L[i])
                 d.first_state = int(p.first_state[i])
                 d.first_bit = int(p.first_bit[i])
@@ -251,7 +287,7 @@ class Server:
             _lib.check(self.lib.mans_batch_workspace_bytes(descs, n, C.byref(b)))
             ws = max(ws, int(b.value))
             launches.append((group, trusted, descs))
-        return launches, out_bytes, ws
+        return launches, out_bytes + in_bytes, ws
 
     def _launch(self, reqs, t_now, hs, check_every):
         torch = self.torch
@@ -268,9 +304,25 @@ class Server:
             r.out_off += off
         fresh = []
         g0 = 0                                  # first status word of the group
+        if self.host:
+            # the requests' inputs: pinned host -> their block, on the copy stream; the
+            # decode stream waits for them
+            cs = self.copy_stream
+            for r in reqs:
+                r.in_off += off
+                _lib.check(self.lib.mans_memcpy_h2d(
+                    C.c_void_p(self.arena.base + r.in_off),
+                    C.c_void_p(int(self.pool.in_ptr[r.stream])),
+                    int(self.pool.in_bytes[r.stream]), C.c_void_p(cs.cuda_stream)))
+            copied = self.torch.cuda.Event()
+            copied.record(cs)
+            hs.wait_event(copied)
         for group, trusted, descs in launches:
             for j, r in enumerate(group):
                 descs[j].out = self.arena.base + r.out_off
+                if self.host:
+                    descs[j].table = self.arena.base + r.in_off
+                    descs[j].units = self.arena.base + r.in_off + 4 * int(self.pool.L[r.stream])
             o = _lib.BatchOpts()
             o.workspace = base + out_bytes
             o.workspace_bytes = ws_bytes
@@ -367,6 +419,8 @@ class Server:
                     # the request's status word
                     est = _up(n) + (3 if self.use_records else 2) * \
                         int(self.pool.n_units[r.stream]) + 4096 + 4
+                    if self.host:
+                        est += int(self.pool.in_bytes[r.stream])
                     if take and need + est > room:
                         break
                     take.append(pending.popleft())

@@ -108,3 +108,58 @@ def test_served_failed_stream_is_reported():
     assert st.checked == len(ids) - n_bad and st.mismatches == 0
     assert not pool.have_rec[bad]
     assert pool.have_rec.sum() == pool.n - 1
+
+
+@pytest.mark.gpu
+def test_served_host_inputs_match_alone():
+    """Requests that bring their compressed stream and table from (pinned) host memory:
+    copied to the device on admission, decoded in the mix, every result equal to the
+    symbols and to the stream decoded alone; nothing stays on the device afterwards."""
+    import torch
+    from multians_amd import serve
+    from multians_amd.decoder import DeviceStream
+    import bench
+
+    specs = serve.request_mix(40, seed=15, max_bytes=1 << 20)
+    items = bench.make_streams(specs)
+    pool = serve.StreamPool([es for es, _ in items], [sym for _, sym in items], host_inputs=True)
+    srv = serve.Server(pool, 1 << 30, max_inflight=3)
+    arr = serve.poisson_arrivals(2000.0, 0.4, seed=13)
+    ids = np.random.default_rng(14).integers(0, pool.n, len(arr))
+    print("serve host inputs: %d requests" % len(arr), flush=True)
+    st = srv.run(arr, ids, check_every=1)
+    s = st.summary()
+    print(s, flush=True)
+    assert s["requests"] == len(arr) and s["failed"] == 0
+    assert st.checked == len(arr) and st.mismatches == 0
+    assert srv.arena.used == 0
+    for i in range(0, pool.n, 9):
+        es, sym = items[i]
+        got = DeviceStream(es).decode()[:es.n_symbols]
+        assert torch.equal(got, pool.expected(i))
+
+
+@pytest.mark.gpu
+def test_served_shape_budget_binds():
+    """The served configuration's request shape -- 1024 or 4096 states, uint8 sizes
+    log-uniform up to 64 MiB, lambda ~ LogU(0.005, 2.0), inputs from the host -- under a
+    budget that holds only a few requests at once: the budget binds (several batches,
+    peak use within it) and every request decodes to its symbols."""
+    from multians_amd import serve
+    import bench
+
+    specs = serve.request_mix(16, seed=21, min_bytes=1 << 20, max_bytes=64 << 20)
+    items = bench.make_streams(specs)
+    pool = serve.StreamPool([es for es, _ in items], [sym for _, sym in items], host_inputs=True)
+    budget = 400 << 20
+    srv = serve.Server(pool, budget, max_inflight=3)
+    ids = np.arange(48) % pool.n
+    print("serve shape: %d requests, %.0f MB of symbols" %
+          (len(ids), pool.n_sym[ids].sum() / 1e6), flush=True)
+    st = srv.run(np.zeros(len(ids)), ids, check_every=1)
+    s = st.summary()
+    print(s, flush=True)
+    assert s["requests"] == len(ids) and s["failed"] == 0 and st.mismatches == 0
+    assert st.checked == len(ids)
+    assert srv.arena.peak <= budget and st.n_batches >= 6
+    assert srv.arena.used == 0
