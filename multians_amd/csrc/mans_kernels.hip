@@ -454,13 +454,13 @@ __device__ uint32_t walk_rounds(const StreamPlan& sp, const Work& w, const LdsTa
 // chunk write theirs (wentry, and mq/wcm if they merge at its first checkpoint) after
 // it: a walk that started one chunk further back can enter chunk g in the same round in
 // which another walk leaves g.
-// nr_max (uniform over the workgroup) bounds the rounds of every lane's walk.  Once all
-// walks have left their first chunk, the ones still running are compacted into the lowest
-// lanes every four rounds (their state moves through LDS), so a long tail of a few walks
-// keeps one wave issuing instead of four.
-// With mrg (LDS, one word per lane, zeroed by the caller) the lanes publish when their
-// walk merges, and a walk leaves its first chunk only if the previous lane's walk merged
-// inside its own chunk: otherwise its start (the previous chunk's spec exit) is not on
+// nr_max (uniform over the workgroup) bounds the rounds of every lane's walk.  Every four
+// rounds the walks still running are compacted into the lowest lanes when that frees a
+// wave (their state moves through LDS), so a long tail of a few walks keeps one wave
+// issuing instead of four.
+// With mrg (LDS, one word per origin lane, zeroed by the caller) the walks publish when
+// they merge in their first chunk, and a walk leaves its first chunk only if the walk of
+// the chunk before it merged inside its own chunk: otherwise its start (the previous chunk's spec exit) is not on
 // the true path, and continuing would overwrite good records with a wrong path (the
 // previous lane's continuing walk covers those chunks instead).
 __device__ void walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& tab,
@@ -468,7 +468,8 @@ __device__ void walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& ta
                            uint32_t entry, uint32_t nr_max, uint32_t* mrg = nullptr) {
     __shared__ uint32_t s_wany[kWG / 64], s_wcnt[kWG / 64];
     uint32_t gc = sp.chunk_base + c;       // chunk the walk is in
-    uint32_t gc0 = gc;                     // first chunk (kNone once lanes are compacted)
+    uint32_t gc0 = gc;                     // its first chunk
+    uint32_t org = threadIdx.x;            // its lane before any compaction (mrg index)
     RLane L;
     L.active = 0u;
     if (want) {
@@ -480,7 +481,7 @@ __device__ void walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& ta
         if (ld_coherent(&w.ck[ck_slot(sp, q0)]) == entry) {
             w.mq[gc] = q0;
             w.wcm[gc] = 0u;
-            if (mrg) mrg[threadIdx.x] = 1u;
+            if (mrg) mrg[org] = 1u;
         } else {
             w.wb[ck_slot(sp, q0)] = entry;
             w.wbc[ck_slot(sp, q0)] = 0u;
@@ -516,7 +517,7 @@ __device__ void walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& ta
                         w.mq[gc] = kNone;
                         w.wexit[gc] = v;
                         w.wtotal[gc] = L.cnt;
-                        if (mrg && gc == gc0 && threadIdx.x > 0u && !mrg[threadIdx.x - 1u]) {
+                        if (mrg && gc == gc0 && org > 0u && !mrg[org - 1u]) {
                             L.active = 0u;         // start not proven: stop here
                             test = 0u;
                         } else {
@@ -538,7 +539,7 @@ __device__ void walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& ta
             if (ld_coherent(&w.ck[ck_slot(sp, q)]) == v) {
                 w.mq[gc] = q;
                 w.wcm[gc] = L.cnt;
-                if (mrg && gc == gc0) mrg[threadIdx.x] = 1u;
+                if (mrg && gc == gc0) mrg[org] = 1u;
                 L.active = 0u;
             } else {
                 w.wb[ck_slot(sp, q)] = v;
@@ -548,10 +549,11 @@ __device__ void walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& ta
         if (!L.active) io.rbase[threadIdx.x] = kNone;
         const uint32_t n_act = (uint32_t)__syncthreads_count(L.active);
         if (n_act == 0u) break;
-        // Compaction: once every walk has left its first chunk, the walks still running
-        // are moved to the lowest lanes every few rounds, so that a long walk tail
-        // keeps as few waves issuing as its number of walks needs.
-        if (r + 1u < r_first || ((r + 1u) & 3u) != 0u || r + 1u >= nr_max) continue;
+        // Compaction: every four rounds, when it frees a wave, the walks still running
+        // move to the lowest lanes (carrying their first chunk and origin lane, which the
+        // merge flags are indexed by), so that the walks left keep as few waves issuing
+        // as their number needs -- also in the first chunk, where most walks end.
+        if (((r + 1u) & 3u) != 0u || r + 1u >= nr_max) continue;
         const uint64_t bal = __ballot(L.active);
         if ((threadIdx.x & 63u) == 0u) {
             s_wany[wave] = bal ? 1u : 0u;
@@ -570,7 +572,8 @@ __device__ void walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& ta
                 (uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
             uint32_t* d = x + pos * 8u;
             d[0] = L.u; d[1] = L.o; d[2] = L.xi; d[3] = L.cnt;
-            d[4] = L.u_stop; d[5] = L.rbase; d[6] = gc;
+            // a walk stays inside its 256-chunk range: gc - gc0 and org fit a byte each
+            d[4] = L.u_stop; d[5] = L.rbase; d[6] = gc; d[7] = (gc - gc0) | (org << 8);
         }
         __syncthreads();
         L.active = threadIdx.x < n_act ? 1u : 0u;
@@ -578,8 +581,9 @@ __device__ void walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& ta
             const uint32_t* d = x + threadIdx.x * 8u;
             L.u = d[0]; L.o = d[1]; L.xi = d[2]; L.cnt = d[3];
             L.u_stop = d[4]; L.rbase = d[5]; gc = d[6];
+            gc0 = gc - (d[7] & 0xFFu);
+            org = d[7] >> 8;
         }
-        gc0 = kNone;                       // every walk is past its first chunk
         io.rbase[threadIdx.x] = L.active ? L.rbase : kNone;
         __syncthreads();                   // new rbase[] visible, exchange reads done
         round_load(io, r + 1u, pf);        // reload the next round for the new lanes

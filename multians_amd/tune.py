@@ -53,6 +53,11 @@ SINGLE_W = [0, 256, 512, 1024, 2048, 4096]
 # the built-in heuristic's constants for batches (L = 4096; L <= 1024 halves safe)
 FLOOR0 = [128, 256, 512, 512, 512, 512]
 SAFE0 = [2048, 4096, 8192, 8192, 8192, 2048]
+# batch streams above BIG units take these chunk lengths (the single-stream sweep of
+# 2^30-symbol streams per bucket: fewer, longer chunks than the fill rule would give;
+# batches of the benchmark mix never hold such streams)
+BIG = 1 << 24
+BIG_S = [1024, 1024, 4096, 4096, 4096, 2048]
 FLOORS = [128, 256, 512, 1024, 2048]
 SAFES = [1024, 2048, 4096, 8192, 16384]
 
@@ -70,6 +75,8 @@ def batch_rules(floor, safe, warm1=None, warm2=None):
                 units_max = 1 << k
                 fill = max(1, units_max >> 8)
                 S = max(floor[b], min(max(safe[b] // div, 128), fill), 128)
+                if units_max > BIG:
+                    S = BIG_S[b]
                 if warm1 is None:
                     W = _lib.WARM_AUTO
                 else:
