@@ -264,6 +264,79 @@ int mo_subseq_last(const uint32_t* units, size_t n_units, const uint32_t* table,
     return 0;
 }
 
+
+/* ------------------------------------------------- interleaved-state streams */
+/* No reference counterpart: the reference names "multiple tables / multiple states" as
+ * future work (README.md:11).  Format restated here (own design): N states share one bit
+ * stream; input symbol i is encoded by state slot k = i mod N with that slot's encoder
+ * table, in input order, each code appended MSB-first exactly as the single-state encoder
+ * (ans_encoder.cc:13-72) appends it; every slot starts at x = L; after the last symbol the
+ * slots' states are the decoder's first states; units are reversed into decoder order.
+ * Decoding (decode order j = 0 .. n-1 yields input symbol n-1-j) runs the single-state
+ * step (multians_gpu_decoder.cu:127-169) on slot (n-1-j) mod N with that slot's decoder
+ * table.  With N = 1 both reduce to mo_encode / mo_decode_seq. */
+size_t mo_encode_interleaved(const uint8_t* in, size_t n, const mo_enc_entry* const* enc,
+                             uint32_t L, uint32_t N, uint32_t* out_units, uint32_t* first_bit,
+                             uint32_t* first_states) {
+    uint64_t acc = 0;
+    uint32_t nacc = 0, x[16];
+    size_t nu = 0;
+    if (N == 0 || N > 16) return 0;
+    for (uint32_t k = 0; k < N; ++k) x[k] = L;
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t k = (uint32_t)(i % N);
+        const mo_enc_entry* e = &enc[k][(size_t)in[i] * L + (x[k] - L)];
+        if (!e->next_state) return 0;
+        acc = (acc << e->len) | e->code;
+        nacc += e->len;
+        x[k] = e->next_state;
+        if (nacc >= 32) {
+            nacc -= 32;
+            out_units[nu++] = (uint32_t)(acc >> nacc);
+            acc &= (nacc ? ((1ull << nacc) - 1) : 0);
+        }
+    }
+    out_units[nu++] = nacc ? (uint32_t)(acc << (32 - nacc)) : 0u;
+    *first_bit = nacc;
+    for (uint32_t k = 0; k < N; ++k) first_states[k] = x[k];
+    for (size_t a = 0, b = nu - 1; a < b; ++a, --b) {
+        uint32_t t = out_units[a]; out_units[a] = out_units[b]; out_units[b] = t;
+    }
+    return nu;
+}
+
+int mo_decode_interleaved(const uint32_t* units, size_t n_units, const uint32_t* const* tables,
+                          uint32_t L, uint32_t N, const uint32_t* first_states,
+                          uint32_t first_bit, uint8_t* out, size_t n_out, size_t n_total,
+                          uint32_t R, mo_il_record* records, uint32_t* end_states,
+                          uint64_t* end_bit) {
+    uint64_t p = 32u - first_bit;
+    uint32_t x[16];
+    if (N == 0 || N > 16) return -1;
+    for (uint32_t k = 0; k < N; ++k) x[k] = first_states[k];
+    size_t n_rec = R ? (n_units + R - 1) / R : 0, next_rec = 0;
+    for (size_t j = 0; j < n_out; ++j) {
+        while (next_rec < n_rec && p >= 32ull * R * next_rec) {
+            for (uint32_t k = 0; k < 2 && k < N; ++k) records[next_rec].state[k] = (uint16_t)(x[k] - L);
+            records[next_rec].bit = (uint32_t)(p - 32ull * R * next_rec);
+            records[next_rec].before = j;
+            ++next_rec;
+        }
+        const uint32_t k = (uint32_t)((n_total - 1 - j) % N);
+        uint8_t s;
+        if (mo_step(units, n_units, tables[k], L, &x[k], &p, &s)) return -1;
+        out[j] = s;
+    }
+    for (; next_rec < n_rec; ++next_rec) {      /* boundaries after the last step */
+        for (uint32_t k = 0; k < 2 && k < N; ++k) records[next_rec].state[k] = (uint16_t)(x[k] - L);
+        records[next_rec].bit = 0;
+        records[next_rec].before = n_out;
+    }
+    if (end_states) for (uint32_t k = 0; k < N; ++k) end_states[k] = x[k];
+    if (end_bit) *end_bit = p;
+    return 0;
+}
+
 /* ------------------------------------------------------- multicore decoder */
 /* Restatement of MulticoreDecoder (multicore_decoder.cc:17-333): T threads, each
  * owning an interval of subsequences (:275-303); a speculative pass from the

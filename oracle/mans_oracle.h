@@ -24,6 +24,11 @@
  *                        re-derivation of entries/counts on tiny streams.
  *   mo_subseq_last       brute-force re-derivation on tiny streams; one step from each
  *                        record lands on the next subsequence's entry (mo_decode_seq).
+ *   mo_encode_interleaved / mo_decode_interleaved (own format, no reference code)
+ *                        N = 1 equals mo_encode / mo_decode_seq unit for unit; N = 2, 3
+ *                        against an independent pure-Python encoder and decoder on tiny
+ *                        streams (one and two tables); round trip; end invariant (every
+ *                        slot back at L, bit 32 n_units); records by brute force.
  *   mo_decode_multicore  vs the compiled reference MulticoreDecoder (same S, T),
  *                        including the streams where the reference drops trailing
  *                        zero-bit symbols.
@@ -89,6 +94,30 @@ int mo_decode_seq(const uint32_t* units, size_t n_units, const uint32_t* table, 
 int mo_subseq_last(const uint32_t* units, size_t n_units, const uint32_t* table, uint32_t L,
                    uint32_t first_state, uint32_t first_bit, size_t n_out, uint32_t S,
                    uint32_t* last_state, uint64_t* last_bit);
+
+/* Interleaved-state streams (own format; the reference names multiple states / tables as
+ * future work, README.md:11).  N states (slots) share one bit stream: input symbol i is
+ * encoded by slot i mod N with encoder table enc[i mod N]; every slot starts at x = L;
+ * first_states[k] = slot k's final state; units in decoder order.  Decode step j
+ * (yielding input symbol n_total-1-j) runs the single-state step on slot
+ * (n_total-1-j) mod N with decoder table tables[that slot].  N = 1 is the single-state
+ * format (mo_encode / mo_decode_seq).
+ * Records (R > 0, N <= 2): one per R units, ceil(n_units / R): the slots' states (x - L)
+ * and the bit offset, within unit R*r, of the first step starting at or after it, and the
+ * steps before it (boundaries after the last step: the final states, bit 0, n_out). */
+typedef struct {
+    uint16_t state[2];
+    uint32_t bit;
+    uint64_t before;
+} mo_il_record;
+size_t mo_encode_interleaved(const uint8_t* in, size_t n, const mo_enc_entry* const* enc,
+                             uint32_t L, uint32_t N, uint32_t* out_units, uint32_t* first_bit,
+                             uint32_t* first_states);
+int mo_decode_interleaved(const uint32_t* units, size_t n_units, const uint32_t* const* tables,
+                          uint32_t L, uint32_t N, const uint32_t* first_states,
+                          uint32_t first_bit, uint8_t* out, size_t n_out, size_t n_total,
+                          uint32_t R, mo_il_record* records, uint32_t* end_states,
+                          uint64_t* end_bit);
 
 /* Restatement of MulticoreDecoder::decode (multicore_decoder.cc:17-90) with pthreads.
  * units must be padded by >= S+2 zero units after n_units (the reference pads 4 and

@@ -37,6 +37,12 @@ def _load_oracle():
                                   C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]
     lib.mo_subseq_last.argtypes = [_u32p, C.c_size_t, _u32p, C.c_uint32, C.c_uint32, C.c_uint32,
                                    C.c_size_t, C.c_uint32, _u32p, _u64p]
+    lib.mo_encode_interleaved.argtypes = [_u8p, C.c_size_t, C.c_void_p, C.c_uint32, C.c_uint32,
+                                          _u32p, C.POINTER(C.c_uint32), _u32p]
+    lib.mo_encode_interleaved.restype = C.c_size_t
+    lib.mo_decode_interleaved.argtypes = [_u32p, C.c_size_t, C.c_void_p, C.c_uint32, C.c_uint32,
+                                          _u32p, C.c_uint32, _u8p, C.c_size_t, C.c_size_t,
+                                          C.c_uint32, C.c_void_p, _u32p, C.POINTER(C.c_uint64)]
     lib.mo_decode_multicore.argtypes = [C.c_size_t, C.c_size_t, _u32p, C.c_size_t, _u32p,
                                         C.c_uint32, C.c_uint32, C.c_uint32, _u8p, C.c_size_t]
     return lib
@@ -162,6 +168,59 @@ def subseq_last(st, S, n_out=None):
     if rc:
         raise ValueError("subseq_last failed")
     return ls[:nsub], lb[:nsub]
+
+
+IL_RECORD_DTYPE = np.dtype([("state", "<u2", (2,)), ("bit", "<u4"), ("before", "<u8")])
+
+
+class ILStream:
+    """An interleaved-state stream: units (decoder order), first_bit, first_states[N],
+    decoder tables[N] (u32[L] each), n symbols."""
+
+    def __init__(self, units, first_bit, first_states, tables, L, n):
+        self.units, self.first_bit = units, first_bit
+        self.first_states = np.ascontiguousarray(first_states, np.uint32)
+        self.tables = [np.ascontiguousarray(t, np.uint32) for t in tables]
+        self.L, self.n, self.N = L, n, len(self.first_states)
+
+
+def encode_interleaved(symbols, N, counts_list, L):
+    """counts_list: one count vector per slot (the same vector N times = one shared
+    table).  Returns (ILStream, encoder table arrays)."""
+    symbols = np.ascontiguousarray(symbols, np.uint8)
+    encs, decs = [], []
+    for c in counts_list:
+        enc, _, dec = build_tables(c, L)
+        encs.append(enc)
+        decs.append(dec)
+    ptrs = (C.c_void_p * N)(*[C.cast(e, C.c_void_p) for e in encs])
+    n = len(symbols)
+    out = np.zeros(oracle().mo_max_units(n, L) + 2, np.uint32)
+    fb = C.c_uint32()
+    fs = np.zeros(N, np.uint32)
+    nu = oracle().mo_encode_interleaved(symbols, n, C.cast(ptrs, C.c_void_p), L, N, out,
+                                        C.byref(fb), fs)
+    if nu == 0:
+        raise ValueError("interleaved encode failed")
+    return ILStream(out[:nu].copy(), fb.value, fs, decs, L, n), encs
+
+
+def decode_interleaved(st, n_out=None, R=0):
+    """Sequential decode (decode order); with R > 0 also the records every R units."""
+    n_out = st.n if n_out is None else n_out
+    N = st.N
+    tabs = (C.c_void_p * N)(*[t.ctypes.data for t in st.tables])
+    out = np.zeros(max(n_out, 1), np.uint8)
+    nrec = (len(st.units) + R - 1) // R if R else 0
+    rec = np.zeros(max(nrec, 1), IL_RECORD_DTYPE)
+    es = np.zeros(N, np.uint32)
+    eb = C.c_uint64()
+    rc = oracle().mo_decode_interleaved(st.units, len(st.units), C.cast(tabs, C.c_void_p), st.L,
+                                        N, st.first_states, st.first_bit, out, n_out, st.n, R,
+                                        rec.ctypes.data if R else None, es, C.byref(eb))
+    if rc:
+        raise ValueError("interleaved decode failed")
+    return out[:n_out], rec[:nrec], es, eb.value
 
 
 def decode_multicore(st, S=4, T=4, n_out=None, pad=None):

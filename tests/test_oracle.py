@@ -270,3 +270,88 @@ def test_normalize_invariants(seed):
     assert ol.normalize(one, L)[200] == L
     with pytest.raises(ValueError):
         ol.normalize(np.ones(256, np.uint64), 128)
+
+
+def _py_bits(units):
+    bits = []
+    for u in units:
+        bits += [(int(u) >> i) & 1 for i in range(32)]
+    return bits + [0] * 64
+
+
+@pytest.mark.parametrize("lam", [0.3, 2.0])
+def test_interleaved_one_slot_is_the_single_format(lam):
+    """N = 1: the interleaved encoder and decoder are the single-state ones (pinned against
+    the reference's ANSEncoder and the sequential decoder)."""
+    sym = ol.gen_symbols(12, lam, 20000)
+    st1, counts = ol.encode(sym, L=1024)
+    il, _ = ol.encode_interleaved(sym, 1, [counts], 1024)
+    assert np.array_equal(il.units, st1.units)
+    assert (il.first_bit, int(il.first_states[0])) == (st1.first_bit, st1.first_state)
+    out, _, es, eb = ol.decode_interleaved(il)
+    ref, _, _, e1, b1 = ol.decode_seq(st1)
+    assert np.array_equal(out, ref) and (int(es[0]), eb) == (e1, b1)
+
+
+@pytest.mark.parametrize("N,shared", [(2, True), (2, False), (3, False)])
+def test_interleaved_bruteforce(N, shared):
+    """Interleaved encode and decode re-derived by an independent pure-Python encoder
+    (codes appended MSB-first, slot i mod N, every slot from L) and decoder (decode order,
+    slot (n-1-j) mod N) on tiny streams; round trip; every slot ends at L and the bit
+    cursor at 32 n_units; records (every R units) by brute force."""
+    L = 64
+    sym = ol.gen_symbols(40 + N, 0.5, 700)
+    hist = np.bincount(sym, minlength=256).astype(np.uint64)
+    c0 = ol.normalize(hist, L)
+    # other slots: the same support, the counts of the present symbols rotated
+    nz = np.nonzero(hist)[0]
+    others = []
+    for k in range(1, N):
+        h = np.zeros_like(hist)
+        h[nz] = np.roll(hist[nz], k)
+        others.append(ol.normalize(h, L))
+    counts = [c0] * N if shared else [c0] + others
+    il, encs = ol.encode_interleaved(sym, N, counts, L)
+    # Python encoder
+    earr = [np.frombuffer(e, dtype=np.uint32).reshape(256, L, 3) for e in encs]
+    x = [L] * N
+    bits = []
+    for i, s in enumerate(sym):
+        k = i % N
+        nx, code, ln = (int(v) for v in earr[k][s, x[k] - L])
+        bits += [(code >> (ln - 1 - b)) & 1 for b in range(ln)]
+        x[k] = nx
+    fb = len(bits) % 32
+    bits += [0] * ((32 - fb) % 32)
+    fwd = [int("".join(map(str, bits[32 * u:32 * u + 32])), 2) for u in range(len(bits) // 32)]
+    if fb == 0:
+        fwd.append(0)
+    assert il.first_bit == fb and list(il.first_states) == x
+    assert list(il.units) == fwd[::-1]
+    # Python decoder
+    ub = _py_bits(il.units)
+    p, xs, out = 32 - il.first_bit, list(il.first_states), []
+    R = 3
+    nrec = -(-len(il.units) // R)
+    rec_bf = [None] * nrec
+    for j in range(len(sym)):
+        for r in range(nrec):
+            if rec_bf[r] is None and p >= 32 * R * r:
+                rec_bf[r] = ([xs[0] - L, (xs[1] - L) if N > 1 else 0], p - 32 * R * r, j)
+        k = (len(sym) - 1 - j) % N
+        e = int(il.tables[k][xs[k] - L])
+        nb = e >> 24
+        out.append((e >> 16) & 0xFF)
+        xs[k] = ((e & 0xFFFF) << nb) | sum(ub[p + b] << b for b in range(nb))
+        p += nb
+    assert out == list(sym[::-1])
+    assert xs == [L] * N and p == 32 * len(il.units)
+    got, rec, es, eb = ol.decode_interleaved(il, R=R)
+    assert np.array_equal(got, np.array(out, np.uint8))
+    assert list(es) == [L] * N and eb == 32 * len(il.units)
+    for r in range(nrec):
+        st_bf, bit_bf, bef_bf = rec_bf[r] if rec_bf[r] is not None else ([0, 0], 0, len(sym))
+        if rec_bf[r] is None:
+            continue
+        assert list(rec[r]["state"][:N if N < 2 else 2]) == st_bf[:N if N < 2 else 2]
+        assert (int(rec[r]["bit"]), int(rec[r]["before"])) == (bit_bf, bef_bf)
