@@ -434,6 +434,63 @@ int mans_encode_device(const mans_encode_job* jobs, size_t n_jobs, void* workspa
 int mans_encode_results(const void* workspace, const mans_encode_job* jobs, size_t n_jobs,
                         void* stream, mans_encode_result* results);
 
+/* ------------------------------------------------- two-state interleaved streams
+ * The reference decodes one tANS state per stream and names multiple states and
+ * multiple tables as future work (README.md:11); this format is the library's own.
+ * Two states (slots) share one bit stream: input symbol i is encoded by slot i % 2 with
+ * that slot's table (the two tables may be the same); both slots start at x = L; the
+ * stream carries both final states.  Units are in decoder order, as for single streams;
+ * decode step j yields input symbol n-1-j from slot (n-1-j) % 2.
+ * Records, written by the encoder in its one forward pass, cut the stream into intervals
+ * of record_units units anchored at the stream's end: record m (m = 0 ..
+ * mans_il_record_count - 1) sits at unit boundary n_units - record_units * m and holds,
+ * for the first step whose start bit is at or after it, both slots' states (x - L), the
+ * bit offset from the boundary and the number of steps before it.  Record 0 is the
+ * end. */
+typedef struct mans_il_record {
+    uint16_t state[2];          /* x - L of slot 0 and slot 1 */
+    uint32_t bit;               /* start bit of the step, from the boundary's bit */
+    uint64_t before;            /* steps before it (output offset, decode order) */
+} mans_il_record;
+
+#define MANS_IL_MIN_RECORD_UNITS 16u
+#define MANS_IL_MAX_RECORD_UNITS (1u << 24)
+#define MANS_IL_WORKSPACE_BYTES 256u
+
+int mans_il_record_count(uint64_t n_units, uint32_t record_units, uint64_t* count);
+/* Host encoder.  enc0 / enc1: encoder tables of mans_build_tables for slot 0 and slot 1
+ * (may be the same).  units: mans_max_encoded_units(n) words; records:
+ * mans_il_record_count(mans_max_encoded_units(n), record_units) entries suffice
+ * (records_cap says how many there are); *n_records gets the count written. */
+int mans_encode_interleaved(const uint8_t* in, uint64_t n, const uint32_t* enc0,
+                            const uint32_t* enc1, uint32_t L, uint32_t record_units,
+                            uint32_t* units, uint64_t* n_units, uint32_t* first_bit,
+                            uint32_t* first_states, mans_il_record* records,
+                            uint64_t records_cap, uint64_t* n_records);
+
+typedef struct mans_il_args {
+    const uint32_t* units;      /* device, 16-byte aligned, readable to n_units rounded up to 4 */
+    uint64_t n_units;
+    uint8_t* out;               /* device, n_symbols bytes (decode order) */
+    uint64_t n_symbols;
+    const uint32_t* tables[2];  /* device, L reference CUHDCodetableItem words per slot */
+    uint32_t L;
+    uint32_t first_states[2];
+    uint32_t first_bit;
+    uint32_t record_units;      /* in [MANS_IL_MIN_RECORD_UNITS, MANS_IL_MAX_RECORD_UNITS] */
+    const mans_il_record* records;   /* device, mans_il_record_count(n_units, record_units) */
+    uint32_t flags;             /* MANS_F_SYNC_ERRORS: synchronise and report */
+    void* workspace;            /* device, >= MANS_IL_WORKSPACE_BYTES (status word) */
+    void* stream;               /* hipStream_t or NULL */
+} mans_il_args;
+
+/* Decodes every record interval in parallel and checks that each ends exactly on the next
+ * record (both states and the bit position) and the last on the stream's end (both
+ * states L, bit 32 n_units).  A mismatch, or records out of order, sets the status to
+ * MANS_ECORRUPT and a rejected table to MANS_ETABLE (mans_decode_status on `workspace`).
+ * Output bytes stay inside [0, n_symbols) whatever the records hold. */
+int mans_decode_interleaved(const mans_il_args* args);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,20 @@ class RefSync(C.Structure):
                 ("threads", C.c_uint32)]
 
 
+IL_RECORD_DTYPE = np.dtype([("state", "<u2", (2,)), ("bit", "<u4"), ("before", "<u8")])
+IL_MIN_RECORD_UNITS = 16
+IL_WORKSPACE_BYTES = 256
+
+
+class IlArgs(C.Structure):
+    """mans_il_args: a two-state interleaved stream on the device."""
+    _fields_ = [("units", C.c_void_p), ("n_units", C.c_uint64), ("out", C.c_void_p),
+                ("n_symbols", C.c_uint64), ("tables", C.c_void_p * 2), ("L", C.c_uint32),
+                ("first_states", C.c_uint32 * 2), ("first_bit", C.c_uint32),
+                ("record_units", C.c_uint32), ("records", C.c_void_p), ("flags", C.c_uint32),
+                ("workspace", C.c_void_p), ("stream", C.c_void_p)]
+
+
 N_KERNEL_EVENTS = 7
 KERNEL_NAMES = ["init", "sync", "check", "fixup", "scan", "write"]
 
@@ -125,6 +139,7 @@ EXPORTS = [
     "mans_sync_point_count", "mans_stream_range", "mans_sync_points_check",
     "mans_encode_workspace_bytes", "mans_encode_device", "mans_encode_results",
     "mans_gen_symbols_device", "mans_ref_sync_fill", "mans_decode_noalloc",
+    "mans_il_record_count", "mans_encode_interleaved", "mans_decode_interleaved",
 ]
 
 _lib = None
@@ -201,6 +216,10 @@ def lib():
         "mans_encode_results": ([vp, C.POINTER(EncodeJob), C.c_size_t, vp,
                                  C.POINTER(EncodeResult)], i32),
         "mans_sync_points_check": ([vp, u64, u32], i32),
+        "mans_il_record_count": ([u64, u32, C.POINTER(u64)], i32),
+        "mans_encode_interleaved": ([vp, u64, vp, vp, u32, u32, vp, C.POINTER(u64),
+                                     C.POINTER(u32), vp, vp, u64, C.POINTER(u64)], i32),
+        "mans_decode_interleaved": ([C.POINTER(IlArgs)], i32),
         "mans_stream_range": ([C.POINTER(StreamDesc), vp, vp, u64, u64, C.POINTER(StreamDesc),
                                C.POINTER(u64)], i32),
     }
@@ -261,6 +280,27 @@ def encode(symbols, enc, L):
     check(lib().mans_encode(_ptr(symbols), n, _ptr(enc), L, _ptr(units), C.byref(nu), C.byref(fb),
                             C.byref(fs)))
     return units[:nu.value].copy(), fb.value, fs.value
+
+
+def il_record_count(n_units, record_units):
+    c = C.c_uint64()
+    check(lib().mans_il_record_count(n_units, record_units, C.byref(c)))
+    return c.value
+
+
+def encode_interleaved(symbols, enc0, enc1, L, record_units):
+    """mans_encode_interleaved: (units, first_bit, first_states[2], records)."""
+    symbols = np.ascontiguousarray(symbols, np.uint8)
+    n = len(symbols)
+    cap_units = lib().mans_max_encoded_units(n)
+    units = np.empty(cap_units, np.uint32)
+    records = np.zeros(il_record_count(cap_units, record_units), IL_RECORD_DTYPE)
+    nu, fb, nrec = C.c_uint64(), C.c_uint32(), C.c_uint64()
+    fs = np.zeros(2, np.uint32)
+    check(lib().mans_encode_interleaved(_ptr(symbols), n, _ptr(enc0), _ptr(enc1), L, record_units,
+                                        _ptr(units), C.byref(nu), C.byref(fb), _ptr(fs),
+                                        _ptr(records), len(records), C.byref(nrec)))
+    return units[:nu.value].copy(), fb.value, fs, records[:nrec.value].copy()
 
 
 def make_stream(seed, lam, n, L, keep_symbols=True):

@@ -22,7 +22,8 @@ ARCH = "gfx950"
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-LIB_SOURCES = ["mans_kernels.hip", "mans_tables.hip", "mans_encode.hip", "mans_runtime.hip",
+LIB_SOURCES = ["mans_kernels.hip", "mans_tables.hip", "mans_encode.hip", "mans_interleaved.hip",
+               "mans_runtime.hip",
                "mans_codec.cc", "mans_container.cc"]
 CUHD_SOURCES = ["cuhd_gpu_decoder.cc", "cuhd_buffers.cc", "ans_compat.cc"]
 

@@ -293,7 +293,117 @@ int mans_encode(const uint8_t* in, uint64_t n, const uint32_t* enc, uint32_t L, 
     return MANS_OK;
 }
 
+int mans_il_record_count(uint64_t n_units, uint32_t record_units, uint64_t* count) {
+    if (!count) return set_error(MANS_EINVAL, "count is NULL");
+    if (record_units < MANS_IL_MIN_RECORD_UNITS || record_units > MANS_IL_MAX_RECORD_UNITS)
+        return set_error(MANS_EINVAL, "record_units=%u outside [%u, %u]", record_units,
+                         MANS_IL_MIN_RECORD_UNITS, MANS_IL_MAX_RECORD_UNITS);
+    *count = (n_units + record_units - 1) / record_units;
+    return MANS_OK;
+}
+
 }  // extern "C"
+
+namespace {
+
+// One slot's encoder: the compact form when the table has its shape, else the full table.
+struct SlotEnc {
+    const uint32_t* full;
+    uint32_t L;
+    CompactEnc c;
+    bool compact;
+    // codeword of s from state x: returns its length (0 = s has no code), sets *code, *x
+    inline int step(uint32_t s, uint32_t& x, uint32_t& code) const {
+        if (compact) {
+            if (!c.f[s]) return -1;
+            const uint32_t nb = c.m[s] - (x < c.thr[s] ? 1u : 0u);
+            code = x & ((1u << nb) - 1u);
+            x = c.ys[c.base[s] + (x >> nb) - c.f[s]];
+            return (int)nb;
+        }
+        const uint32_t* e = full + 3ull * ((uint64_t)s * L + (x - L));
+        if (!e[0]) return -1;
+        code = e[1];
+        x = e[0];
+        return (int)e[2];
+    }
+};
+
+}  // namespace
+
+extern "C" int mans_encode_interleaved(const uint8_t* in, uint64_t n, const uint32_t* enc0,
+                                       const uint32_t* enc1, uint32_t L, uint32_t record_units,
+                                       uint32_t* units, uint64_t* n_units, uint32_t* first_bit,
+                                       uint32_t* first_states, mans_il_record* records,
+                                       uint64_t records_cap, uint64_t* n_records) {
+    if ((!in && n) || !enc0 || !enc1 || !units || !n_units || !first_bit || !first_states ||
+        !records || !n_records)
+        return set_error(MANS_EINVAL, "NULL argument");
+    if (L < 2u || L > 16384u || (L & (L - 1u)))
+        return set_error(MANS_EINVAL, "L=%u is not a power of two in [2, 16384]", L);
+    uint64_t rc_max = 0;
+    int rc = mans_il_record_count(0, record_units, &rc_max);
+    if (rc) return rc;
+    SlotEnc se[2];
+    const uint32_t* enc[2] = {enc0, enc1};
+    for (int k = 0; k < 2; ++k) {
+        se[k].full = enc[k];
+        se[k].L = L;
+        se[k].compact = n >= 4096u && compact_encoder(enc[k], L, se[k].c);
+    }
+    // Forward pass.  E counts the bits emitted so far; record m belongs to forward bit
+    // F_m = 32 R m: it is taken from the state before the first symbol whose codeword
+    // would end beyond F_m (the last symbol ending at or before F_m is the first the
+    // decoder meets at or after the mirrored boundary), so a record is written the moment
+    // a codeword straddles or starts at its F_m.
+    const uint64_t span = 32ull * record_units;
+    uint64_t acc = 0, E = 0, F = 0, m = 0;
+    uint32_t fill = 0, x[2] = {L, L};
+    uint64_t nu = 0;
+    auto put = [&](uint64_t before, uint32_t bit) -> bool {
+        if (m >= records_cap) return false;
+        records[m].state[0] = (uint16_t)(x[0] - L);
+        records[m].state[1] = (uint16_t)(x[1] - L);
+        records[m].bit = bit;
+        records[m].before = before;
+        ++m;
+        F += span;
+        return true;
+    };
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint32_t k = (uint32_t)(i & 1u);
+        uint32_t xs = x[k], code = 0;
+        const int nb = se[k].step(in[i], xs, code);
+        if (nb < 0) return set_error(MANS_EINVAL, "symbol %u has no code in slot %u", in[i], k);
+        if (E + (uint64_t)nb > F && !put(n - i, (uint32_t)(F - E)))
+            return set_error(MANS_EINVAL, "records_cap %llu too small", (unsigned long long)records_cap);
+        x[k] = xs;
+        acc = (acc << nb) | code;
+        fill += (uint32_t)nb;
+        E += (uint64_t)nb;
+        if (fill >= 32) {
+            fill -= 32;
+            units[nu++] = (uint32_t)(acc >> fill);
+            acc &= (1ull << fill) - 1;
+        }
+    }
+    units[nu++] = fill ? (uint32_t)(acc << (32 - fill)) : 0u;
+    // boundaries at or after the last codeword's end: the decoder's first step
+    const uint64_t M = (nu + record_units - 1) / record_units;
+    while (m < M) {
+        if (!put(0, (uint32_t)(F - E)))
+            return set_error(MANS_EINVAL, "records_cap %llu too small", (unsigned long long)records_cap);
+    }
+    if (m != M) return set_error(MANS_EINVAL, "record count %llu != %llu", (unsigned long long)m,
+                                 (unsigned long long)M);
+    std::reverse(units, units + nu);
+    *n_units = nu;
+    *first_bit = fill;
+    first_states[0] = x[0];
+    first_states[1] = x[1];
+    *n_records = m;
+    return MANS_OK;
+}
 
 extern "C" int mans_make_stream(uint64_t seed, double lambda, uint64_t n, uint32_t L,
                                 uint8_t* symbols_out, uint32_t* units_out, uint64_t units_cap,

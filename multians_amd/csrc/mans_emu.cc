@@ -323,3 +323,67 @@ int mans_emu_decode_index(const uint32_t* units, uint64_t n_units, uint8_t* out,
 }
 
 }  // extern "C"
+
+// ------------------------------------------------ two-state interleaved streams
+// The lanes of k_il_decode run one after another through the same per-lane code
+// (mans_il_lane.h), with a host array standing in for LDS: tables at dword kT0 (slot 0)
+// and after it (slot 1), the ring after both.
+#include "mans_il_lane.h"
+
+namespace {
+
+struct HostMem {
+    uint32_t* lds;
+    uint32_t ld(uint32_t a) const { return lds[a >> 2]; }
+    void st(uint32_t a, uint32_t v) const { lds[a >> 2] = v; }
+    void out8(uint8_t* p, uint32_t v) const { *p = (uint8_t)v; }
+    void out32(uint8_t* p, uint32_t v) const { std::memcpy(p, &v, 4); }
+    void out128(uint8_t* p, uint32_t a, uint32_t b, uint32_t c, uint32_t d) const {
+        const uint32_t v[4] = {a, b, c, d};
+        std::memcpy(p, v, 16);
+    }
+};
+
+}  // namespace
+
+extern "C" int mans_emu_il_decode(const uint32_t* units, uint64_t n_units, const uint32_t* tab0,
+                                  const uint32_t* tab1, uint32_t L, const uint32_t* first_states,
+                                  uint32_t first_bit, uint64_t n, uint32_t R,
+                                  const mans_il_record* rec, uint8_t* out, uint32_t* lane_status) {
+    constexpr uint32_t kT0 = 36;      // a nonzero table base, as in LDS after static words
+    const uint32_t tw = (L + 3u) & ~3u;
+    std::vector<uint32_t> lds(kT0 + 2u * tw + kIlRingStride + 4u, 0xDEADBEEFu);
+    const bool two = tab0 != tab1;
+    const uint32_t t1 = two ? kT0 + tw : kT0;
+    int bad = 0, cyc = 0;
+    for (uint32_t k = 0; k < (two ? 2u : 1u); ++k) {
+        const uint32_t tb = k ? t1 : kT0;
+        const uint32_t* t = k ? tab1 : tab0;
+        for (uint32_t i = 0; i < L; ++i) lds[tb + i] = derive_entry(t[i], L, i, &bad, &cyc) + (tb << 16);
+    }
+    if (bad) return -4;
+    IlArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.units = units;
+    a.tab[0] = tab0;
+    a.tab[1] = tab1;
+    a.two_tables = two;
+    a.rec = rec;
+    a.out = out;
+    a.n = n;
+    a.n_units = (uint32_t)n_units;
+    a.L = L;
+    a.first_state[0] = first_states[0];
+    a.first_state[1] = first_states[1];
+    a.first_bit = first_bit;
+    a.R = R;
+    a.M = (uint32_t)((n_units + R - 1) / R);
+    HostMem m{lds.data()};
+    uint32_t all = 0;
+    for (uint32_t q = 0; q <= a.M; ++q) {
+        const uint32_t f = il_lane(a, q, m, kT0, t1, 4u * (kT0 + (two ? 2u : 1u) * tw));
+        if (lane_status) lane_status[q] = f;
+        all |= f;
+    }
+    return (int)all;
+}

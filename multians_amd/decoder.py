@@ -147,6 +147,93 @@ class DeviceStream:
         return s.value
 
 
+@dataclass
+class InterleavedEncoded:
+    """A two-state interleaved stream (mans_encode_interleaved) on the host."""
+    units: np.ndarray          # u32, decoder order
+    first_bit: int
+    first_states: np.ndarray   # u32[2], slot 0 and slot 1
+    tables: tuple              # two u32[L] decoder tables (the same array when shared)
+    L: int
+    n_symbols: int
+    record_units: int
+    records: np.ndarray        # _lib.IL_RECORD_DTYPE
+
+    @property
+    def n_units(self):
+        return len(self.units)
+
+
+def encode_interleaved(symbols, L, record_units=256, per_slot_tables=False):
+    """Tables (one shared from the whole histogram, or one per slot from the histogram of
+    that slot's symbols: even input positions for slot 0, odd for slot 1) -> host encode
+    with records."""
+    symbols = np.ascontiguousarray(symbols, np.uint8)
+    parts = [symbols[0::2], symbols[1::2]] if per_slot_tables else [symbols]
+    encs, decs = [], []
+    for p in parts:
+        src = p if len(p) else symbols[:1]
+        hist = np.bincount(src, minlength=256).astype(np.uint64)
+        if not hist.any():
+            hist[0] = 1
+        enc, dec = _lib.build_tables(_lib.normalize_counts(hist, L), L)
+        encs.append(enc)
+        decs.append(dec)
+    if len(encs) == 1:
+        encs.append(encs[0])
+        decs.append(decs[0])
+    units, fb, fs, rec = _lib.encode_interleaved(symbols, encs[0], encs[1], L, record_units)
+    return InterleavedEncoded(units, fb, fs, (decs[0], decs[1]), L, len(symbols), record_units, rec)
+
+
+class DeviceInterleaved:
+    """An interleaved stream resident in device memory; decode() runs
+    mans_decode_interleaved into `out` (decode order)."""
+
+    def __init__(self, ie: InterleavedEncoded, device="cuda"):
+        torch = _torch()
+        self.ie = ie
+        self.units = torch.from_numpy(ie.units.view(np.int32)).to(device)
+        self.tables = [torch.from_numpy(ie.tables[0].view(np.int32)).to(device)]
+        if ie.tables[1] is ie.tables[0]:
+            self.tables.append(self.tables[0])
+        else:
+            self.tables.append(torch.from_numpy(ie.tables[1].view(np.int32)).to(device))
+        rec = ie.records if len(ie.records) else np.zeros(1, _lib.IL_RECORD_DTYPE)
+        self.records = torch.from_numpy(rec.view(np.uint8).copy()).to(device)
+        self.out = torch.zeros(max(ie.n_symbols, 1), dtype=torch.uint8, device=device)
+        self.ws = torch.zeros(_lib.IL_WORKSPACE_BYTES, dtype=torch.uint8, device=device)
+
+    def args(self, stream=None, flags=0, records=None):
+        torch = _torch()
+        a = _lib.IlArgs()
+        a.units, a.n_units = self.units.data_ptr(), self.ie.n_units
+        a.out, a.n_symbols = self.out.data_ptr(), self.ie.n_symbols
+        a.tables[0], a.tables[1] = self.tables[0].data_ptr(), self.tables[1].data_ptr()
+        a.L = self.ie.L
+        a.first_states[0], a.first_states[1] = int(self.ie.first_states[0]), int(self.ie.first_states[1])
+        a.first_bit = self.ie.first_bit
+        a.record_units = self.ie.record_units
+        a.records = (records if records is not None else self.records).data_ptr()
+        a.flags = flags
+        a.workspace = self.ws.data_ptr()
+        a.stream = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        return a
+
+    def decode(self, **kw):
+        a = self.args(**kw)
+        _lib.check(_lib.lib().mans_decode_interleaved(C.byref(a)))
+        return self.out
+
+    def status(self, stream=None):
+        torch = _torch()
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        s = C.c_int()
+        _lib.check(_lib.lib().mans_decode_status(C.c_void_p(self.ws.data_ptr()), C.c_void_p(st),
+                                                 C.byref(s)))
+        return s.value
+
+
 def plan_windows(full_descs, records, windows):
     """Sub-stream descriptors for windows [(stream, first, last), ...] (decode order):
     full_descs[stream] has chunk_units and its sync points on the device in index_in,

@@ -313,25 +313,25 @@ int mo_decode_interleaved(const uint32_t* units, size_t n_units, const uint32_t*
     uint64_t p = 32u - first_bit;
     uint32_t x[16];
     if (N == 0 || N > 16) return -1;
+    if (R && (N > 2 || n_out != n_total)) return -1;
     for (uint32_t k = 0; k < N; ++k) x[k] = first_states[k];
-    size_t n_rec = R ? (n_units + R - 1) / R : 0, next_rec = 0;
-    for (size_t j = 0; j < n_out; ++j) {
-        while (next_rec < n_rec && p >= 32ull * R * next_rec) {
-            for (uint32_t k = 0; k < 2 && k < N; ++k) records[next_rec].state[k] = (uint16_t)(x[k] - L);
-            records[next_rec].bit = (uint32_t)(p - 32ull * R * next_rec);
-            records[next_rec].before = j;
-            ++next_rec;
+    /* records are crossed in decreasing m (increasing T_m) */
+    int64_t m = R ? (int64_t)((n_units + R - 1) / R) - 1 : -1;
+    for (size_t j = 0; j <= n_out; ++j) {
+        while (m >= 0 && p >= 32ull * (n_units - (uint64_t)R * (uint64_t)m)) {
+            for (uint32_t k = 0; k < 2; ++k)
+                records[m].state[k] = (uint16_t)(k < N ? x[k] - L : 0u);
+            records[m].bit = (uint32_t)(p - 32ull * (n_units - (uint64_t)R * (uint64_t)m));
+            records[m].before = j;
+            --m;
         }
+        if (j == n_out) break;
         const uint32_t k = (uint32_t)((n_total - 1 - j) % N);
         uint8_t s;
         if (mo_step(units, n_units, tables[k], L, &x[k], &p, &s)) return -1;
         out[j] = s;
     }
-    for (; next_rec < n_rec; ++next_rec) {      /* boundaries after the last step */
-        for (uint32_t k = 0; k < 2 && k < N; ++k) records[next_rec].state[k] = (uint16_t)(x[k] - L);
-        records[next_rec].bit = 0;
-        records[next_rec].before = n_out;
-    }
+    if (m >= 0) return -1;      /* a record past the stream's end: not a valid stream */
     if (end_states) for (uint32_t k = 0; k < N; ++k) end_states[k] = x[k];
     if (end_bit) *end_bit = p;
     return 0;

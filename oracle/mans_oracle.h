@@ -102,9 +102,13 @@ int mo_subseq_last(const uint32_t* units, size_t n_units, const uint32_t* table,
  * (yielding input symbol n_total-1-j) runs the single-state step on slot
  * (n_total-1-j) mod N with decoder table tables[that slot].  N = 1 is the single-state
  * format (mo_encode / mo_decode_seq).
- * Records (R > 0, N <= 2): one per R units, ceil(n_units / R): the slots' states (x - L)
- * and the bit offset, within unit R*r, of the first step starting at or after it, and the
- * steps before it (boundaries after the last step: the final states, bit 0, n_out). */
+ * Records (R > 0, N <= 2, n_out = n_total), anchored at the stream's end so that an
+ * encoder can emit them in its one forward pass: record m (m = 0 .. ceil(n_units/R) - 1)
+ * is at unit boundary T_m = n_units - R*m (bit 32*T_m of the decoder's stream, forward
+ * bit 32*R*m of the encoder's).  It holds, for the first step j (0 <= j <= n_total; j =
+ * n_total is the end) whose start bit p_j >= 32*T_m: the slots' states before it (x - L),
+ * bit = p_j - 32*T_m, and before = j.  Record 0 is the end (usually states L, bit 0,
+ * before n_total; earlier when the last decoded symbols take zero bits). */
 typedef struct {
     uint16_t state[2];
     uint32_t bit;

@@ -328,30 +328,31 @@ def test_interleaved_bruteforce(N, shared):
         fwd.append(0)
     assert il.first_bit == fb and list(il.first_states) == x
     assert list(il.units) == fwd[::-1]
-    # Python decoder
+    # Python decoder; positions and states before every step (j = n: the end)
     ub = _py_bits(il.units)
     p, xs, out = 32 - il.first_bit, list(il.first_states), []
-    R = 3
-    nrec = -(-len(il.units) // R)
-    rec_bf = [None] * nrec
+    trace = []
     for j in range(len(sym)):
-        for r in range(nrec):
-            if rec_bf[r] is None and p >= 32 * R * r:
-                rec_bf[r] = ([xs[0] - L, (xs[1] - L) if N > 1 else 0], p - 32 * R * r, j)
+        trace.append((p, list(xs)))
         k = (len(sym) - 1 - j) % N
         e = int(il.tables[k][xs[k] - L])
         nb = e >> 24
         out.append((e >> 16) & 0xFF)
         xs[k] = ((e & 0xFFFF) << nb) | sum(ub[p + b] << b for b in range(nb))
         p += nb
+    trace.append((p, list(xs)))
     assert out == list(sym[::-1])
     assert xs == [L] * N and p == 32 * len(il.units)
+    R = 3 if N <= 2 else 0          # records hold two slots
     got, rec, es, eb = ol.decode_interleaved(il, R=R)
     assert np.array_equal(got, np.array(out, np.uint8))
     assert list(es) == [L] * N and eb == 32 * len(il.units)
-    for r in range(nrec):
-        st_bf, bit_bf, bef_bf = rec_bf[r] if rec_bf[r] is not None else ([0, 0], 0, len(sym))
-        if rec_bf[r] is None:
-            continue
-        assert list(rec[r]["state"][:N if N < 2 else 2]) == st_bf[:N if N < 2 else 2]
-        assert (int(rec[r]["bit"]), int(rec[r]["before"])) == (bit_bf, bef_bf)
+    # records anchored at the end: record m at unit n_units - R m, first step at/after it
+    nu = len(il.units)
+    assert len(rec) == (-(-nu // R) if R else 0)
+    for m in range(len(rec)):
+        T = 32 * (nu - R * m)
+        j = next(j for j, (pj, _) in enumerate(trace) if pj >= T)
+        pj, xj = trace[j]
+        assert [int(v) for v in rec[m]["state"][:min(N, 2)]] == [v - L for v in xj[:2]]
+        assert (int(rec[m]["bit"]), int(rec[m]["before"])) == (pj - T, j)
