@@ -589,6 +589,43 @@ def bench_stress(args, torch):
     return out
 
 
+def bench_interleaved(args, torch):
+    """Two-state interleaved streams (the library's own format; the reference names
+    multiple states / tables as future work): 2^log2n symbols for lambda = 0.2 and 1.0,
+    L = 4096, one shared table, records every 256 units, host-encoded; bit-exact on the
+    device, then the median of 10 decodes (HIP events).  `ratio` counts the records."""
+    from multians_amd import _lib
+    from multians_amd.decoder import DeviceInterleaved, encode_interleaved
+    n = 1 << args.log2n
+    out = []
+    for i, lam in enumerate([0.2, 1.0]):
+        t0 = time.time()
+        sym = _lib.gen_symbols(5000 + i, lam, n)
+        ie = encode_interleaved(sym, 4096, record_units=256)
+        d = DeviceInterleaved(ie)
+        log("interleaved lambda=%g: %d units, %d records, encoded on the host in %.1fs" %
+            (lam, ie.n_units, len(ie.records), time.time() - t0))
+        d.decode()
+        torch.cuda.synchronize()
+        ref = torch.from_numpy(sym).cuda()
+        ok = d.status() == 0 and bool(torch.equal(d.out, ref.flip(0)))
+        del ref
+        if not ok:
+            raise SystemExit("interleaved decode mismatch (lambda %g)" % lam)
+        for _ in range(2):
+            d.decode()
+        ms = sorted(time_events(torch, d.decode, 1) for _ in range(10))[5]
+        comp = 4.0 * ie.n_units + 16.0 * len(ie.records)
+        out.append({"lambda": lam, "symbols": n, "record_units": 256,
+                    "ratio": round(n / comp, 3), "ms": round(ms, 4),
+                    "decoded_GBps": round(n / ms / 1e6, 2),
+                    "hbm_frac": round((comp + n) / ms / 1e6 / HBM_PEAK_GBS, 4), "bit_exact": True})
+        log("interleaved", json.dumps(out[-1]))
+        del d, ie, sym
+        torch.cuda.empty_cache()
+    return out
+
+
 def bench_served(args, torch):
     """Served configuration: open-loop Poisson arrivals at 70% of the measured served
     capacity on the request mix (L 1024/4096 50/50, uint8 sizes log-uniform 1 KiB-64 MiB,
@@ -742,6 +779,8 @@ def main():
                     help="also build decoder tables on the device for the first N streams")
     ap.add_argument("--no-stress", action="store_true",
                     help="skip the two 2^32-symbol streams (lambda 0.002 and 3.0)")
+    ap.add_argument("--no-interleaved", action="store_true",
+                    help="skip the two-state interleaved leg")
     ap.add_argument("--no-c1", action="store_true",
                     help="skip the 4 MiB CPU-runnable configuration (GPU vs multicore)")
     ap.add_argument("--no-plain", action="store_true",
@@ -819,6 +858,8 @@ def main():
         res["device_tables"] = bench_tables(args, torch)
     if not args.no_stress:
         res["stress"] = bench_stress(args, torch)
+    if not args.no_interleaved:
+        res["interleaved"] = bench_interleaved(args, torch)
     if not args.no_c1:
         res["c1_plain"] = bench_plain_c1(args, torch)
     if args.serve_seconds > 0:

@@ -180,6 +180,24 @@ MANS_HD U4 load_block(const uint32_t* units, uint32_t n, uint32_t b) {
     return r;
 }
 
+// The same block with the address clamped into the stream but the values not masked, for
+// prefetches: units past the end are only ever read as the unused top of a window (a
+// stream's codes end at bit 32 n, and its true path consumes no bit beyond them; any
+// other path is discarded whatever it reads), and masking the values would make the
+// prefetch wait for its data as soon as it is issued.
+MANS_HD U4 load_block_raw(const uint32_t* units, uint32_t n, uint32_t b) {
+    const uint32_t ba = b < n ? b : 0u;
+    U4 r;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint4 v = ((gptr_u4)units)[ba >> 2];
+    r.x = v.x; r.y = v.y; r.z = v.z; r.w = v.w;
+#else
+    r.x = units[ba]; r.y = ba + 1u < n ? units[ba + 1u] : 0u;
+    r.z = ba + 2u < n ? units[ba + 2u] : 0u; r.w = ba + 3u < n ? units[ba + 3u] : 0u;
+#endif
+    return r;
+}
+
 MANS_HD uint32_t pick4(const U4& v, uint32_t k) {
     return k == 0u ? v.x : (k == 1u ? v.y : (k == 2u ? v.z : v.w));
 }

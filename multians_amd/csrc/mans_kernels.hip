@@ -275,9 +275,10 @@ __device__ __forceinline__ void round_load(const RoundIO& io, uint32_t r, Prefet
     for (uint32_t j = 0; j < kPieces; ++j) {
         uint32_t lane, piece;
         piece_of(j, &lane, &piece);
+        // lanes without a load (rbase kNone) read an in-stream block they never use;
+        // no branch and no mask, so the loads stay in flight while the round decodes
         const uint32_t rb = io.rbase[lane];
-        U4 u{0u, 0u, 0u, 0u};
-        if (rb != kNone) u = load_block(io.units, io.n, rb + r * kR + 4u * piece);
+        const U4 u = load_block_raw(io.units, io.n, rb != kNone ? rb + r * kR + 4u * piece : 0u);
         pf.v[j] = make_uint4(u.x, u.y, u.z, u.w);
     }
 }
@@ -1071,7 +1072,7 @@ __device__ __forceinline__ void pass_load(const uint32_t* units, uint32_t n, uin
     for (uint32_t j = 0; j < kWPieces; ++j) {
         const uint32_t idx = threadIdx.x + kWG * j;
         const uint32_t lane = idx / kWPieces, piece = idx - lane * kWPieces;
-        const U4 u = load_block(units, n, u_first + lane * kWI + 4u * piece);
+        const U4 u = load_block_raw(units, n, u_first + lane * kWI + 4u * piece);
         pf.v[j] = make_uint4(u.x, u.y, u.z, u.w);
     }
 }
