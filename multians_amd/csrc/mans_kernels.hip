@@ -247,7 +247,14 @@ __global__ void k_init(KArgs a, uint32_t n_status) {
 #endif
 constexpr uint32_t kR = MANS_ROUND_UNITS;  // units decoded per round (= checkpoint stride)
 constexpr uint32_t kRL = kR + 4;   // units staged per lane per round (kR + lookahead)
-constexpr uint32_t kLS = kRL;      // LDS stride per lane in units (16-byte multiple)
+// LDS stride per lane in units: the round plus the two units its last window can reach
+// (p stays below (kR + 1) * 32 after the last pair, see round_decode); the rest of the
+// last 16-byte load is not stored.  18 is 2 mod 4: lanes at the same offset of their segments
+// (lockstep rounds keep them close) spread over twice the banks a 16-byte multiple would
+// give, the segments take 8-byte halves (ds_write2_b64), and four sync workgroups fit a
+// CU's LDS.
+constexpr uint32_t kLS = kR + 2u;
+static_assert(kLS <= kRL && kLS % 4u == 2u, "round segment stride");
 constexpr uint32_t kPieces = kRL / 4;   // 16-byte loads per lane segment
 static_assert(kR % 4 == 0 && kR >= 8, "round length");
 
@@ -288,7 +295,9 @@ __device__ __forceinline__ void round_store(const RoundIO& io, const Prefetch& p
     for (uint32_t j = 0; j < kPieces; ++j) {
         uint32_t lane, piece;
         piece_of(j, &lane, &piece);
-        *reinterpret_cast<uint4*>(io.buf + lane * kLS + 4u * piece) = pf.v[j];
+        uint2* d = reinterpret_cast<uint2*>(io.buf + lane * kLS + 4u * piece);
+        if (4u * piece < kLS) d[0] = make_uint2(pf.v[j].x, pf.v[j].y);
+        if (4u * piece + 2u < kLS) d[1] = make_uint2(pf.v[j].z, pf.v[j].w);
     }
 }
 
@@ -1057,7 +1066,9 @@ constexpr uint32_t kWF = kR >= 32u ? 1u : 32u / kR;
 constexpr uint32_t kWI = kR * kWF;          // units per item
 constexpr uint32_t kWPieces = kWI / 4u + 1u;  // 16-byte loads per lane (item + lookahead)
 // LDS stride per lane: the item plus the two units a step may look ahead (8-byte
-// aligned, stored as 8-byte halves); the shorter stride lets three workgroups share a CU
+// aligned, stored as 8-byte halves; 2 mod 4, which spreads lanes at the same offset over
+// twice the banks a 16-byte multiple would); the shorter stride lets three workgroups
+// share a CU
 constexpr uint32_t kWL = kWI + 2u;
 
 __host__ __device__ __forceinline__ uint32_t n_write_items(uint32_t n_ck) {
