@@ -173,11 +173,24 @@ MANS_HD uint32_t il_step2(IlLane& s, const Mem& m, const uint32_t* units, uint32
     return two;
 }
 
+// 16 symbols (8 pairs) as four dwords.
+template <class Mem>
+MANS_HD void il_quad(IlLane& s, Mem& m, const uint32_t* units, uint32_t n_units, uint32_t* v) {
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t x = il_step2(s, m, units, n_units);
+        const uint32_t y = il_step2(s, m, units, n_units);
+        v[k] = il_join(y, x);
+    }
+}
+
 // Decodes cnt steps into dst: bytes up to 4-byte alignment, dwords up to 16-byte
-// alignment, 16-byte stores, then the tail.  Block checks: after the head (at most 3
-// single steps and 3 dwords, 225 bits), after the 16 symbols of each 16-byte store but
-// before issuing it (240 bits), and after the tail (225 bits; the next check may be a
-// head later, 450 bits).
+// alignment, 16-byte stores up to 64-byte alignment, then whole 64-byte halves of lines
+// held in registers and stored back to back (a lane's half line reaches the L2 at once
+// instead of over the time its 64 symbols take, which would leave it partly written while
+// thousands of lanes each hold one open line), then 16-byte stores, dwords and bytes.
+// Block checks, at most 240 bits apart: after the head (at most 3 single steps and
+// 3 dwords, 225 bits), after every 16 symbols (the last one of a line just before its
+// stores), and after the tail (225 bits; the next check may be a head later, 450 bits).
 template <class Mem>
 MANS_HD void il_emit(IlLane& s, Mem& m, const uint32_t* units, uint32_t n_units, uint8_t* dst,
                      uint32_t cnt) {
@@ -197,18 +210,33 @@ MANS_HD void il_emit(IlLane& s, Mem& m, const uint32_t* units, uint32_t n_units,
     il_check(s, m, units, n_units);
     dw += 4u * ha;
     nd -= ha;
-    const uint32_t nq = nd >> 2;
-    for (uint32_t d = 0; d < nq; ++d) {
+    uint32_t nq = nd >> 2;                          // 16-byte blocks
+    const uint32_t wl = (uint32_t)(((64u - ((uintptr_t)dw & 63u)) & 63u) >> 4);
+    const uint32_t hl = wl < nq ? wl : nq;
+    for (uint32_t d = 0; d < hl; ++d) {
         uint32_t v[4];
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t x = il_step2(s, m, units, n_units);
-            const uint32_t y = il_step2(s, m, units, n_units);
-            v[k] = il_join(y, x);
-        }
+        il_quad(s, m, units, n_units, v);
         il_check(s, m, units, n_units);
-        m.out128(dw + 16u * d, v[0], v[1], v[2], v[3]);
+        m.out128(dw, v[0], v[1], v[2], v[3]);
+        dw += 16;
     }
-    dw += 16u * nq;
+    nq -= hl;
+    for (uint32_t l = nq >> 2; l; --l) {
+        uint32_t v[16];
+        for (int k = 0; k < 4; ++k) {
+            il_quad(s, m, units, n_units, v + 4 * k);
+            il_check(s, m, units, n_units);
+        }
+        for (int k = 0; k < 4; ++k) m.out128(dw + 16 * k, v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+        dw += 64;
+    }
+    for (uint32_t d = 0; d < (nq & 3u); ++d) {
+        uint32_t v[4];
+        il_quad(s, m, units, n_units, v);
+        il_check(s, m, units, n_units);
+        m.out128(dw, v[0], v[1], v[2], v[3]);
+        dw += 16;
+    }
     for (uint32_t d = 0; d < (nd & 3u); ++d) {
         const uint32_t x = il_step2(s, m, units, n_units);
         const uint32_t y = il_step2(s, m, units, n_units);
