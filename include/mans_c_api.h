@@ -288,6 +288,19 @@ int mans_graph_destroy(void* graph);
 #define MANS_WARM_AUTO 0xFFFFFFFFu
 #define MANS_RULE_SINGLE 1u
 #define MANS_RULE_BATCH 2u
+/* Lineage rules (scope exactly MANS_RULE_LINEAGE; chunk_units = D, rounded down to a
+ * multiple of 32; warm_units unused): a single-stream decode (mans_decode with automatic
+ * geometry, no sync points in or out and no ref_sync) that a lineage rule matches finds its
+ * true path by lineages -- a guessed-state decode starts every D units and runs until it
+ * meets a path another lineage recorded at a 32-unit checkpoint -- and decodes it from the
+ * resulting 32-unit item records with the write pass alone.  When the records do not hold
+ * (a checkpoint's slot list full, a lineage longer than the cap, maps that do not resolve,
+ * a rejected table), the same call runs the chunked pipeline instead, on the device, with
+ * no host round trip.  The workspace must hold mans_workspace_bytes(n_units) (the lineage
+ * layout is part of that bound); with less, the chunked pipeline runs.  Rules of other
+ * scopes never select lineages.  No reference counterpart (the reference synchronises
+ * subsequences only, src/multians_gpu_decoder.cu:195-466). */
+#define MANS_RULE_LINEAGE 4u
 typedef struct mans_geometry_rule {
     uint32_t L_max;
     uint32_t chunk_units;
@@ -297,6 +310,16 @@ typedef struct mans_geometry_rule {
     uint32_t scope;
 } mans_geometry_rule;
 int mans_set_geometry_rules(const mans_geometry_rule* rules, size_t n);
+/* Checkpoints one lineage may pass before a lineage decode gives up on lineages and runs
+ * the chunked pipeline (0 = the default, 2048; merges take a few dozen).  Process-wide; for
+ * tests and tuning.  No reference counterpart. */
+int mans_set_lineage_cap(uint32_t checkpoints);
+/* The lineage control words of the last lineage decode enqueued on this workspace
+ * (synchronises stream): out4[0] 1 when the chunked pipeline ran instead, out4[1] its
+ * reason bits (1 a slot list full, 2 the lineage cap, 4 maps unresolved, 8 a missing link,
+ * 16 the table), out4[2] lineages run, out4[3] checkpoints they decoded.  No reference
+ * counterpart. */
+int mans_lineage_stats(void* workspace, void* stream, uint32_t* out4);
 /* Polls of the sync kernel's wait for the previous range's last exit (a range of 256
  * chunks starts its first merge walk there).  A wait that runs out leaves that chunk to
  * the fixup rounds (the result is the same; status word 7 counts such waits).  Applies

@@ -114,6 +114,7 @@ class GeometryRule(C.Structure):
 WARM_AUTO = 0xFFFFFFFF
 RULE_SINGLE = 1
 RULE_BATCH = 2
+RULE_LINEAGE = 4
 
 
 class BatchStats(C.Structure):
@@ -140,6 +141,7 @@ EXPORTS = [
     "mans_encode_workspace_bytes", "mans_encode_device", "mans_encode_results",
     "mans_gen_symbols_device", "mans_ref_sync_fill", "mans_decode_noalloc",
     "mans_il_record_count", "mans_encode_interleaved", "mans_decode_interleaved",
+    "mans_set_lineage_cap", "mans_lineage_stats",
 ]
 
 _lib = None
@@ -220,6 +222,8 @@ def lib():
         "mans_encode_interleaved": ([vp, u64, vp, vp, u32, u32, vp, C.POINTER(u64),
                                      C.POINTER(u32), vp, vp, u64, C.POINTER(u64)], i32),
         "mans_decode_interleaved": ([C.POINTER(IlArgs)], i32),
+        "mans_set_lineage_cap": ([u32], i32),
+        "mans_lineage_stats": ([vp, vp, C.POINTER(u32)], i32),
         "mans_stream_range": ([C.POINTER(StreamDesc), vp, vp, u64, u64, C.POINTER(StreamDesc),
                                C.POINTER(u64)], i32),
     }

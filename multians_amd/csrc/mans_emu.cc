@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "mans_core.h"
+#include "mans_lineage.h"
 
 using namespace mans;
 
@@ -323,6 +324,146 @@ int mans_emu_decode_index(const uint32_t* units, uint64_t n_units, uint8_t* out,
 }
 
 }  // extern "C"
+
+// ------------------------------------------------ lineage synchronisation
+// The lineage path of a single-stream decode (mans_lineage.h), run on the CPU: lineages in
+// the kernel's ticket order (the last start first), each decoded one step at a time,
+// searching and taking slots as k_lin does; then the map, resolve, scan and trace lanes
+// (shared code) and the trusted write pass.  Returns 0 with the output written, 1 when the
+// lineage records do not hold (the decode would run the chunked pipeline; reason bits in
+// stats[2]), -4 for a rejected table.  D: units between lineage starts (a multiple of 32);
+// slot_limit (<= kLinSlots, 0 = all) and cap_ck (0 = none: checkpoints one lineage may
+// pass) force the failure paths.  stats (8 u64): lineages, checkpoints decoded by
+// lineages, reason bits, map chunks, most slots taken at a checkpoint, items, 0, 0.
+static int emu_lineage(const uint32_t* units, uint64_t n_units, uint8_t* out, uint64_t n_out,
+                const uint32_t* table, uint32_t L, uint32_t first_state, uint32_t first_bit,
+                uint32_t D, uint32_t slot_limit, uint32_t cap_ck, uint64_t* stats,
+                SyncPoint* rec_out) {
+    if (!n_out) return 0;
+    if (D < kLinQ || D % kLinQ) return -2;
+    std::vector<uint32_t> tab(L);
+    int bad = 0, cyc = 0;
+    for (uint32_t i = 0; i < L; ++i) tab[i] = derive_entry(table[i], L, i, &bad, &cyc);
+    if (bad) return -4;
+    const uint32_t nu = (uint32_t)n_units;
+    const uint32_t n_q = (nu + kLinQ - 1u) / kLinQ;
+    const uint32_t n_c = (n_q + kLinChunk - 1u) / kLinChunk;
+    const uint32_t SL = slot_limit && slot_limit < kLinSlots ? slot_limit : kLinSlots;
+    std::vector<uint32_t> val((size_t)n_q * kLinSlots, kLinEmpty), inf((size_t)n_q * kLinSlots, 0u),
+        cnt(n_q, 0u), mslot((size_t)n_c * kLinSlots), entry(n_c), ctl(8, 0u);
+    std::vector<uint64_t> msteps((size_t)n_c * kLinSlots), count(n_c), coff(n_c);
+    std::vector<SyncPoint> rec(n_q);
+    LinWork lw;
+    lw.val = val.data(); lw.inf = inf.data(); lw.cnt = cnt.data(); lw.mslot = mslot.data();
+    lw.msteps = msteps.data(); lw.entry = entry.data(); lw.count = count.data();
+    lw.rec = rec.data(); lw.ctl = ctl.data(); lw.n_q = n_q; lw.n_c = n_c;
+    HostTab ht{tab.data()};
+    uint32_t reason = cyc && n_q > 1u ? 16u : 0u;
+    uint64_t st_lin = 0, st_ck = 0, most = 1;
+    // the true start: slot 0 of checkpoint 0
+    val[0] = pack_entry(first_state - L, 32u - first_bit);
+    cnt[0] = 1u;
+    const uint32_t n_lin = (nu + D - 1u) / D;
+    for (uint32_t t = 0; t < n_lin && !reason; ++t) {
+        const uint32_t j = n_lin - 1u - t;            // ticket order: the last start first
+        const uint32_t u0 = j * D;
+        uint32_t q = u0 / kLinQ, slot = kLinEmpty, xi;
+        Cursor cur;
+        if (j == 0u) {
+            slot = 0u;
+            xi = first_state - L;
+            cursor_init(cur, units, nu, 0u, 32u - first_bit);
+        } else {
+            if (q + 1u >= n_q) continue;              // no checkpoint after its start
+            xi = lin_guess(j, L);
+            cursor_init(cur, units, nu, u0, 0u);
+        }
+        ++st_lin;
+        uint32_t passed = 0;
+        while (q + 1u < n_q) {
+            // decode to checkpoint q + 1: every step that starts before its unit
+            const uint32_t b = (q + 1u) * kLinQ;
+            uint32_t steps = 0;
+            while (cur.u < b) {
+                uint32_t crossed;
+                step(ht, cur, xi, crossed);
+                ++steps;
+            }
+            ++q;
+            ++st_ck;
+            const uint32_t v = pack_entry(xi, cur.o);
+            const uint64_t base = (uint64_t)q * kLinSlots;
+            uint32_t found = kLinEmpty;
+            for (uint32_t s = 0; s < lin_slots_used(cnt[q]) && s < SL; ++s)
+                if (val[base + s] == v) { found = s; break; }
+            if (found != kLinEmpty) {
+                if (slot != kLinEmpty) inf[(uint64_t)(q - 1u) * kLinSlots + slot] = lin_link(found, steps);
+                break;                                  // merged
+            }
+            const uint32_t ns = cnt[q]++;
+            if (ns >= SL) {
+                if (slot != kLinEmpty)
+                    inf[(uint64_t)(q - 1u) * kLinSlots + slot] = lin_link(kLinBroken, 0u);
+                reason |= 1u;
+                break;
+            }
+            val[base + ns] = v;
+            most = std::max<uint64_t>(most, ns + 1u);
+            if (slot != kLinEmpty) inf[(uint64_t)(q - 1u) * kLinSlots + slot] = lin_link(ns, steps);
+            slot = ns;
+            if (q + 1u == n_q) {                        // the last checkpoint
+                inf[base + ns] = lin_link(kLinEnd, 0u);
+                break;
+            }
+            if (cap_ck && ++passed >= cap_ck) {
+                inf[base + ns] = lin_link(kLinBroken, 0u);
+                reason |= 2u;
+                break;
+            }
+        }
+        // lineage 0 in a stream of one checkpoint: nothing to link
+        if (j == 0u && n_q == 1u) inf[0] = lin_link(kLinEnd, 0u);
+    }
+    if (!reason) {
+        for (uint32_t c = 0; c < n_c; ++c)
+            for (uint32_t s = 0; s < kLinSlots; ++s) lin_map_lane(lw, c, s);
+        for (uint32_t c = 0; c < n_c && !reason; ++c) reason |= lin_resolve_lane(lw, c);
+    }
+    if (!reason) {
+        uint64_t run = 0;                               // k_scan over the chunk counts
+        for (uint32_t c = 0; c < n_c; ++c) { coff[c] = run; run += count[c]; }
+        for (uint32_t c = 0; c < n_c && !reason; ++c) reason |= lin_trace_lane(lw, c, coff[c]);
+    }
+    if (stats) {
+        stats[0] = st_lin; stats[1] = st_ck; stats[2] = reason; stats[3] = n_c;
+        stats[4] = most; stats[5] = n_q; stats[6] = 0; stats[7] = 0;
+    }
+    if (reason) return 1;
+    if (rec_out) std::copy(rec.begin(), rec.end(), rec_out);
+    // the trusted write pass (k_write with item records)
+    for (uint32_t q = 0; q < n_q; ++q) {
+        uint64_t off = rec[q].before, end = q + 1u < n_q ? rec[q + 1u].before : n_out;
+        if (end > n_out) end = n_out;
+        if (off >= end) continue;
+        Cursor cur;
+        cursor_init(cur, units, nu, q * kLinQ, q == 0u ? 32u - first_bit : rec[q].bit);
+        uint32_t xi = q == 0u ? first_state - L : rec[q].state;
+        for (uint64_t i = off; i < end; ++i) {
+            uint32_t crossed;
+            out[i] = (uint8_t)(step(ht, cur, xi, crossed) >> 8);
+        }
+    }
+    return 0;
+}
+
+extern "C" int mans_emu_lineage_decode(const uint32_t* units, uint64_t n_units, uint8_t* out,
+                                       uint64_t n_out, const uint32_t* table, uint32_t L,
+                                       uint32_t first_state, uint32_t first_bit, uint32_t D,
+                                       uint32_t slot_limit, uint32_t cap_ck, uint64_t* stats,
+                                       void* rec_out) {
+    return emu_lineage(units, n_units, out, n_out, table, L, first_state, first_bit, D,
+                       slot_limit, cap_ck, stats, (SyncPoint*)rec_out);
+}
 
 // ------------------------------------------------ two-state interleaved streams
 // The lanes of k_il_decode run one after another through the same per-lane code

@@ -23,6 +23,7 @@
 
 #include "mans_core.h"
 #include "mans_internal.h"
+#include "mans_lineage.h"
 
 using namespace mans;
 
@@ -82,7 +83,19 @@ struct KArgs {
     uint32_t* ref_sizes;
     uint8_t* ref_flags;
     uint32_t ref_S, ref_nsub, ref_nblocks;
+    // optional (lineage decodes): k_scan takes its per-chunk counts from lin_count
+    const uint64_t* lin_count;
+    // optional: the kernel runs only when *gate == gate_val (a device word written by an
+    // earlier kernel of the same sequence, so the choice needs no host round trip)
+    const uint32_t* gate;
+    uint32_t gate_val;
 };
+
+// True when this launch is switched off by its gate word (uniform over the grid).
+__device__ __forceinline__ bool gated_off(const KArgs& a) {
+    return a.gate &&
+           __hip_atomic_load(a.gate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.gate_val;
+}
 
 // Chunk task of workgroup b: batches run their tasks longest-expected first, so the
 // long ones do not start last and hold the end of the kernel.
@@ -217,6 +230,7 @@ __device__ void fold_zero_bit_runs(uint32_t* lds, uint32_t L, uint32_t* scratch,
 // Clears the status words, tickets and fixup counters, and marks the last exit of each
 // range that another range waits for (k_sync) as not yet published.
 __global__ void k_init(KArgs a, uint32_t n_status) {
+    if (gated_off(a)) return;
     const Work& w = a.w;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_status) w.status[i] = 0u;
@@ -610,6 +624,7 @@ __device__ void walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& ta
 // workgroup waits for it, a bounded number of polls (a wait that runs out skips that
 // walk and marks the chunk's start kNone, so k_check lists it and k_fixup walks it).
 __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
+    if (gated_off(a)) return;
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_task, s_stream, s_entry;
     __shared__ uint32_t s_rbase[kWG];
@@ -752,6 +767,7 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
 // Consistency check over all chunks (grid as k_sync): a chunk whose recorded walk start
 // differs from its predecessor's exit goes on its stream's fixup list.  Clears fflag.
 __global__ __launch_bounds__(kWG) void k_check(KArgs a) {
+    if (gated_off(a)) return;
     __shared__ uint32_t s_stream;
     const uint32_t t = chunk_task(a, blockIdx.x);
     if (threadIdx.x == 0) s_stream = stream_of_task<false>(a, t);
@@ -779,6 +795,7 @@ __global__ __launch_bounds__(kWG) void k_check(KArgs a) {
 // still inconsistent then in status[0] bit 1 instead of looping on.  Streams are
 // independent, so no grid-wide sync.
 __global__ __launch_bounds__(kWG) void k_fixup(KArgs a) {
+    if (gated_off(a)) return;
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_n, s_n2;
     __shared__ uint32_t s_rbase[kWG];
@@ -896,6 +913,7 @@ constexpr uint64_t kScanAgg = 1ull << 62, kScanIncl = 2ull << 62, kScanVal = kSc
 constexpr uint32_t kScanPolls = 1u << 22;   // bound of one look-back wait (never reached)
 
 __global__ __launch_bounds__(kWG) void k_scan(KArgs a, uint32_t total, uint64_t* desc) {
+    if (gated_off(a)) return;
     __shared__ uint64_t s_w[4];
     __shared__ uint32_t s_blk;
     __shared__ uint64_t s_prefix;
@@ -914,7 +932,8 @@ __global__ __launch_bounds__(kWG) void k_scan(KArgs a, uint32_t total, uint64_t*
         if (gc < total) {
             while (a.plans && sid + 1u < a.n_streams && a.plans[sid + 1u].chunk_base <= gc) ++sid;
             const StreamPlan& sp = *plan_of(a, sid);
-            if (gc - sp.chunk_base < sp.n_chunks) c[i] = chunk_count(sp, a.w, gc - sp.chunk_base);
+            if (a.lin_count) c[i] = a.lin_count[gc];
+            else if (gc - sp.chunk_base < sp.n_chunks) c[i] = chunk_count(sp, a.w, gc - sp.chunk_base);
         }
         v += c[i];
     }
@@ -1190,6 +1209,7 @@ __device__ __forceinline__ uint64_t item_record(const StreamPlan& sp, uint32_t i
 }
 
 __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint32_t per_wg) {
+    if (gated_off(a)) return;
     // Persistent workgroups take blocks of per_wg consecutive passes from a device
     // counter (ctick[0], zeroed by k_init), so slow passes (high symbols per unit) do
     // not pile up on a few workgroups.
@@ -1278,6 +1298,262 @@ __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint3
     }
 }
 
+// ------------------------------------------------------------ lineage synchronisation
+// Single-stream decodes that a lineage rule selects find the true path with lineages
+// (mans_lineage.h: a guessed-state decode starts every D units and runs until it meets a
+// path another lineage recorded) instead of k_sync / k_check / k_fixup, then decode it with
+// the trusted form of k_write.  The chunked pipeline is enqueued after the lineage kernels
+// with a gate on the failure word, so it runs only when the lineage records do not hold.
+
+struct LinArgs {
+    LinWork lw;
+    uint32_t D;               // units between lineage starts (multiple of kLinQ)
+    uint32_t n_lin;           // lineages: ceil(n_units / D)
+    uint32_t cap;             // checkpoints one lineage may pass
+    uint32_t n_scan_blocks;   // k_scan descriptors to clear
+    uint32_t* status;         // status words (offset 0 of the workspace)
+    uint32_t* ctick;
+    uint64_t* bsum;
+};
+
+__device__ __forceinline__ uint32_t lin_ld(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A failure also exhausts the lineage ticket, so lanes stop taking lineages at their
+// next grab (no lane polls the failure word: one word read by every lane at every
+// checkpoint serialises on its cache line).
+__device__ __forceinline__ void lin_fail(const LinArgs& la, uint32_t reason) {
+    atomicOr(&la.lw.ctl[2], reason);
+    __hip_atomic_store(&la.lw.ctl[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    atomicMax(&la.lw.ctl[0], 0x40000000u);
+}
+
+// Clears the status and control words, the scan's tickets and descriptors, the slot
+// counts and slot values, and records the stream start in slot 0 of checkpoint 0.
+__global__ void k_lin_init(KArgs a, LinArgs la) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const LinWork& lw = la.lw;
+    if (i < 128u) la.status[i] = 0u;
+    if (i < 8u) la.ctick[i] = 0u;
+    if (i < la.n_scan_blocks) la.bsum[i] = 0ull;
+    if (i < lw.n_q) {
+        uint4* v = reinterpret_cast<uint4*>(lw.val + (uint64_t)i * kLinSlots);
+        const uint4 e = make_uint4(kLinEmpty, kLinEmpty, kLinEmpty, kLinEmpty);
+#pragma unroll
+        for (uint32_t k = 0; k < kLinSlots / 4u; ++k) v[k] = e;
+        lw.cnt[i] = 0u;
+        if (i == 0u) {
+            lw.val[0] = pack_entry(a.one.first_state - a.one.L, 32u - a.one.first_bit);
+            lw.cnt[0] = 1u;
+        }
+    }
+}
+
+// A lane's own round of units (kRL from unit rs, a multiple of 4), and its store into the
+// lane's LDS segment (kLS units, 8-byte aligned).
+__device__ __forceinline__ void lane_load(const uint32_t* units, uint32_t n, uint32_t rs,
+                                          Prefetch& pf) {
+#pragma unroll
+    for (uint32_t j = 0; j < kPieces; ++j) {
+        const U4 u = load_block_raw(units, n, rs + 4u * j);
+        pf.v[j] = make_uint4(u.x, u.y, u.z, u.w);
+    }
+}
+
+__device__ __forceinline__ void lane_store(uint32_t* seg, const Prefetch& pf) {
+#pragma unroll
+    for (uint32_t j = 0; j < kPieces; ++j) {
+        uint2* d = reinterpret_cast<uint2*>(seg + 4u * j);
+        if (4u * j < kLS) d[0] = make_uint2(pf.v[j].x, pf.v[j].y);
+        if (4u * j + 2u < kLS) d[1] = make_uint2(pf.v[j].z, pf.v[j].w);
+    }
+}
+
+// Persistent lineage kernel: every lane takes lineages from a ticket (the last start
+// first, lineage 0 from the true start last) and runs each in rounds of kR units from
+// its own LDS segment (the count-only round engine of k_sync, zero-bit runs folded),
+// loading the next round's units while it decodes the current one.  At each checkpoint
+// (two rounds) it searches the checkpoint's slots for its entry: found, it links its last
+// slot there and ends; not found, it takes a slot, links to it and goes on.  No barriers
+// after the table is staged: lanes are independent.
+__global__ __launch_bounds__(kWG) void k_lin(KArgs a, LinArgs la) {
+    extern __shared__ uint32_t lds[];
+    const StreamPlan& sp = a.one;
+    const LinWork& lw = la.lw;
+    uint32_t* buf = lds + table_words(sp.L);
+    const uint32_t tf = stage_table(sp, lds, la.status, true);
+    if ((tf & 2u) || ((tf & 1u) && lw.n_q > 1u)) {       // uniform: leave it to k_sync
+        if (threadIdx.x == 0) lin_fail(la, 16u);
+        return;
+    }
+    fold_zero_bit_runs(lds, sp.L, buf, kWG * kLS);
+    __syncthreads();
+    const LdsTab tab{lds_addr(lds) >> 2};
+    uint32_t* seg = buf + threadIdx.x * kLS;
+    RLane L;
+    L.active = 0u;
+    L.u_stop = sp.n_units;
+    uint32_t q = 0, slot = kLinEmpty, half = 0, passed = 0, nlin = 0, nck = 0;
+    const uint32_t lane = __lane_id();
+    const uint32_t qstep = la.D / kLinQ;
+    // Tickets: a wave takes 64 at a time (one atomic) and hands them to its lanes as they
+    // need them, in rank order (the rest of a block first); every lineage is run (lineage 0,
+    // the true path, last).  Each lane holds its next lineage's ticket (nt, kNone when none
+    // is left) and has that lineage's first round of units in flight (pn) while it decodes
+    // the current one.  take() is called by every lane still in the loop (wave-uniform).
+    uint32_t pool = 0u, pool_left = 0u;
+    auto take = [&](bool want) -> uint32_t {
+        const uint64_t all = __ballot(true);
+        const uint64_t m = __ballot(want);
+        if (!m) return kNone;
+        const uint32_t n = (uint32_t)__popcll(m);
+        const uint32_t r = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        uint32_t b = pool;
+        if (n > pool_left) {
+            const uint32_t lead = (uint32_t)__builtin_ctzll(all);
+            uint32_t nb = 0u;
+            if (lane == lead) nb = atomicAdd(&lw.ctl[0], 64u);
+            b = __shfl(nb, (int)lead, 64);
+        }
+        const uint32_t t = r < pool_left ? pool + r : b + (r - pool_left);
+        if (n > pool_left) {
+            pool = b + (n - pool_left);
+            pool_left = 64u - (n - pool_left);
+        } else {
+            pool += n;
+            pool_left -= n;
+        }
+        return want && t < la.n_lin ? t : kNone;
+    };
+    auto first_unit = [&](uint32_t t) -> uint32_t {
+        return (la.n_lin - 1u - t) * qstep * kLinQ;
+    };
+    Prefetch pf, pn;
+    uint32_t nt = take(true);
+    if (nt != kNone) lane_load(sp.units, sp.n_units, first_unit(nt), pn);
+    uint32_t cv[kLinSlots], cu = 0u;                  // checkpoint slots read ahead
+    for (;;) {
+        // lanes without a lineage start the held one (its first round is in pn) and take
+        // the next ticket; a lane with neither leaves
+        const bool need = !L.active;
+        const uint32_t t = need ? nt : kNone;
+        const uint32_t t2 = take(need && t != kNone);
+        if (need) {
+            if (t == kNone) break;
+            pf = pn;
+            nt = t2;
+            if (nt != kNone) lane_load(sp.units, sp.n_units, first_unit(nt), pn);
+            const uint32_t j = la.n_lin - 1u - t;
+            q = j * qstep;
+            if (q + 1u >= lw.n_q) continue;           // no checkpoint after it
+            ++nlin;
+            passed = 0u;
+            half = 0u;
+            if (j == 0u) {
+                slot = 0u;
+                L.xi = sp.first_state - sp.L;
+                const uint32_t b0 = 32u - sp.first_bit;
+                L.u = b0 >> 5;
+                L.o = b0 & 31u;
+            } else {
+                slot = kLinEmpty;
+                L.xi = lin_guess(j, sp.L);
+                L.u = q * kLinQ;
+                L.o = 0u;
+            }
+            L.rbase = q * kLinQ;
+            L.cnt = 0u;
+            L.active = 1u;
+        }
+        lane_store(seg, pf);
+        lane_load(sp.units, sp.n_units, L.rbase + kR, pf);  // the next round, in flight
+        if (half) {
+            // the slots of the checkpoint this round ends at, read before decoding it (a
+            // value written meanwhile is only missed: the lineage then takes a slot of its
+            // own, the same path twice)
+            const uint64_t base = (uint64_t)(q + 1u) * kLinSlots;
+            cu = lin_slots_used(lin_ld(&lw.cnt[q + 1u]));
+#pragma unroll
+            for (uint32_t s = 0; s < kLinSlots; ++s) cv[s] = lin_ld(&lw.val[base + s]);
+        }
+        round_decode(tab, seg, L, 0u);
+        L.rbase += kR;
+        if (!half) {
+            half = 1u;
+            continue;
+        }
+        // checkpoint q + 1 (unit L.rbase): the lane is at its entry
+        half = 0u;
+        ++q;
+        ++nck;
+        const uint32_t steps = L.cnt;
+        L.cnt = 0u;
+        const uint32_t v = pack_entry(L.xi, L.o);
+        const uint64_t base = (uint64_t)q * kLinSlots;
+        const uint64_t prev = (uint64_t)(q - 1u) * kLinSlots + slot;
+        uint32_t found = kLinEmpty;
+#pragma unroll
+        for (uint32_t s = kLinSlots; s-- > 0;)
+            if (s < cu && cv[s] == v) found = s;
+        if (found != kLinEmpty) {                             // merged
+            if (slot != kLinEmpty) lw.inf[prev] = lin_link(found, steps);
+            L.active = 0u;
+            continue;
+        }
+        const uint32_t ns = atomicAdd(&lw.cnt[q], 1u);
+        if (ns >= kLinSlots) {                                // slot list full
+            if (slot != kLinEmpty) lw.inf[prev] = lin_link(kLinBroken, 0u);
+            lin_fail(la, 1u);
+            L.active = 0u;
+            continue;
+        }
+        __hip_atomic_store(&lw.val[base + ns], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (slot != kLinEmpty) lw.inf[prev] = lin_link(ns, steps);
+        slot = ns;
+        if (q + 1u == lw.n_q) {                               // the last checkpoint
+            lw.inf[base + ns] = lin_link(kLinEnd, 0u);
+            L.active = 0u;
+            continue;
+        }
+        if (++passed >= la.cap) {                             // longer than any merge
+            lw.inf[base + ns] = lin_link(kLinBroken, 0u);
+            lin_fail(la, 2u);
+            L.active = 0u;
+            continue;
+        }
+    }
+    // statistics: one atomic per wave
+    for (uint32_t o = 32; o > 0; o >>= 1) {
+        nlin += __shfl_xor(nlin, (int)o, 64);
+        nck += __shfl_xor(nck, (int)o, 64);
+    }
+    if (lane == 0u) {
+        atomicAdd(&lw.ctl[3], nlin);
+        atomicAdd(&lw.ctl[4], nck);
+    }
+}
+
+__global__ void k_lin_map(LinArgs la) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= la.lw.n_c * kLinSlots || lin_ld(&la.lw.ctl[1])) return;
+    lin_map_lane(la.lw, i / kLinSlots, i % kLinSlots);
+}
+
+__global__ void k_lin_resolve(LinArgs la) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= la.lw.n_c || lin_ld(&la.lw.ctl[1])) return;
+    const uint32_t r = lin_resolve_lane(la.lw, c);
+    if (r) lin_fail(la, r);
+}
+
+__global__ void k_lin_trace(KArgs a, LinArgs la) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= la.lw.n_c || lin_ld(&la.lw.ctl[1])) return;
+    const uint32_t r = lin_trace_lane(la.lw, c, a.w.coff[c]);
+    if (r) lin_fail(la, r);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ host side
@@ -1324,6 +1600,28 @@ static RuleHit rule_geometry(uint64_t n_units, uint64_t n_out, uint32_t L, uint3
     for (const mans_geometry_rule& r : kTunedRules)
         if (match(r)) return {r.chunk_units, r.warm_units};
     return {0u, MANS_WARM_AUTO};
+}
+
+// Lineage spacing D (units) for a single-stream decode, from the first rule of scope
+// MANS_RULE_LINEAGE that matches (rules of other scopes never select it); 0 = none.
+static uint32_t rule_lineage(uint64_t n_units, uint64_t n_out, uint32_t L) {
+    const double spu = n_units ? (double)n_out / (double)n_units : 4.0;
+    auto match = [&](const mans_geometry_rule& r) {
+        return r.chunk_units && (r.scope & MANS_RULE_LINEAGE) && L <= r.L_max &&
+               spu <= r.spu_max && n_units <= r.units_max;
+    };
+    uint32_t D = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_rules_mu);
+        if (!g_rules.empty()) {
+            for (const mans_geometry_rule& r : g_rules)
+                if (match(r)) { D = r.chunk_units; break; }
+            return D / kLinQ * kLinQ;
+        }
+    }
+    for (const mans_geometry_rule& r : kTunedRules)
+        if (match(r)) { D = r.chunk_units; break; }
+    return D / kLinQ * kLinQ;
 }
 
 // Spec warm-up (units decoded before each chunk, a multiple of K).  Built-in choice (no
@@ -1427,7 +1725,9 @@ Layout make_layout(uint64_t chunks, uint64_t cks, uint64_t n_plans) {
     l.cks = cks;
     uint64_t o = 0;
     auto take = [&](uint64_t bytes) { uint64_t r = o; o = align256(o + bytes); return r; };
-    l.o_status = take(4 * 64);   // first: mans_decode_status reads offset 0
+    // first: mans_decode_status reads offset 0; words 0-63 are cleared by k_init, the
+    // lineage control words (kLinCtl..) follow and survive it
+    l.o_status = take(4 * 128);
     l.o_exitv = take(4 * chunks);
     l.o_own = take(4 * chunks);
     l.o_went = take(4 * chunks);
@@ -1475,6 +1775,37 @@ Work make_work(void* base, const Layout& l) {
     w.bsum = (uint64_t*)(b + l.o_bsum);
     w.fcnt = (uint32_t*)(b + l.o_fcnt);
     return w;
+}
+
+// Workspace layout of a lineage decode.  The status block sits at offset 0 as in the
+// chunked layout (same words, same place), so the gate word (a control word after the
+// 64 words k_init clears) is the same word for both pipelines.
+struct LinLayout {
+    uint64_t n_q = 0, n_c = 0, bytes = 0;
+    uint64_t o_tick, o_bsum, o_coff, o_count, o_entry, o_mslot, o_msteps, o_cnt, o_val, o_inf,
+        o_rec;
+};
+
+static LinLayout lin_layout(uint64_t n_units) {
+    LinLayout l;
+    l.n_q = (n_units + kLinQ - 1u) / kLinQ;
+    l.n_c = (l.n_q + kLinChunk - 1u) / kLinChunk;
+    uint64_t o = 0;
+    auto take = [&](uint64_t bytes) { uint64_t r = o; o = align256(o + bytes); return r; };
+    take(4 * 128);                          // status and control words
+    l.o_tick = take(4 * 8);
+    l.o_bsum = take(8 * (l.n_c / kScanBlock + 2));
+    l.o_coff = take(8 * l.n_c);
+    l.o_count = take(8 * l.n_c);
+    l.o_entry = take(4 * l.n_c);
+    l.o_mslot = take(4 * l.n_c * kLinSlots);
+    l.o_msteps = take(8 * l.n_c * kLinSlots);
+    l.o_cnt = take(4 * l.n_q);
+    l.o_val = take(4 * l.n_q * kLinSlots);
+    l.o_inf = take(4 * l.n_q * kLinSlots);
+    l.o_rec = take(sizeof(SyncPoint) * l.n_q);
+    l.bytes = o;
+    return l;
 }
 
 static int launch_pipeline(KArgs a, uint32_t n_chunk_tasks, uint32_t n_item_tasks,
@@ -1558,7 +1889,7 @@ extern "C" int mans_workspace_bytes(uint64_t n_units, uint32_t L, uint64_t* byte
     // smallest geometry the automatic choice uses: S = kMinRuleChunk; K (the round
     // length) >= 8
     uint64_t chunks = n_units / kMinRuleChunk + 2u, cks = n_units / 8u + 2u;
-    *bytes = make_layout(chunks, cks, 1).bytes;
+    *bytes = std::max(make_layout(chunks, cks, 1).bytes, lin_layout(n_units).bytes);
     return MANS_OK;
 }
 
@@ -1652,6 +1983,123 @@ static int single_plan(const mans_decode_args* d, KArgs* out, uint32_t* nct, uin
     return MANS_OK;
 }
 
+// ------------------------------------------------------------ lineage decodes
+static std::atomic<uint32_t> g_lin_cap{2048u};
+
+// Lineage spacing for this decode, or 0 for the chunked pipeline: a lineage rule matches,
+// the geometry is automatic, nothing asks for chunk-based sync state, and the workspace
+// holds the lineage layout.
+static uint32_t lineage_choice(const mans_decode_args* d) {
+    if (d->chunk_units || d->index_in || d->index_out || d->ref_sync ||
+        (d->flags & MANS_F_TRUST_SYNC_POINTS))
+        return 0u;
+    const uint64_t n_q = (d->n_units + kLinQ - 1u) / kLinQ;
+    if (n_q < 2u) return 0u;
+    const uint32_t D = rule_lineage(d->n_units, d->n_out, d->L);
+    if (!D || D > (1u << 24)) return 0u;
+    if (!d->workspace || d->workspace_bytes < lin_layout(d->n_units).bytes) return 0u;
+    return D;
+}
+
+// The lineage kernels, then the chunked pipeline gated on the failure word, then the
+// trusted write pass gated on success.  a: the chunked plan of the same decode.
+static int launch_lineage(const mans_decode_args* d, const KArgs& a, uint32_t nct, uint32_t nit,
+                          uint32_t D, hipStream_t st, void** ev) {
+    auto mark = [&](int i) { if (ev) (void)hipEventRecord((hipEvent_t)ev[i], st); };
+    const LinLayout l = lin_layout(d->n_units);
+    char* b = (char*)d->workspace;
+    LinArgs la;
+    std::memset(&la, 0, sizeof(la));
+    la.lw.val = (uint32_t*)(b + l.o_val);
+    la.lw.inf = (uint32_t*)(b + l.o_inf);
+    la.lw.cnt = (uint32_t*)(b + l.o_cnt);
+    la.lw.mslot = (uint32_t*)(b + l.o_mslot);
+    la.lw.msteps = (uint64_t*)(b + l.o_msteps);
+    la.lw.entry = (uint32_t*)(b + l.o_entry);
+    la.lw.count = (uint64_t*)(b + l.o_count);
+    la.lw.rec = (SyncPoint*)(b + l.o_rec);
+    la.status = (uint32_t*)b;
+    la.lw.ctl = la.status + kLinCtl;
+    la.lw.n_q = (uint32_t)l.n_q;
+    la.lw.n_c = (uint32_t)l.n_c;
+    la.D = D;
+    la.n_lin = (uint32_t)((d->n_units + D - 1u) / D);
+    la.cap = std::max<uint32_t>(1u, g_lin_cap.load());
+    la.n_scan_blocks = (uint32_t)((l.n_c + kScanBlock - 1u) / kScanBlock);
+    la.ctick = (uint32_t*)(b + l.o_tick);
+    la.bsum = (uint64_t*)(b + l.o_bsum);
+    // the lineage path's own arguments for k_scan and the trusted k_write
+    KArgs k = a;
+    std::memset(&k.w, 0, sizeof(k.w));
+    k.w.status = la.status;
+    k.w.ctick = la.ctick;
+    k.w.bsum = la.bsum;
+    k.w.coff = (uint64_t*)(b + l.o_coff);
+    k.one.index_in = la.lw.rec;
+    k.direct = 1u;
+    k.any_index_out = 0u;
+    k.ref_sync = nullptr;
+    k.lin_count = la.lw.count;
+    k.gate = nullptr;
+    k.n_scan_blocks = la.n_scan_blocks;
+    mark(0);
+    {
+        const uint64_t nthr = std::max<uint64_t>({l.n_q, 128u, la.n_scan_blocks});
+        hipLaunchKernelGGL(k_lin_init, dim3((uint32_t)((nthr + 255u) / 256u)), dim3(256), 0, st,
+                           k, la);
+    }
+    mark(1);
+    {
+        const size_t lds_s = (size_t)table_words(a.one.L) * 4u + (size_t)kWG * kLS * 4u;
+        const uint32_t nwg = std::min<uint32_t>((la.n_lin + kWG - 1u) / kWG, 1024u);
+        hipLaunchKernelGGL(k_lin, dim3(nwg), dim3(kWG), lds_s, st, k, la);
+    }
+    mark(2);
+    hipLaunchKernelGGL(k_lin_map, dim3((uint32_t)((l.n_c * kLinSlots + 255u) / 256u)), dim3(256),
+                       0, st, la);
+    hipLaunchKernelGGL(k_lin_resolve, dim3((uint32_t)((l.n_c + 255u) / 256u)), dim3(256), 0, st,
+                       la);
+    mark(3);
+    hipLaunchKernelGGL(k_scan, dim3(la.n_scan_blocks), dim3(kWG), 0, st, k, (uint32_t)l.n_c,
+                       la.bsum);
+    hipLaunchKernelGGL(k_lin_trace, dim3((uint32_t)((l.n_c + 255u) / 256u)), dim3(256), 0, st, k,
+                       la);
+    mark(4);
+    // the chunked pipeline, run only if the lineage records do not hold
+    KArgs g = a;
+    g.gate = la.lw.ctl + 1;
+    g.gate_val = 1u;
+    int rc = launch_pipeline(g, nct, nit, a.one.n_chunks, a.one.L, a.one.K, st, nullptr);
+    if (rc) return rc;
+    mark(5);
+    {
+        k.gate = la.lw.ctl + 1;
+        k.gate_val = 0u;
+        const uint32_t per = 4u;
+        const uint32_t nwg = std::min<uint32_t>((nit + per - 1) / per, 2048u);
+        const size_t lds_w = (size_t)table_words(a.one.L) * 4u + (size_t)kWG * kWL * 4u;
+        hipLaunchKernelGGL(k_write, dim3(nwg), dim3(kWG), lds_w, st, k, nit, per);
+    }
+    mark(6);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(MANS_EHIP, "kernel launch: %s", hipGetErrorString(e));
+    return MANS_OK;
+}
+
+extern "C" int mans_set_lineage_cap(uint32_t checkpoints) {
+    g_lin_cap.store(checkpoints ? checkpoints : 2048u);
+    return MANS_OK;
+}
+
+extern "C" int mans_lineage_stats(void* workspace, void* stream, uint32_t* out4) {
+    if (!workspace || !out4) return set_error(MANS_EINVAL, "NULL argument");
+    const uint32_t* ctl = (const uint32_t*)workspace + kLinCtl;
+    hipError_t e = hipMemcpyAsync(out4, ctl + 1, 16, hipMemcpyDeviceToHost, (hipStream_t)stream);
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) return set_error(MANS_EHIP, "lineage stats read: %s", hipGetErrorString(e));
+    return MANS_OK;
+}
+
 extern "C" int mans_decode(const mans_decode_args* d) {
     if (!d) return set_error(MANS_EINVAL, "args is NULL");
     if (d->n_out == 0)
@@ -1661,8 +2109,12 @@ extern "C" int mans_decode(const mans_decode_args* d) {
     uint32_t nct, nit;
     int rc = single_plan(d, &a, &nct, &nit);
     if (rc) return rc;
-    rc = launch_pipeline(a, nct, nit, a.one.n_chunks, a.one.L, a.one.K, (hipStream_t)d->stream,
-                         d->kernel_events);
+    const uint32_t D = lineage_choice(d);
+    if (D)
+        rc = launch_lineage(d, a, nct, nit, D, (hipStream_t)d->stream, d->kernel_events);
+    else
+        rc = launch_pipeline(a, nct, nit, a.one.n_chunks, a.one.L, a.one.K,
+                             (hipStream_t)d->stream, d->kernel_events);
     if (rc) return rc;
     if (d->flags & MANS_F_SYNC_ERRORS) {
         int status = 0;
@@ -2298,6 +2750,11 @@ extern "C" int mans_set_geometry_rules(const mans_geometry_rule* rules, size_t n
         if (r.chunk_units < kMinRuleChunk || r.chunk_units > (1u << 24))
             return set_error(MANS_EINVAL, "rule %zu: chunk_units %u outside [%u, 2^24]", i,
                              r.chunk_units, kMinRuleChunk);
+        if (r.scope == MANS_RULE_LINEAGE) {          // lineage spacing: a multiple of 32
+            r.chunk_units = r.chunk_units / kLinQ * kLinQ;
+            v.push_back(r);
+            continue;
+        }
         if (r.scope > (MANS_RULE_SINGLE | MANS_RULE_BATCH))
             return set_error(MANS_EINVAL, "rule %zu: scope %u", i, r.scope);
         uint32_t p = kMinRuleChunk;

@@ -146,6 +146,19 @@ class DeviceStream:
                                                  C.byref(s)))
         return s.value
 
+    def lineage_stats(self, stream=None):
+        """Control words of the last lineage decode on this stream's workspace: (fell back
+        to the chunked pipeline, reason bits, lineages run, checkpoints they decoded)."""
+        return lineage_stats(self.ws, stream)
+
+
+def lineage_stats(ws, stream=None):
+    torch = _torch()
+    st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    out = (C.c_uint32 * 4)()
+    _lib.check(_lib.lib().mans_lineage_stats(C.c_void_p(ws.data_ptr()), C.c_void_p(st), out))
+    return tuple(int(v) for v in out)
+
 
 @dataclass
 class InterleavedEncoded:
