@@ -289,7 +289,8 @@ int mans_graph_destroy(void* graph);
 #define MANS_RULE_SINGLE 1u
 #define MANS_RULE_BATCH 2u
 /* Lineage rules (scope exactly MANS_RULE_LINEAGE; chunk_units = D, rounded down to a
- * multiple of 32; warm_units unused): a single-stream decode (mans_decode with automatic
+ * multiple of 32, or 0 = the chunked pipeline for the streams the rule matches, so that a
+ * later rule does not catch them; warm_units unused): a single-stream decode (mans_decode with automatic
  * geometry, no sync points in or out and no ref_sync) that a lineage rule matches finds its
  * true path by lineages -- a guessed-state decode starts every D units and runs until it
  * meets a path another lineage recorded at a 32-unit checkpoint -- and decodes it from the
@@ -314,6 +315,10 @@ int mans_set_geometry_rules(const mans_geometry_rule* rules, size_t n);
  * the chunked pipeline (0 = the default, 2048; merges take a few dozen).  Process-wide; for
  * tests and tuning.  No reference counterpart. */
 int mans_set_lineage_cap(uint32_t checkpoints);
+/* The lineage spacing the rules give a single stream of this shape (0 = the chunked
+ * pipeline); mans_decode also needs automatic geometry and no sync points in or out for a
+ * lineage decode. */
+int mans_lineage_spacing(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t* D);
 /* The lineage control words of the last lineage decode enqueued on this workspace
  * (synchronises stream): out4[0] 1 when the chunked pipeline ran instead, out4[1] its
  * reason bits (1 a slot list full, 2 the lineage cap, 4 maps unresolved, 8 a missing link,

@@ -141,7 +141,7 @@ EXPORTS = [
     "mans_encode_workspace_bytes", "mans_encode_device", "mans_encode_results",
     "mans_gen_symbols_device", "mans_ref_sync_fill", "mans_decode_noalloc",
     "mans_il_record_count", "mans_encode_interleaved", "mans_decode_interleaved",
-    "mans_set_lineage_cap", "mans_lineage_stats",
+    "mans_set_lineage_cap", "mans_lineage_stats", "mans_lineage_spacing",
 ]
 
 _lib = None
@@ -224,6 +224,7 @@ def lib():
         "mans_decode_interleaved": ([C.POINTER(IlArgs)], i32),
         "mans_set_lineage_cap": ([u32], i32),
         "mans_lineage_stats": ([vp, vp, C.POINTER(u32)], i32),
+        "mans_lineage_spacing": ([u64, u64, u32, C.POINTER(u32)], i32),
         "mans_stream_range": ([C.POINTER(StreamDesc), vp, vp, u64, u64, C.POINTER(StreamDesc),
                                C.POINTER(u64)], i32),
     }
@@ -392,6 +393,13 @@ class KernelTimer:
     def close(self):
         for e in self.evs:
             lib().mans_event_destroy(e)
+
+
+def lineage_spacing(n_units, n_out, L):
+    """Lineage spacing the rules give a single stream of this shape (0 = chunked)."""
+    v = C.c_uint32()
+    check(lib().mans_lineage_spacing(n_units, n_out, L, C.byref(v)))
+    return v.value
 
 
 def set_geometry_rules(rules):

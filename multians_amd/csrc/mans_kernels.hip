@@ -89,6 +89,11 @@ struct KArgs {
     // earlier kernel of the same sequence, so the choice needs no host round trip)
     const uint32_t* gate;
     uint32_t gate_val;
+    // optional (k_refsync after a lineage decode): the decode's 32-unit item records, used
+    // when its lineage failure word (lin_fail_word) is 0
+    const SyncPoint* lin_rec;
+    const uint32_t* lin_fail_word;
+    uint32_t lin_nq;
 };
 
 // True when this launch is switched off by its gate word (uniform over the grid).
@@ -1031,15 +1036,29 @@ __global__ __launch_bounds__(kWG) void k_refsync(KArgs a) {
     const Work& w = a.w;
     const uint32_t q = blockIdx.x * kWG + threadIdx.x;
     if (q < a.ref_nblocks && a.ref_flags) a.ref_flags[q] = (w.status[0] & 2u) ? 0u : 1u;
-    if (q >= sp.n_ck) return;
+    // intervals: the proven checkpoints (K units), or the 32-unit item records of a
+    // lineage decode whose records held
+    const bool lin = a.lin_rec &&
+        __hip_atomic_load(a.lin_fail_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+    const uint32_t n_int = lin ? a.lin_nq : sp.n_ck, K = lin ? kLinQ : sp.K;
+    if (q >= n_int) return;
     uint32_t entry, e2;
-    const uint64_t off = checkpoint_offset(sp, w, q, &entry);
-    uint64_t end = q + 1u < sp.n_ck ? checkpoint_offset(sp, w, q + 1u, &e2) : sp.n_out;
+    uint64_t off, end;
+    if (lin) {
+        const SyncPoint r = a.lin_rec[q];
+        entry = q == 0u ? pack_entry(sp.first_state - sp.L, 32u - sp.first_bit)
+                        : pack_entry(r.state, r.bit);
+        off = r.before;
+        end = q + 1u < n_int ? a.lin_rec[q + 1u].before : sp.n_out;
+    } else {
+        off = checkpoint_offset(sp, w, q, &entry);
+        end = q + 1u < n_int ? checkpoint_offset(sp, w, q + 1u, &e2) : sp.n_out;
+    }
     if (end > sp.n_out) end = sp.n_out;
     if (off >= end) return;
-    const bool last_interval = q + 1u == sp.n_ck;
+    const bool last_interval = q + 1u == n_int;
     const uint32_t S = a.ref_S, nsub = a.ref_nsub;
-    const uint64_t k0 = (uint64_t)q * sp.K, k1 = k0 + sp.K;      // interval units [k0, k1)
+    const uint64_t k0 = (uint64_t)q * K, k1 = k0 + K;            // interval units [k0, k1)
     GlobalTab tab{sp.table, sp.L};
     BCursor cur;
     bcursor_init(cur, sp.units, sp.n_units, (uint32_t)k0, entry_off(entry));
@@ -1603,12 +1622,13 @@ static RuleHit rule_geometry(uint64_t n_units, uint64_t n_out, uint32_t L, uint3
 }
 
 // Lineage spacing D (units) for a single-stream decode, from the first rule of scope
-// MANS_RULE_LINEAGE that matches (rules of other scopes never select it); 0 = none.
+// MANS_RULE_LINEAGE that matches (rules of other scopes never select it); 0 = none, or a
+// matching rule with chunk_units 0 (the chunked pipeline for that class of streams).
 static uint32_t rule_lineage(uint64_t n_units, uint64_t n_out, uint32_t L) {
     const double spu = n_units ? (double)n_out / (double)n_units : 4.0;
     auto match = [&](const mans_geometry_rule& r) {
-        return r.chunk_units && (r.scope & MANS_RULE_LINEAGE) && L <= r.L_max &&
-               spu <= r.spu_max && n_units <= r.units_max;
+        return r.scope == MANS_RULE_LINEAGE && L <= r.L_max && spu <= r.spu_max &&
+               n_units <= r.units_max;
     };
     uint32_t D = 0;
     {
@@ -2086,6 +2106,12 @@ static int launch_lineage(const mans_decode_args* d, const KArgs& a, uint32_t nc
     return MANS_OK;
 }
 
+extern "C" int mans_lineage_spacing(uint64_t n_units, uint64_t n_out, uint32_t L, uint32_t* D) {
+    if (!D) return set_error(MANS_EINVAL, "D is NULL");
+    *D = rule_lineage(n_units, n_out, L);
+    return MANS_OK;
+}
+
 extern "C" int mans_set_lineage_cap(uint32_t checkpoints) {
     g_lin_cap.store(checkpoints ? checkpoints : 2048u);
     return MANS_OK;
@@ -2208,7 +2234,17 @@ extern "C" int mans_ref_sync_fill(const mans_decode_args* d) {
     uint32_t nct, nit;
     int rc = single_plan(d, &a, &nct, &nit);
     if (rc) return rc;
-    const uint32_t lanes = std::max(a.one.n_ck, a.ref_nblocks);
+    // a decode with these arguments (and no ref_sync) ran by lineages when the rules chose
+    // them; its item records hold unless it fell back (read on the device)
+    mans_decode_args dd = *d;
+    dd.ref_sync = nullptr;
+    if (lineage_choice(&dd)) {
+        const LinLayout l = lin_layout(d->n_units);
+        a.lin_rec = (const SyncPoint*)((const char*)d->workspace + l.o_rec);
+        a.lin_fail_word = (const uint32_t*)d->workspace + kLinCtl + 1u;
+        a.lin_nq = (uint32_t)l.n_q;
+    }
+    const uint32_t lanes = std::max({a.one.n_ck, a.ref_nblocks, a.lin_nq});
     hipLaunchKernelGGL(k_refsync, dim3((lanes + kWG - 1u) / kWG), dim3(kWG), 0,
                        (hipStream_t)d->stream, a);
     hipError_t e = hipGetLastError();
@@ -2747,14 +2783,17 @@ extern "C" int mans_set_geometry_rules(const mans_geometry_rule* rules, size_t n
     std::vector<mans_geometry_rule> v;
     for (size_t i = 0; i < n; ++i) {
         mans_geometry_rule r = rules[i];
-        if (r.chunk_units < kMinRuleChunk || r.chunk_units > (1u << 24))
-            return set_error(MANS_EINVAL, "rule %zu: chunk_units %u outside [%u, 2^24]", i,
-                             r.chunk_units, kMinRuleChunk);
-        if (r.scope == MANS_RULE_LINEAGE) {          // lineage spacing: a multiple of 32
+        if (r.scope == MANS_RULE_LINEAGE) {          // lineage spacing: a multiple of 32, or 0
+            if (r.chunk_units > (1u << 24))
+                return set_error(MANS_EINVAL, "rule %zu: lineage spacing %u above 2^24", i,
+                                 r.chunk_units);
             r.chunk_units = r.chunk_units / kLinQ * kLinQ;
             v.push_back(r);
             continue;
         }
+        if (r.chunk_units < kMinRuleChunk || r.chunk_units > (1u << 24))
+            return set_error(MANS_EINVAL, "rule %zu: chunk_units %u outside [%u, 2^24]", i,
+                             r.chunk_units, kMinRuleChunk);
         if (r.scope > (MANS_RULE_SINGLE | MANS_RULE_BATCH))
             return set_error(MANS_EINVAL, "rule %zu: scope %u", i, r.scope);
         uint32_t p = kMinRuleChunk;

@@ -158,6 +158,65 @@ def tune_single(log):
     return rules, rec
 
 
+LIN_D = [32, 64, 96, 128, 192, 256]
+
+
+def tune_lineage(single_rules, log, Ls=None):
+    """Lineage rules: for every (state count, bucket, size) of the single-stream sweep, the
+    chunked decode under the single-stream rules against lineage decodes at each spacing of
+    LIN_D (median of 5 each, every decode checked against its symbols and required not to
+    fall back).  A cell takes the fastest spacing when it beats the chunked decode by 3%,
+    else a rule with spacing 0 (the chunked pipeline), so the first-hit rules never give a
+    cell a spacing that lost there."""
+    import torch
+    from .decoder import GeneratedBatch, lineage_stats, workspace_bytes
+    rules, rec = [], []
+    for L in (Ls or SINGLE_L):
+        for b, lam in enumerate(BUCKET_LAMBDA):
+            found = []
+            for log2n in (SINGLE_LOG2N if L < SINGLE_L[-1] else SINGLE_LOG2N_LARGE_L):
+                gb = GeneratedBatch([(9000 + 97 * b + log2n + L, lam, 1 << log2n, L)])
+                nu, n = int(gb.n_units[0]), int(gb.n_symbols[0])
+                out = torch.empty(n, dtype=torch.uint8, device="cuda")
+                ws = torch.zeros(workspace_bytes(nu, L), dtype=torch.uint8, device="cuda")
+                ref = gb.symbols(0).flip(0)
+                times = {}
+                for D in [0] + LIN_D:
+                    lin = [(1 << 16, 1e30, 1 << 40, D, 0, _lib.RULE_LINEAGE)] if D else []
+                    _lib.set_geometry_rules(single_rules + lin)
+                    out.zero_()
+                    gb.decode_one(0, out, ws)
+                    torch.cuda.synchronize()
+                    if gb.status(ws) != 0 or not torch.equal(out, ref):
+                        raise RuntimeError("tuning decode mismatch (D=%d)" % D)
+                    if D and lineage_stats(ws)[0]:
+                        continue                    # fell back: not a lineage time
+                    times[D] = _time(lambda: gb.decode_one(0, out, ws), torch, reps=5)
+                lin_t = {D: t for D, t in times.items() if D}
+                best = min(lin_t, key=lin_t.get) if lin_t else 0
+                pick = best if best and lin_t[best] < 0.97 * times[0] else 0
+                log("lineage L=%d bucket %d (lambda %g), 2^%d symbols, %d units: chunked %.3f "
+                    "ms, best lineage D=%d %.3f ms -> %s" % (
+                        L, b, lam, log2n, nu, times[0], best, lin_t.get(best, float("nan")),
+                        ("D=%d" % pick) if pick else "chunked"))
+                found.append((nu, pick))
+                rec.append({"L": L, "bucket": b, "lambda": lam, "log2n": log2n, "units": nu,
+                            "chunked_ms": round(times[0], 4), "D": pick,
+                            "lineage_ms": {str(D): round(t, 4) for D, t in lin_t.items()}})
+                del gb, out, ws, ref
+                torch.cuda.empty_cache()
+            for i, (nu, D) in enumerate(found):
+                hi = int(np.sqrt(nu * found[i + 1][0])) if i + 1 < len(found) else 1 << 40
+                r = (L if L < SINGLE_L[-1] else 1 << 14, SINGLE_EDGES[b], hi, D, 0,
+                     _lib.RULE_LINEAGE)
+                if rules and rules[-1][0] == r[0] and rules[-1][1] == r[1] and rules[-1][3] == D:
+                    rules[-1] = r                   # same spacing as the smaller sizes
+                else:
+                    rules.append(r)
+    _lib.set_geometry_rules([])
+    return rules, rec
+
+
 def tune_mixed(n_streams, single_rules, log, sweeps=2):
     """Batch rules by coordinate descent on the mixed batch's decode time."""
     import torch
@@ -252,6 +311,8 @@ def main():
                     help="keep the single-stream rules of the existing JSON")
     ap.add_argument("--no-mixed", action="store_true",
                     help="keep the batch parameters of the existing JSON")
+    ap.add_argument("--no-lineage", action="store_true",
+                    help="keep the lineage rules of the existing JSON")
     args = ap.parse_args()
     t0 = time.time()
 
@@ -275,10 +336,19 @@ def main():
         ms = meta0["mixed_ms"]
     else:
         (floor, safe, w1, w2), ms = tune_mixed(args.streams, single, log)
-    rules = single + batch_rules(floor, safe, w1, w2)
+    if args.no_lineage:
+        meta0 = json.load(open(out_json))
+        lineage = [tuple(r[k] for k in ("L_max", "spu_max", "units_max", "chunk_units",
+                                        "warm_units", "scope"))
+                   for r in meta0["rules"] if r["scope"] == _lib.RULE_LINEAGE]
+        lineage_rec = meta0.get("lineage", [])
+    else:
+        lineage, lineage_rec = tune_lineage(single, log)
+    rules = single + lineage + batch_rules(floor, safe, w1, w2)
     meta = {"device": "MI355X (gfx950)", "floor": floor, "safe": safe, "warm1": w1,
             "warm2": w2, "spu_edges": SPU_EDGES, "single_edges": SINGLE_EDGES,
-            "mixed_ms": ms, "mixed_streams": args.streams, "single": single_rec}
+            "mixed_ms": ms, "mixed_streams": args.streams, "single": single_rec,
+            "lineage": lineage_rec}
     write_rules(rules, meta, out_json, args.inc)
     log("wrote %d rules in %.0fs" % (len(rules), time.time() - t0))
 

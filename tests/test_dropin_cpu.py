@@ -149,17 +149,23 @@ def test_tuner_rule_table_matches_builtin_choice():
                                   "scope")) for r in meta["rules"]]
     batch = [r for r in rules if r[5] == _lib.RULE_BATCH]
     assert batch == tune.batch_rules(meta["floor"], meta["safe"], meta["warm1"], meta["warm2"])
-    assert all(r[5] in (_lib.RULE_SINGLE, _lib.RULE_BATCH) for r in rules)
+    assert all(r[5] in (_lib.RULE_SINGLE, _lib.RULE_BATCH, _lib.RULE_LINEAGE) for r in rules)
+    # lineage rules: a spacing (a multiple of 32) or 0 (the chunked pipeline) per cell
+    assert all(r[3] % 32 == 0 for r in rules if r[5] == _lib.RULE_LINEAGE)
     rng = np.random.default_rng(4)
     shapes = [(int(np.exp(rng.uniform(0, np.log(1 << 32)))), float(np.exp(rng.uniform(0, 5))),
                int(rng.choice([256, 1024, 2048, 4096, 8192]))) for _ in range(3000)]
     builtin = [_lib.chunk_units(n, int(n * spu), L) for n, spu, L in shapes]
+    builtin_lin = [_lib.lineage_spacing(n, int(n * spu), L) for n, spu, L in shapes]
     try:
         _lib.set_geometry_rules(rules)
         tuned = [_lib.chunk_units(n, int(n * spu), L) for n, spu, L in shapes]
+        tuned_lin = [_lib.lineage_spacing(n, int(n * spu), L) for n, spu, L in shapes]
     finally:
         _lib.set_geometry_rules([])
     assert tuned == builtin
+    assert tuned_lin == builtin_lin
+    assert any(builtin_lin) and not all(builtin_lin)
 
 
 def test_histogram_matches_bincount():
