@@ -159,6 +159,12 @@ def tune_single(log):
 
 
 LIN_D = [32, 64, 96, 128, 192, 256]
+# lineage buckets (finer than the single-stream ones around 8 symbols per unit, where the
+# choice between lineages and chunks turns at 2^30 symbols) and their tuning streams'
+# lambda (symbols per unit at L = 4096 in brackets): 0.01 (4.3), 0.03 (5.0), 0.06 (5.8),
+# 0.1 (6.6), 0.15 (7.6), 0.2 (8.5), 0.25 (9.4), 0.5 (13), 1.0 (21), 1.6 (32), 3.0 (106)
+LIN_EDGES = [4.6, 5.2, 6.0, 7.0, 8.0, 9.0, 11.0, 16.0, 25.0, 50.0, 1e30]
+LIN_LAMBDA = [0.01, 0.03, 0.06, 0.1, 0.15, 0.2, 0.25, 0.5, 1.0, 1.6, 3.0]
 
 
 def tune_lineage(single_rules, log, Ls=None):
@@ -172,7 +178,7 @@ def tune_lineage(single_rules, log, Ls=None):
     from .decoder import GeneratedBatch, lineage_stats, workspace_bytes
     rules, rec = [], []
     for L in (Ls or SINGLE_L):
-        for b, lam in enumerate(BUCKET_LAMBDA):
+        for b, lam in enumerate(LIN_LAMBDA):
             found = []
             for log2n in (SINGLE_LOG2N if L < SINGLE_L[-1] else SINGLE_LOG2N_LARGE_L):
                 gb = GeneratedBatch([(9000 + 97 * b + log2n + L, lam, 1 << log2n, L)])
@@ -207,7 +213,7 @@ def tune_lineage(single_rules, log, Ls=None):
                 torch.cuda.empty_cache()
             for i, (nu, D) in enumerate(found):
                 hi = int(np.sqrt(nu * found[i + 1][0])) if i + 1 < len(found) else 1 << 40
-                r = (L if L < SINGLE_L[-1] else 1 << 14, SINGLE_EDGES[b], hi, D, 0,
+                r = (L if L < SINGLE_L[-1] else 1 << 14, LIN_EDGES[b], hi, D, 0,
                      _lib.RULE_LINEAGE)
                 if rules and rules[-1][0] == r[0] and rules[-1][1] == r[1] and rules[-1][3] == D:
                     rules[-1] = r                   # same spacing as the smaller sizes
