@@ -6,13 +6,17 @@ warm-up W (units each chunk's guessed-state decode starts early), per stream sha
 S trades lanes against failed walks: a short chunk gives a stream more lanes (lower
 latency) but more walks that cross their chunk without merging.  W trades work against
 walk length: a warm-up longer than the merge distance puts most chunks' guessed paths on
-the true path at the chunk start, so their walks merge at once.  Two rule sets are fitted
-on the device, both keyed by symbols-per-unit bucket (n_symbols / n_units, the entropy
+the true path at the chunk start, so their walks merge at once.  Three rule sets are fitted
+on the device, all keyed by symbols-per-unit bucket (n_symbols / n_units, the entropy
 proxy the decoder knows before decoding) and stream size:
 
   * single (mans_decode, one stream on the GPU: latency): for one stream per (bucket,
     size) -- symbols 2^20 .. 2^30, lambda chosen to land in each bucket -- every (S, W) of
     a grid is timed (median of 3 decodes) and the fastest kept;
+  * lineage (mans_decode, one stream): for one stream per (bucket, size) on finer buckets
+    of their own, the chunked decode under the single rules against lineage decodes at
+    spacings 32-256 units; each cell gets the fastest spacing when it wins by 3%, else
+    spacing 0 (the chunked pipeline);
   * batch (mans_decode_batch: throughput of the whole batch): the built-in shape
     S = clamp(pow2ceil(n_units / 256), floor_b, safe_b), with warm-ups W1_b for streams
     of one 256-chunk range and W2_b for longer ones, fitted by coordinate descent on the
