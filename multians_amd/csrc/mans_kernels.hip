@@ -1423,14 +1423,13 @@ __global__ __launch_bounds__(kWG) void k_lin(KArgs a, LinArgs la) {
     // the current one.  take() is called by every lane still in the loop (wave-uniform).
     uint32_t pool = 0u, pool_left = 0u;
     auto take = [&](bool want) -> uint32_t {
-        const uint64_t all = __ballot(true);
         const uint64_t m = __ballot(want);
         if (!m) return kNone;
         const uint32_t n = (uint32_t)__popcll(m);
         const uint32_t r = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
         uint32_t b = pool;
         if (n > pool_left) {
-            const uint32_t lead = (uint32_t)__builtin_ctzll(all);
+            const uint32_t lead = (uint32_t)__builtin_ctzll(__ballot(true));
             uint32_t nb = 0u;
             if (lane == lead) nb = atomicAdd(&lw.ctl[0], 64u);
             b = __shfl(nb, (int)lead, 64);
@@ -1451,7 +1450,8 @@ __global__ __launch_bounds__(kWG) void k_lin(KArgs a, LinArgs la) {
     Prefetch pf, pn;
     uint32_t nt = take(true);
     if (nt != kNone) lane_load(sp.units, sp.n_units, first_unit(nt), pn);
-    uint32_t cv[kLinSlots], cu = 0u;                  // checkpoint slots read ahead
+    constexpr uint32_t kAhead = 8u;                  // slots read ahead (the most used: 7-9)
+    uint32_t cv[kAhead], cu = 0u;
     for (;;) {
         // lanes without a lineage start the held one (its first round is in pn) and take
         // the next ticket; a lane with neither leaves
@@ -1494,7 +1494,7 @@ __global__ __launch_bounds__(kWG) void k_lin(KArgs a, LinArgs la) {
             const uint64_t base = (uint64_t)(q + 1u) * kLinSlots;
             cu = lin_slots_used(lin_ld(&lw.cnt[q + 1u]));
 #pragma unroll
-            for (uint32_t s = 0; s < kLinSlots; ++s) cv[s] = lin_ld(&lw.val[base + s]);
+            for (uint32_t s = 0; s < kAhead; ++s) cv[s] = lin_ld(&lw.val[base + s]);
         }
         round_decode(tab, seg, L, 0u);
         L.rbase += kR;
@@ -1513,8 +1513,12 @@ __global__ __launch_bounds__(kWG) void k_lin(KArgs a, LinArgs la) {
         const uint64_t prev = (uint64_t)(q - 1u) * kLinSlots + slot;
         uint32_t found = kLinEmpty;
 #pragma unroll
-        for (uint32_t s = kLinSlots; s-- > 0;)
+        for (uint32_t s = kAhead; s-- > 0;)
             if (s < cu && cv[s] == v) found = s;
+        if (found == kLinEmpty && cu > kAhead) {              // rare: more slots in use
+            for (uint32_t s = kAhead; s < cu; ++s)
+                if (lin_ld(&lw.val[base + s]) == v) { found = s; break; }
+        }
         if (found != kLinEmpty) {                             // merged
             if (slot != kLinEmpty) lw.inf[prev] = lin_link(found, steps);
             L.active = 0u;
