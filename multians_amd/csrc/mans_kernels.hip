@@ -532,6 +532,13 @@ __device__ void walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& ta
         round_store(io, pf);
         __syncthreads();
         if (r + 1u < nr_max) round_load(io, r + 1u, pf);
+        // the spec checkpoint at this round's end, read before the round decodes (the
+        // range's spec records are complete before its walks start)
+        uint32_t ckv = kNone;
+        if (L.active) {
+            const uint32_t re = L.rbase + (r + 1u) * kR;
+            if (re < L.u_stop) ckv = ld_coherent(&w.ck[ck_slot(sp, re >> sp.log2K)]);
+        }
         round_decode(tab, io.buf + threadIdx.x * kLS, L, r);
         const bool bnd = ((r + 1u) & (r_first - 1u)) == 0u;   // chunk boundary round (uniform)
         uint32_t test = 0u, v = 0u, q = 0u;            // merge test at checkpoint q
@@ -565,7 +572,7 @@ __device__ void walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& ta
         if (bnd) __syncthreads();          // closing records before entering ones
         if (test) {
             if (bnd) w.wentry[gc] = v;
-            if (ld_coherent(&w.ck[ck_slot(sp, q)]) == v) {
+            if (ckv == v) {                        // ck[q], read before the round
                 w.mq[gc] = q;
                 w.wcm[gc] = L.cnt;
                 if (mrg && gc == gc0) mrg[org] = 1u;
