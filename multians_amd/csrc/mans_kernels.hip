@@ -1240,7 +1240,7 @@ __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint3
     // counter (ctick[0], zeroed by k_init), so slow passes (high symbols per unit) do
     // not pile up on a few workgroups.
     extern __shared__ uint32_t lds[];
-    __shared__ uint32_t s_stream, s_have, s_next, s_skip;
+    __shared__ uint32_t s_have, s_next, s_skip;
     __shared__ uint64_t s_off[kWG + 1];
     if (threadIdx.x == 0) {
         s_have = kNone;
@@ -1250,22 +1250,24 @@ __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint3
     uint32_t t = s_next;
     if (t >= n_passes) return;
     uint32_t t_end = min(n_passes, t + per_wg);
+    // The stream of pass t (sid) and the first pass of the stream after it (next_first),
+    // uniform: a pass usually follows the previous one in the same stream or the next,
+    // so the stream search runs only where a block of passes starts elsewhere.
+    auto first_pass = [&](uint32_t s) -> uint32_t {
+        return a.plans && s < a.n_streams
+                   ? __builtin_amdgcn_readfirstlane(a.plans[s].task_item) : 0xFFFFFFFFu;
+    };
+    uint32_t sid = __builtin_amdgcn_readfirstlane(stream_of_task<true>(a, t));
+    uint32_t next_first = first_pass(sid + 1u);
     // prefetch the first pass
-    if (threadIdx.x == 0) s_stream = stream_of_task<true>(a, t);
-    __syncthreads();
     WPrefetch pf;
     {
-        const StreamPlan sp = *plan_of(a, __builtin_amdgcn_readfirstlane(s_stream));
-        pass_load(sp.units, sp.n_units, (t - sp.task_item) * kWG * kWI, pf);
+        const StreamPlan* sp0 = plan_of(a, sid);
+        pass_load(sp0->units, sp0->n_units, (t - sp0->task_item) * kWG * kWI, pf);
     }
     while (t < n_passes) {
         __syncthreads();
-        if (threadIdx.x == 0) {
-            s_stream = stream_of_task<true>(a, t);
-            if (t + 1u == t_end) s_next = atomicAdd(&a.w.ctick[0], per_wg);
-        }
-        __syncthreads();
-        const uint32_t sid = __builtin_amdgcn_readfirstlane(s_stream);
+        if (threadIdx.x == 0 && t + 1u == t_end) s_next = atomicAdd(&a.w.ctick[0], per_wg);
         const StreamPlan sp = *plan_of(a, sid);
         uint32_t* buf = lds + table_words(sp.L);
         if (s_have != sid) {
@@ -1305,11 +1307,20 @@ __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint3
         }
         __syncthreads();
         const uint32_t tn = t + 1u < t_end ? t + 1u : __builtin_amdgcn_readfirstlane(s_next);
+        uint32_t nsid = sid, nnext = next_first;
         if (tn < n_passes) {
-            // the next pass may belong to another stream
-            const uint32_t ns = stream_of_task<true>(a, tn);
-            const StreamPlan np = *plan_of(a, __builtin_amdgcn_readfirstlane(ns));
-            pass_load(np.units, np.n_units, (tn - np.task_item) * kWG * kWI, pf);
+            if (tn >= next_first) {                // the next pass belongs to another stream
+                const uint32_t f2 = tn == t + 1u ? first_pass(sid + 2u) : 0u;
+                if (tn < f2) {
+                    nsid = sid + 1u;
+                    nnext = f2;
+                } else {
+                    nsid = __builtin_amdgcn_readfirstlane(stream_of_task<true>(a, tn));
+                    nnext = first_pass(nsid + 1u);
+                }
+            }
+            const StreamPlan* np = plan_of(a, nsid);
+            pass_load(np->units, np->n_units, (tn - np->task_item) * kWG * kWI, pf);
         }
         if (threadIdx.x < nitems && !s_skip) {
             uint64_t off = s_off[threadIdx.x], end = s_off[threadIdx.x + 1];
@@ -1321,6 +1332,8 @@ __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint3
         }
         if (t + 1u == t_end) t_end = min(n_passes, tn + per_wg);
         t = tn;
+        sid = nsid;
+        next_first = nnext;
     }
 }
 
