@@ -149,10 +149,21 @@ __device__ uint32_t stage_table(const StreamPlan& sp, uint32_t* lds, uint32_t* s
     int bad = 0, cyc = 0;
     // tb + base < 2^16: tb <= 160 KiB / 4 and base < L <= 16384
     const uint32_t tb = lds_addr(lds) >> 2;
-    for (uint32_t i = threadIdx.x; i < sp.L; i += blockDim.x) {
-        uint32_t e = load_unit(sp.table, sp.L, i);
-        const uint32_t d = derive_entry(e, sp.L, i, &bad, &cyc);
-        lds[i] = (count_only ? (d & ~0xFF00u) : d) + (tb << 16);
+    // kG entries per thread per group, all loads of a group issued before any is used
+    // (one memory latency per group instead of one per entry)
+    constexpr uint32_t kG = 16;
+    for (uint32_t i0 = threadIdx.x; i0 < sp.L; i0 += blockDim.x * kG) {
+        uint32_t e[kG];
+#pragma unroll
+        for (uint32_t k = 0; k < kG; ++k) e[k] = load_unit(sp.table, sp.L, i0 + k * blockDim.x);
+#pragma unroll
+        for (uint32_t k = 0; k < kG; ++k) {
+            const uint32_t i = i0 + k * blockDim.x;
+            if (i < sp.L) {
+                const uint32_t d = derive_entry(e[k], sp.L, i, &bad, &cyc);
+                lds[i] = (count_only ? (d & ~0xFF00u) : d) + (tb << 16);
+            }
+        }
     }
     if (bad) {
         atomicOr(&status[0], 1u);
@@ -1259,6 +1270,9 @@ __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint3
     };
     uint32_t sid = __builtin_amdgcn_readfirstlane(stream_of_task<true>(a, t));
     uint32_t next_first = first_pass(sid + 1u);
+    // thread 0: the item that ended the last pass (stream, index, offset, entry)
+    uint32_t carry_sid = kNone, carry_it = 0u, carry_entry = 0u;
+    uint64_t carry_off = 0u;
     // prefetch the first pass
     WPrefetch pf;
     {
@@ -1285,7 +1299,10 @@ __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint3
         const uint32_t n_items = n_write_items(sp.n_ck);
         if (threadIdx.x < nitems) {
             const uint32_t it = i0 + threadIdx.x;
-            if (a.direct) {                     // trusted records, one per item
+            if (carry_sid == sid && carry_it == it) {   // thread 0: the last pass's end
+                s_off[threadIdx.x] = carry_off;
+                entry = carry_entry;
+            } else if (a.direct) {              // trusted records, one per item
                 s_off[threadIdx.x] = item_record(sp, it, &entry);
             } else {
                 s_off[threadIdx.x] = checkpoint_offset(sp, a.w, it * kWF, &entry);
@@ -1299,11 +1316,18 @@ __global__ __launch_bounds__(kWG) void k_write(KArgs a, uint32_t n_passes, uint3
             }
         }
         if (threadIdx.x == 0) {
-            uint32_t e2;
+            // the pass's end: the next item's offset, kept (with its entry) for the next
+            // pass when that pass starts at this item
+            uint32_t e2 = 0;
             const uint32_t ie = i0 + nitems;
-            s_off[nitems] = ie >= n_items ? sp.n_out
-                          : a.direct ? item_record(sp, ie, &e2)
-                                     : checkpoint_offset(sp, a.w, ie * kWF, &e2);
+            const uint64_t oe = ie >= n_items ? sp.n_out
+                              : a.direct ? item_record(sp, ie, &e2)
+                                         : checkpoint_offset(sp, a.w, ie * kWF, &e2);
+            s_off[nitems] = oe;
+            carry_sid = ie < n_items ? sid : kNone;
+            carry_it = ie;
+            carry_off = oe;
+            carry_entry = e2;
         }
         __syncthreads();
         const uint32_t tn = t + 1u < t_end ? t + 1u : __builtin_amdgcn_readfirstlane(s_next);
