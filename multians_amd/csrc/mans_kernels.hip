@@ -293,6 +293,7 @@ struct RoundIO {
     uint32_t* rbase;      // LDS, kWG: first unit of round 0 per lane (kNone = no load)
     const uint32_t* units;
     uint32_t n;
+    uint32_t lim = kWG;   // lanes [lim, kWG) never decode: their segments are not staged
 };
 
 struct Prefetch { uint4 v[kPieces]; };
@@ -312,6 +313,7 @@ __device__ __forceinline__ void round_load(const RoundIO& io, uint32_t r, Prefet
     for (uint32_t j = 0; j < kPieces; ++j) {
         uint32_t lane, piece;
         piece_of(j, &lane, &piece);
+        if (lane >= io.lim) continue;
         // lanes without a load (rbase kNone) read an in-stream block they never use;
         // no branch and no mask, so the loads stay in flight while the round decodes
         const uint32_t rb = io.rbase[lane];
@@ -325,6 +327,7 @@ __device__ __forceinline__ void round_store(const RoundIO& io, const Prefetch& p
     for (uint32_t j = 0; j < kPieces; ++j) {
         uint32_t lane, piece;
         piece_of(j, &lane, &piece);
+        if (lane >= io.lim) continue;
         uint2* d = reinterpret_cast<uint2*>(io.buf + lane * kLS + 4u * piece);
         if (4u * piece < kLS) d[0] = make_uint2(pf.v[j].x, pf.v[j].y);
         if (4u * piece + 2u < kLS) d[1] = make_uint2(pf.v[j].z, pf.v[j].w);
@@ -696,6 +699,7 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
     }
     const bool valid = c < sp.n_chunks;
     const uint32_t c_end = min(c0 + (uint32_t)kWG, sp.n_chunks);
+    io.lim = c_end - c0;                       // lanes with a chunk of this range
     // ---- spec (a chunk c >= 1 starts its guessed decode sp.warm units early; its first
     // checkpoint is the state at the round boundary u0, and its counts start there)
     RLane L;
