@@ -94,7 +94,13 @@ struct KArgs {
     const SyncPoint* lin_rec;
     const uint32_t* lin_fail_word;
     uint32_t lin_nq;
+    // chunks per sync task (the sync kernels' workgroup size): 0 = kWG
+    uint32_t range;
 };
+
+__host__ __device__ __forceinline__ uint32_t range_of(const KArgs& a) {
+    return a.range ? a.range : (uint32_t)kWG;
+}
 
 // True when this launch is switched off by its gate word (uniform over the grid).
 __device__ __forceinline__ bool gated_off(const KArgs& a) {
@@ -180,6 +186,23 @@ __device__ uint32_t stage_table(const StreamPlan& sp, uint32_t* lds, uint32_t* s
     return __builtin_amdgcn_readfirstlane(f);
 }
 
+// Round geometry (see the round engine below).
+#ifndef MANS_ROUND_UNITS
+#define MANS_ROUND_UNITS 16
+#endif
+constexpr uint32_t kR = MANS_ROUND_UNITS;  // units decoded per round (= checkpoint stride)
+constexpr uint32_t kRL = kR + 4;   // units staged per lane per round (kR + lookahead)
+// LDS stride per lane in units: the round plus the two units its last window can reach
+// (p stays below (kR + 1) * 32 after the last pair, see round_decode); the rest of the
+// last 16-byte load is not stored.  18 is 2 mod 4: lanes at the same offset of their segments
+// (lockstep rounds keep them close) spread over twice the banks a 16-byte multiple would
+// give, the segments take 8-byte halves (ds_write2_b64), and four sync workgroups fit a
+// CU's LDS.
+constexpr uint32_t kLS = kR + 2u;
+static_assert(kLS <= kRL && kLS % 4u == 2u, "round segment stride");
+constexpr uint32_t kPieces = kRL / 4;   // 16-byte loads per lane segment
+static_assert(kR % 4 == 0 && kR >= 8, "round length");
+
 // Zero-bit runs folded into the staged table, for the kernels that count steps without
 // emitting symbols (k_sync, k_fixup).  A zero-bit step reads no bits, so from a state
 // whose step reads none the next states are fixed until one whose step reads bits: the
@@ -194,6 +217,7 @@ __device__ uint32_t stage_table(const StreamPlan& sp, uint32_t* lds, uint32_t* s
 // 4096 states.  The table must be staged count_only (symbol bytes zero).
 constexpr uint32_t kMaxRun = 255;
 
+template <int W>
 __device__ void fold_zero_bit_runs(uint32_t* lds, uint32_t L, uint32_t* scratch,
                                    uint32_t cap_words) {
     if (L > cap_words || L > 4096u) return;         // uniform
@@ -221,12 +245,14 @@ __device__ void fold_zero_bit_runs(uint32_t* lds, uint32_t L, uint32_t* scratch,
         if (!__syncthreads_or(changed)) break;
     }
     // the folded words (read all, barrier, write all: lds[] is read by other threads);
-    // L <= 4096 here, so each of the kWG threads holds at most kPer of them
-    constexpr uint32_t kPer = 4096u / kWG;
+    // L <= min(cap_words, 4096) here, cap_words <= W * kLS, so each of the W threads holds
+    // at most kPer of them
+    constexpr uint32_t kCap = (uint32_t)W * kLS < 4096u ? (uint32_t)W * kLS : 4096u;
+    constexpr uint32_t kPer = (kCap + W - 1u) / W;
     uint32_t nw[kPer];
 #pragma unroll
     for (uint32_t k = 0; k < kPer; ++k) {
-        const uint32_t i = threadIdx.x + k * kWG;
+        const uint32_t i = threadIdx.x + k * W;
         nw[k] = 0u;
         if (i < L) {
             const uint32_t p = scratch[i];
@@ -237,7 +263,7 @@ __device__ void fold_zero_bit_runs(uint32_t* lds, uint32_t L, uint32_t* scratch,
     __syncthreads();
 #pragma unroll
     for (uint32_t k = 0; k < kPer; ++k) {
-        const uint32_t i = threadIdx.x + k * kWG;
+        const uint32_t i = threadIdx.x + k * W;
         if (i < L) lds[i] = nw[k];
     }
     __syncthreads();
@@ -259,7 +285,7 @@ __global__ void k_init(KArgs a, uint32_t n_status) {
     }
     if (i < a.n_tasks) {
         const StreamPlan& sp = *plan_of(a, stream_of_task<false>(a, i));
-        const uint32_t c0 = (i - sp.task_chunk) * kWG;
+        const uint32_t c0 = (i - sp.task_chunk) * range_of(a);
         if (c0 > 0u && c0 < sp.n_chunks) w.exitv[sp.chunk_base + c0 - 1u] = kNone;
     }
 }
@@ -272,47 +298,35 @@ __global__ void k_init(KArgs a, uint32_t n_status) {
 // round r is decoded and written to LDS after it, so global-memory latency overlaps
 // a whole round of decoding.  A round is exactly one checkpoint interval (kR = K),
 // so checkpoints and merge tests happen between rounds, outside the step loop.
-#ifndef MANS_ROUND_UNITS
-#define MANS_ROUND_UNITS 16
-#endif
-constexpr uint32_t kR = MANS_ROUND_UNITS;  // units decoded per round (= checkpoint stride)
-constexpr uint32_t kRL = kR + 4;   // units staged per lane per round (kR + lookahead)
-// LDS stride per lane in units: the round plus the two units its last window can reach
-// (p stays below (kR + 1) * 32 after the last pair, see round_decode); the rest of the
-// last 16-byte load is not stored.  18 is 2 mod 4: lanes at the same offset of their segments
-// (lockstep rounds keep them close) spread over twice the banks a 16-byte multiple would
-// give, the segments take 8-byte halves (ds_write2_b64), and four sync workgroups fit a
-// CU's LDS.
-constexpr uint32_t kLS = kR + 2u;
-static_assert(kLS <= kRL && kLS % 4u == 2u, "round segment stride");
-constexpr uint32_t kPieces = kRL / 4;   // 16-byte loads per lane segment
-static_assert(kR % 4 == 0 && kR >= 8, "round length");
 
+// The round engine is instantiated per workgroup size W (lanes = threads).
 struct RoundIO {
-    uint32_t* buf;        // LDS, kWG * kLS units
-    uint32_t* rbase;      // LDS, kWG: first unit of round 0 per lane (kNone = no load)
+    uint32_t* buf;        // LDS, W * kLS units
+    uint32_t* rbase;      // LDS, W: first unit of round 0 per lane (kNone = no load)
     const uint32_t* units;
     uint32_t n;
-    uint32_t lim = kWG;   // lanes [lim, kWG) never decode: their segments are not staged
+    uint32_t lim;         // lanes [lim, W) never decode: their segments are not staged
 };
 
 struct Prefetch { uint4 v[kPieces]; };
 
 // Piece j of thread t: the (lane, piece) pairs of all lane segments are numbered
-// lane-major, kPieces per lane, and thread t takes numbers t, t + kWG, ...; consecutive
+// lane-major, kPieces per lane, and thread t takes numbers t, t + W, ...; consecutive
 // threads read consecutive 16-byte pieces of a segment.
+template <int W>
 __device__ __forceinline__ void piece_of(uint32_t j, uint32_t* lane, uint32_t* piece) {
-    const uint32_t idx = threadIdx.x + kWG * j;
+    const uint32_t idx = threadIdx.x + W * j;
     *lane = idx / kPieces;
     *piece = idx - *lane * kPieces;
 }
 
 // issue the loads of round r for all lanes
+template <int W>
 __device__ __forceinline__ void round_load(const RoundIO& io, uint32_t r, Prefetch& pf) {
 #pragma unroll
     for (uint32_t j = 0; j < kPieces; ++j) {
         uint32_t lane, piece;
-        piece_of(j, &lane, &piece);
+        piece_of<W>(j, &lane, &piece);
         if (lane >= io.lim) continue;
         // lanes without a load (rbase kNone) read an in-stream block they never use;
         // no branch and no mask, so the loads stay in flight while the round decodes
@@ -322,11 +336,12 @@ __device__ __forceinline__ void round_load(const RoundIO& io, uint32_t r, Prefet
     }
 }
 
+template <int W>
 __device__ __forceinline__ void round_store(const RoundIO& io, const Prefetch& pf) {
 #pragma unroll
     for (uint32_t j = 0; j < kPieces; ++j) {
         uint32_t lane, piece;
-        piece_of(j, &lane, &piece);
+        piece_of<W>(j, &lane, &piece);
         if (lane >= io.lim) continue;
         uint2* d = reinterpret_cast<uint2*>(io.buf + lane * kLS + 4u * piece);
         if (4u * piece < kLS) d[0] = make_uint2(pf.v[j].x, pf.v[j].y);
@@ -394,17 +409,17 @@ struct NoStop {
     __device__ void operator()(RLane&) const {}
 };
 
-template <typename F, typename G = NoStop>
+template <int W, typename F, typename G = NoStop>
 __device__ void run_rounds(const LdsTab& tab, RoundIO& io, RLane& L, uint32_t nr_max, F&& at_end,
                            G&& at_stop = G()) {
     __syncthreads();                       // rbase[] visible
     Prefetch pf;
-    round_load(io, 0u, pf);
+    round_load<W>(io, 0u, pf);
     for (uint32_t r = 0; r < nr_max; ++r) {
         __syncthreads();                   // previous round's reads of buf done
-        round_store(io, pf);
+        round_store<W>(io, pf);
         __syncthreads();
-        if (r + 1u < nr_max) round_load(io, r + 1u, pf);
+        if (r + 1u < nr_max) round_load<W>(io, r + 1u, pf);
         round_decode(tab, io.buf + threadIdx.x * kLS, L, r);
         if (L.active) {
             const uint32_t re = L.rbase + (r + 1u) * kR;
@@ -459,7 +474,7 @@ __device__ uint32_t walk_rounds(const StreamPlan& sp, const Work& w, const LdsTa
         }
     }
     io.rbase[threadIdx.x] = L.active ? L.rbase : kNone;
-    run_rounds(tab, io, L, (sp.S + kR - 1u) / kR, [&](RLane& l, uint32_t re) {
+    run_rounds<kWG>(tab, io, L, (sp.S + kR - 1u) / kR, [&](RLane& l, uint32_t re) {
         const uint32_t q = re >> sp.log2K;
         const uint32_t v = pack_entry(l.xi, l.o);
         if (ld_coherent(&w.ck[ck_slot(sp, q)]) == v) {
@@ -506,10 +521,11 @@ __device__ uint32_t walk_rounds(const StreamPlan& sp, const Work& w, const LdsTa
 // the chunk before it merged inside its own chunk: otherwise its start (the previous chunk's spec exit) is not on
 // the true path, and continuing would overwrite good records with a wrong path (the
 // previous lane's continuing walk covers those chunks instead).
+template <int W>
 __device__ void walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& tab,
                            RoundIO& io, uint32_t want, uint32_t c, uint32_t c_end,
                            uint32_t entry, uint32_t nr_max, uint32_t* mrg = nullptr) {
-    __shared__ uint32_t s_wany[kWG / 64], s_wcnt[kWG / 64];
+    __shared__ uint32_t s_wany[W / 64], s_wcnt[W / 64];
     uint32_t gc = sp.chunk_base + c;       // chunk the walk is in
     uint32_t gc0 = gc;                     // its first chunk
     uint32_t org = threadIdx.x;            // its lane before any compaction (mrg index)
@@ -540,12 +556,12 @@ __device__ void walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& ta
     const uint32_t wave = threadIdx.x >> 6;
     __syncthreads();                       // rbase[] visible
     Prefetch pf;
-    round_load(io, 0u, pf);
+    round_load<W>(io, 0u, pf);
     for (uint32_t r = 0; r < nr_max; ++r) {
         __syncthreads();                   // previous round's reads of buf done
-        round_store(io, pf);
+        round_store<W>(io, pf);
         __syncthreads();
-        if (r + 1u < nr_max) round_load(io, r + 1u, pf);
+        if (r + 1u < nr_max) round_load<W>(io, r + 1u, pf);
         // the spec checkpoint at this round's end, read before the round decodes (the
         // range's spec records are complete before its walks start)
         uint32_t ckv = kNone;
@@ -611,7 +627,7 @@ __device__ void walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& ta
         }
         __syncthreads();
         uint32_t waves_busy = 0u, before = 0u;
-        for (uint32_t k = 0; k < kWG / 64u; ++k) {
+        for (uint32_t k = 0; k < W / 64u; ++k) {
             waves_busy += s_wany[k];
             if (k < wave) before += s_wcnt[k];
         }
@@ -636,7 +652,7 @@ __device__ void walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& ta
         }
         io.rbase[threadIdx.x] = L.active ? L.rbase : kNone;
         __syncthreads();                   // new rbase[] visible, exchange reads done
-        round_load(io, r + 1u, pf);        // reload the next round for the new lanes
+        round_load<W>(io, r + 1u, pf);     // reload the next round for the new lanes
     }
 }
 
@@ -649,13 +665,15 @@ __device__ void walk_chain(const StreamPlan& sp, const Work& w, const LdsTab& ta
 // that range's workgroup as soon as its spec pass ends (k_init set it to kNone): the
 // workgroup waits for it, a bounded number of polls (a wait that runs out skips that
 // walk and marks the chunk's start kNone, so k_check lists it and k_fixup walks it).
-__global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
+// W: lanes per workgroup = chunks per range (the launch's range_of(a)).
+template <int W>
+__global__ __launch_bounds__(W) void k_sync(KArgs a) {
     if (gated_off(a)) return;
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_task, s_stream, s_entry;
-    __shared__ uint32_t s_rbase[kWG];
-    __shared__ uint32_t s_exit[kWG];
-    __shared__ uint32_t s_mrg[kWG];
+    __shared__ uint32_t s_rbase[W];
+    __shared__ uint32_t s_exit[W];
+    __shared__ uint32_t s_mrg[W];
     if (threadIdx.x == 0) {
         const uint32_t t = chunk_task(a, atomicAdd(&a.w.ctick[1], 1u));
         s_task = t;
@@ -666,11 +684,11 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
     const StreamPlan sp = *plan_of(a, __builtin_amdgcn_readfirstlane(s_stream));
     const uint32_t sf = stage_table(sp, lds, a.w.status, true);
     const uint32_t cyc = sf & 1u;
-    if (!sf) fold_zero_bit_runs(lds, sp.L, lds + table_words(sp.L), kWG * kLS);
+    if (!sf) fold_zero_bit_runs<W>(lds, sp.L, lds + table_words(sp.L), W * kLS);
     LdsTab tab{lds_addr(lds) >> 2};
-    RoundIO io{lds + table_words(sp.L), s_rbase, sp.units, sp.n_units};
+    RoundIO io{lds + table_words(sp.L), s_rbase, sp.units, sp.n_units, (uint32_t)W};
     const Work& w = a.w;
-    const uint32_t c0 = (t - sp.task_chunk) * kWG;
+    const uint32_t c0 = (t - sp.task_chunk) * W;
     const uint32_t c = c0 + threadIdx.x;
     if (sf & 2u) {
         // rejected table (uniform): empty, consistent records, so that the later
@@ -698,7 +716,7 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
         return;
     }
     const bool valid = c < sp.n_chunks;
-    const uint32_t c_end = min(c0 + (uint32_t)kWG, sp.n_chunks);
+    const uint32_t c_end = min(c0 + (uint32_t)W, sp.n_chunks);
     io.lim = c_end - c0;                       // lanes with a chunk of this range
     // ---- spec (a chunk c >= 1 starts its guessed decode sp.warm units early; its first
     // checkpoint is the state at the round boundary u0, and its counts start there)
@@ -730,7 +748,7 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
         }
     }
     s_rbase[threadIdx.x] = valid ? L.rbase : kNone;
-    run_rounds(tab, io, L, (sp.S + sp.warm + kR - 1u) / kR, [&](RLane& l, uint32_t re) {
+    run_rounds<W>(tab, io, L, (sp.S + sp.warm + kR - 1u) / kR, [&](RLane& l, uint32_t re) {
         if (re < u0) return;                       // still warming up
         if (re == u0) l.cnt = 0u;                  // chunk start: counts begin here
         w.ck[ck_slot(sp, re >> sp.log2K)] = pack_entry(l.xi, l.o);
@@ -771,7 +789,7 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
         want = 0u;
     }
     const uint32_t nr = ((c_end - c0) * sp.S + kR - 1u) / kR;
-    walk_chain(sp, w, tab, io, want, c, c_end, entry, nr, s_mrg);
+    walk_chain<W>(sp, w, tab, io, want, c, c_end, entry, nr, s_mrg);
     // ---- repair passes: a chunk whose recorded start is not its predecessor's exit (a
     // walk that stopped at its chunk's end although its start was right, found right by a
     // longer walk) is walked again from that exit.  The walks of a pass start together and
@@ -787,7 +805,7 @@ __global__ __launch_bounds__(kWG) void k_sync(KArgs a) {
             want2 = ld_coherent(&w.wentry[sp.chunk_base + c]) != entry2 ? 1u : 0u;
         }
         if (!__syncthreads_or(want2)) break;
-        walk_chain(sp, w, tab, io, want2, c, c_end, entry2, nr);
+        walk_chain<W>(sp, w, tab, io, want2, c, c_end, entry2, nr);
     }
 }
 
@@ -802,7 +820,7 @@ __global__ __launch_bounds__(kWG) void k_check(KArgs a) {
     const uint32_t sid = __builtin_amdgcn_readfirstlane(s_stream);
     const StreamPlan sp = *plan_of(a, sid);
     const Work& w = a.w;
-    const uint32_t c = (t - sp.task_chunk) * kWG + threadIdx.x;
+    const uint32_t c = (t - sp.task_chunk) * blockDim.x + threadIdx.x;   // range = blockDim.x
     if (c >= sp.n_chunks) return;
     const uint32_t g = sp.chunk_base + c;
     w.fflag[g] = 0u;
@@ -835,9 +853,9 @@ __global__ __launch_bounds__(kWG) void k_fixup(KArgs a) {
         if (s_n == 0u) continue;
         const StreamPlan sp = *plan_of(a, sid);
         if (stage_table(sp, lds, w.status, true)) continue;
-        fold_zero_bit_runs(lds, sp.L, lds + table_words(sp.L), kWG * kLS);
+        fold_zero_bit_runs<kWG>(lds, sp.L, lds + table_words(sp.L), kWG * kLS);
         LdsTab tab{lds_addr(lds) >> 2};
-        RoundIO io{lds + table_words(sp.L), s_rbase, sp.units, sp.n_units};
+        RoundIO io{lds + table_words(sp.L), s_rbase, sp.units, sp.n_units, (uint32_t)kWG};
         uint32_t* list = w.flist + sp.chunk_base;
         uint32_t* next = w.flist2 + sp.chunk_base;
         if (threadIdx.x == 0) {                                    // statistics
@@ -1029,7 +1047,7 @@ __global__ __launch_bounds__(kWG) void k_index(KArgs a) {
     __syncthreads();
     const StreamPlan sp = *plan_of(a, __builtin_amdgcn_readfirstlane(s_stream));
     if (!sp.index_out) return;
-    const uint32_t c = (t - sp.task_chunk) * kWG + threadIdx.x;
+    const uint32_t c = (t - sp.task_chunk) * blockDim.x + threadIdx.x;   // range = blockDim.x
     if (c < sp.n_chunks) sp.index_out[c] = chunk_sync_point(sp, a.w, c);
 }
 
@@ -1454,7 +1472,7 @@ __global__ __launch_bounds__(kWG) void k_lin(KArgs a, LinArgs la) {
         if (threadIdx.x == 0) lin_fail(la, 16u);
         return;
     }
-    fold_zero_bit_runs(lds, sp.L, buf, kWG * kLS);
+    fold_zero_bit_runs<kWG>(lds, sp.L, buf, kWG * kLS);
     __syncthreads();
     const LdsTab tab{lds_addr(lds) >> 2};
     uint32_t* seg = buf + threadIdx.x * kLS;
@@ -1791,7 +1809,7 @@ struct Layout {
 
 static uint64_t align256(uint64_t v) { return (v + 255u) & ~255ull; }
 
-Layout make_layout(uint64_t chunks, uint64_t cks, uint64_t n_plans) {
+Layout make_layout(uint64_t chunks, uint64_t cks, uint64_t n_plans, uint32_t R = kWG) {
     Layout l;
     l.chunks = chunks;
     l.cks = cks;
@@ -1818,8 +1836,8 @@ Layout make_layout(uint64_t chunks, uint64_t cks, uint64_t n_plans) {
     l.o_tick = take(4 * 8);
     l.o_bsum = take(8 * (chunks / 2048 + 2));
     l.o_fcnt = take(4 * (n_plans + 1));
-    // plans, then the chunk-task order (at most chunks/256 + n_plans tasks)
-    l.o_plans = take(sizeof(StreamPlan) * n_plans + 4 * (chunks / 256 + 2 * n_plans + 1));
+    // plans, then the chunk-task order (at most chunks/R + n_plans tasks, R chunks a task)
+    l.o_plans = take(sizeof(StreamPlan) * n_plans + 4 * (chunks / R + 2 * n_plans + 1));
     l.bytes = o;
     return l;
 }
@@ -1885,6 +1903,8 @@ static int launch_pipeline(KArgs a, uint32_t n_chunk_tasks, uint32_t n_item_task
                            void** ev = nullptr) {
     size_t lds = (size_t)table_words(maxL) * 4u;
     size_t lds_s = lds + (size_t)kWG * kLS * 4u;   // table + round buffer
+    const uint32_t R = range_of(a);                // chunks per sync task = k_sync's lanes
+    const size_t lds_r = lds + (size_t)R * kLS * 4u;
     auto mark = [&](int i) { if (ev) (void)hipEventRecord((hipEvent_t)ev[i], st); };
     a.wait_polls = g_wait_polls.load();
     a.round_limit = g_round_limit.load();
@@ -1905,9 +1925,14 @@ static int launch_pipeline(KArgs a, uint32_t n_chunk_tasks, uint32_t n_item_task
         if (e != hipSuccess) return set_error(MANS_EHIP, "kernel launch: %s", hipGetErrorString(e));
         return MANS_OK;
     }
-    hipLaunchKernelGGL(k_sync, dim3(n_chunk_tasks), dim3(kWG), lds_s, st, a);
+    if (R == 64u)
+        hipLaunchKernelGGL(k_sync<64>, dim3(n_chunk_tasks), dim3(64), lds_r, st, a);
+    else if (R == 128u)
+        hipLaunchKernelGGL(k_sync<128>, dim3(n_chunk_tasks), dim3(128), lds_r, st, a);
+    else
+        hipLaunchKernelGGL(k_sync<kWG>, dim3(n_chunk_tasks), dim3(kWG), lds_s, st, a);
     mark(2);
-    hipLaunchKernelGGL(k_check, dim3(n_chunk_tasks), dim3(kWG), 0, st, a);
+    hipLaunchKernelGGL(k_check, dim3(n_chunk_tasks), dim3(R), 0, st, a);
     mark(3);
     hipLaunchKernelGGL(k_fixup, dim3(a.n_streams), dim3(kWG), lds_s, st, a);
     mark(4);
@@ -1915,7 +1940,7 @@ static int launch_pipeline(KArgs a, uint32_t n_chunk_tasks, uint32_t n_item_task
         const uint32_t nb = (total_chunks + kScanBlock - 1) / kScanBlock;
         hipLaunchKernelGGL(k_scan, dim3(nb), dim3(kWG), 0, st, a, total_chunks, a.w.bsum);
     }
-    if (a.any_index_out) hipLaunchKernelGGL(k_index, dim3(n_chunk_tasks), dim3(kWG), 0, st, a);
+    if (a.any_index_out) hipLaunchKernelGGL(k_index, dim3(n_chunk_tasks), dim3(R), 0, st, a);
     if (a.ref_sync) {
         const uint32_t lanes = std::max(a.one.n_ck, a.ref_nblocks);
         hipLaunchKernelGGL(k_refsync, dim3((lanes + kWG - 1u) / kWG), dim3(kWG), 0, st, a);
@@ -2322,10 +2347,11 @@ extern "C" int mans_decode_status(void* workspace, void* stream, int* status) {
 namespace mans {
 
 // Plans for a batch: per-stream geometry, global chunk/checkpoint bases, task bases.
+// R: chunks per sync task (range) of this pipeline.
 static int build_plans(const mans_stream_desc* d, size_t n, std::vector<StreamPlan>& plans,
                        uint64_t* chunks, uint64_t* cks, uint32_t* chunk_tasks,
                        uint32_t* item_tasks, uint32_t* maxL, uint32_t* maxK, uint64_t* units,
-                       uint64_t* syms) {
+                       uint64_t* syms, uint32_t R = kWG) {
     plans.resize(n);
     uint64_t cb = 0, qb = 0, tc = 0, ti = 0, tu = 0, ts = 0;
     uint32_t ml = 2, mk = 1;
@@ -2368,7 +2394,7 @@ static int build_plans(const mans_stream_desc* d, size_t n, std::vector<StreamPl
         if (!d[i].n_out) sp.n_chunks = 0;
         cb += std::max<uint32_t>(sp.n_chunks, 1u);
         qb += std::max<uint64_t>(ck_slots(sp.n_chunks, S, K), 1u);
-        tc += std::max<uint64_t>((sp.n_chunks + kWG - 1) / kWG, 1);
+        tc += std::max<uint64_t>((sp.n_chunks + R - 1) / R, 1);
         ti += std::max<uint64_t>((n_write_items(sp.n_ck) + kWG - 1) / kWG, 1);
         tu += d[i].n_units;
         ts += d[i].n_out;
@@ -2443,10 +2469,20 @@ struct BatchPart {
     std::vector<StreamPlan> plans;
     uint64_t chunks = 0, cks = 0, units = 0, syms = 0, off = 0;
     uint32_t ct = 0, it = 0, ml = 0, mk = 0;
+    uint32_t R = kWG;                    // chunks per sync task (k_sync's workgroup size)
     uint32_t any_index_out = 0;
     uint32_t direct = 0;
     Layout l;
 };
+
+// Streams of at most kSmallRange chunks sync in workgroups of kSmallRange lanes (one
+// range each): a workgroup then holds an LDS table and kSmallRange round segments
+// instead of kWG, so more such streams sync at once on a CU.  0 = off.
+#ifndef MANS_SMALL_RANGE
+#define MANS_SMALL_RANGE 64
+#endif
+constexpr uint32_t kSmallRange = MANS_SMALL_RANGE;
+constexpr size_t kSmallPartMin = 256;     // fewer small streams: no part of their own
 
 static int plan_parts(const mans_stream_desc* d, size_t n, size_t k, std::vector<BatchPart>& parts,
                       uint64_t* bytes) {
@@ -2455,7 +2491,27 @@ static int plan_parts(const mans_stream_desc* d, size_t n, size_t k, std::vector
         parts[0].descs.assign(d, d + n);
         for (size_t i = 0; i < n; ++i) parts[0].orig.push_back(i);
     } else {
-        // longest processing time first: each stream to the part with the fewest symbols
+        // the small streams (see kSmallRange), when there are enough of them, form part 0
+        std::vector<char> small(n, 0);
+        size_t n_small = 0;
+        if (kSmallRange) {
+            for (size_t i = 0; i < n; ++i) {
+                uint32_t S = 0, K = 0, W = 0;
+                if (!d[i].n_out || stream_geometry(d[i].n_units, d[i].n_out, d[i].L,
+                                                   d[i].chunk_units, 0, d[i].index_in,
+                                                   d[i].index_out, MANS_RULE_BATCH, &S, &K, &W))
+                    continue;
+                if ((d[i].n_units + S - 1u) / S <= kSmallRange) {
+                    small[i] = 1;
+                    ++n_small;
+                }
+            }
+        }
+        const size_t p0 = n_small >= kSmallPartMin && n - n_small >= kSmallPartMin ? 1 : 0;
+        if (!p0) std::fill(small.begin(), small.end(), 0);
+        else parts[0].R = kSmallRange;
+        // longest processing time first: each other stream to the part with the fewest
+        // symbols
         std::vector<size_t> idx(n);
         for (size_t i = 0; i < n; ++i) idx[i] = i;
         std::stable_sort(idx.begin(), idx.end(),
@@ -2463,8 +2519,12 @@ static int plan_parts(const mans_stream_desc* d, size_t n, size_t k, std::vector
         std::vector<uint64_t> load(k, 0);
         std::vector<std::vector<size_t>> mine(k);
         for (size_t i : idx) {
-            size_t p = 0;
-            for (size_t q = 1; q < k; ++q)
+            if (small[i]) {
+                mine[0].push_back(i);
+                continue;
+            }
+            size_t p = p0;
+            for (size_t q = p0 + 1; q < k; ++q)
                 if (load[q] < load[p]) p = q;
             load[p] += d[i].n_out + 1u;
             mine[p].push_back(i);
@@ -2480,11 +2540,11 @@ static int plan_parts(const mans_stream_desc* d, size_t n, size_t k, std::vector
     uint64_t o = 0;
     for (BatchPart& bp : parts) {
         int rc = build_plans(bp.descs.data(), bp.descs.size(), bp.plans, &bp.chunks, &bp.cks,
-                             &bp.ct, &bp.it, &bp.ml, &bp.mk, &bp.units, &bp.syms);
+                             &bp.ct, &bp.it, &bp.ml, &bp.mk, &bp.units, &bp.syms, bp.R);
         if (rc) return rc;
         for (const mans_stream_desc& x : bp.descs)
             if (x.index_out) bp.any_index_out = 1u;
-        bp.l = make_layout(bp.chunks, bp.cks, bp.descs.size());
+        bp.l = make_layout(bp.chunks, bp.cks, bp.descs.size(), bp.R);
         bp.off = o;
         o = align256(o + bp.l.bytes);
     }
@@ -2594,6 +2654,7 @@ static KArgs part_args(const BatchPart& bp, void* workspace) {
     a.n_tasks = bp.ct;
     a.any_index_out = bp.any_index_out;
     a.direct = bp.direct;
+    a.range = bp.R;
     a.w = make_work(base, bp.l);
     return a;
 }
