@@ -154,3 +154,35 @@ def test_parts_graph_replays_match_eager():
         assert b.status() == 0
         assert torch.equal(b.out, eager), "replay %d" % rep
     g.close()
+
+
+def test_batch_with_small_stream_part():
+    """A split batch with at least 256 streams of at most 64 sync chunks decodes those in
+    their own part, whose sync kernel runs 64-lane workgroups; every stream bit-exact."""
+    import torch
+    from multians_amd.decoder import DeviceBatch, encode_symbols
+    print("case batch small-stream part: 260 small + 260 larger streams", flush=True)
+    rng = np.random.default_rng(11)
+    items = []
+    for i in range(520):
+        if i % 2 == 0:     # small: a few thousand units, a handful of chunks
+            n = int(rng.integers(4096, 16384))
+            lam = float(np.exp(rng.uniform(np.log(0.005), np.log(2.0))))
+        else:              # larger: 256 Ki symbols of high entropy, 256 chunks or more
+            n = 1 << 18
+            lam = float(rng.uniform(0.005, 0.05))
+        sym = ol.gen_symbols(11_000 + i, lam, n)
+        items.append((encode_symbols(sym, 4096), sym))
+    b = DeviceBatch([es for es, _ in items])
+    b.decode()
+    torch.cuda.synchronize()
+    assert b.status() == 0
+    for i, (es, sym) in enumerate(items):
+        assert np.array_equal(b.output(i).cpu().numpy()[::-1], sym), "stream %d" % i
+    # the same streams as one pipeline (256-lane sync workgroups throughout)
+    first = b.out.clone()
+    b.out.zero_()
+    b.decode(one_pipeline=True)
+    torch.cuda.synchronize()
+    assert b.status() == 0
+    assert torch.equal(first, b.out)
