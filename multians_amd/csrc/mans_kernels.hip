@@ -2366,6 +2366,9 @@ static int build_plans(const mans_stream_desc* d, size_t n, std::vector<StreamPl
                              d[i].index_in, d[i].index_out, MANS_RULE_BATCH, &S, &K, &W);
         if (rc) return set_error(rc, "stream %zu: %s", i, mans_last_error());
         if (K < 8u) K = 8u;
+        // a small-stream part (R < kWG): chunks lengthened until the stream is one range
+        if (R < (uint32_t)kWG && !d[i].index_in && !d[i].chunk_units)
+            while ((d[i].n_units + S - 1u) / S > R) S *= 2u;
         sp.units = d[i].units;
         sp.out = d[i].out;
         sp.table = d[i].table;
@@ -2475,9 +2478,10 @@ struct BatchPart {
     Layout l;
 };
 
-// Streams of at most kSmallRange chunks sync in workgroups of kSmallRange lanes (one
-// range each): a workgroup then holds an LDS table and kSmallRange round segments
-// instead of kWG, so more such streams sync at once on a CU.  0 = off.
+// Streams of at most 2 kSmallRange chunks (at their rule's chunk length) sync in
+// workgroups of kSmallRange lanes, one range each (with chunks twice as long where
+// needed): a workgroup then holds an LDS table and kSmallRange round segments instead of
+// kWG, so more such streams sync at once on a CU.  0 = off.
 #ifndef MANS_SMALL_RANGE
 #define MANS_SMALL_RANGE 64
 #endif
@@ -2501,7 +2505,11 @@ static int plan_parts(const mans_stream_desc* d, size_t n, size_t k, std::vector
                                                    d[i].chunk_units, 0, d[i].index_in,
                                                    d[i].index_out, MANS_RULE_BATCH, &S, &K, &W))
                     continue;
-                if ((d[i].n_units + S - 1u) / S <= kSmallRange) {
+                // at most twice as many chunks as a range: twice as long chunks make it one
+                // (streams whose chunk length is given keep it)
+                const uint32_t lim = d[i].index_in || d[i].chunk_units ? kSmallRange
+                                                                        : 2u * kSmallRange;
+                if ((d[i].n_units + S - 1u) / S <= lim) {
                     small[i] = 1;
                     ++n_small;
                 }
