@@ -31,6 +31,10 @@ namespace {
 
 constexpr int kWG = 256;           // lanes per workgroup (4 waves)
 constexpr uint32_t kRepairPasses = 4;   // in-range repair passes after the walks (k_sync)
+// Write passes a k_write workgroup takes per grab of the pass counter (blocks of
+// consecutive passes, usually of one stream; 2 measured 0.8% faster than 4 on the mixed
+// batch, 1 and 8 slower)
+constexpr uint32_t kWritePer = 2;
 
 // LDS words by byte address.  The staged table's words carry LDS dword indices (table
 // start + next-state base), so a decode step forms the address of the next table word
@@ -1916,7 +1920,7 @@ static int launch_pipeline(KArgs a, uint32_t n_chunk_tasks, uint32_t n_item_task
     mark(1);
     if (a.direct) {                         // trusted item records: the write pass alone
         for (int i = 2; i <= 5; ++i) mark(i);
-        const uint32_t per = 4u;
+        const uint32_t per = kWritePer;
         const uint32_t nwg = std::min<uint32_t>((n_item_tasks + per - 1) / per, 2048u);
         const size_t lds_w = lds + (size_t)kWG * kWL * 4u;
         hipLaunchKernelGGL(k_write, dim3(nwg), dim3(kWG), lds_w, st, a, n_item_tasks, per);
@@ -1947,8 +1951,9 @@ static int launch_pipeline(KArgs a, uint32_t n_chunk_tasks, uint32_t n_item_task
     }
     mark(5);
     {
-        // persistent write pass: workgroups take blocks of 4 passes from a device counter
-        const uint32_t per = 4u;
+        // persistent write pass: workgroups take blocks of kWritePer passes from a device
+        // counter
+        const uint32_t per = kWritePer;
         uint32_t nwg = std::min<uint32_t>((n_item_tasks + per - 1) / per, 2048u);
         (void)maxK;
         const size_t lds_w = lds + (size_t)kWG * kWL * 4u;
@@ -2172,7 +2177,7 @@ static int launch_lineage(const mans_decode_args* d, const KArgs& a, uint32_t nc
     {
         k.gate = la.lw.ctl + 1;
         k.gate_val = 0u;
-        const uint32_t per = 4u;
+        const uint32_t per = kWritePer;
         const uint32_t nwg = std::min<uint32_t>((nit + per - 1) / per, 2048u);
         const size_t lds_w = (size_t)table_words(a.one.L) * 4u + (size_t)kWG * kWL * 4u;
         hipLaunchKernelGGL(k_write, dim3(nwg), dim3(kWG), lds_w, st, k, nit, per);
