@@ -217,14 +217,66 @@ static_assert(kR % 4 == 0 && kR >= 8, "round length");
 // The runs come from pointer jumping in LDS over `scratch` (L words: next | z << 16;
 // each word is read and written whole, so jumps taken in any order stay consistent):
 // about log2(run length) passes.  Tables with zero-bit cycles (flag bit 0) or rejected
-// entries keep their plain words; so do tables larger than `cap_words` (the scratch) or
-// 4096 states.  The table must be staged count_only (symbol bytes zero).
+// entries keep their plain words; so do tables of more than 4096 states.  Tables larger
+// than `cap_words` (the scratch) fold in place instead (below).  The table must be staged
+// count_only (symbol bytes zero).
 constexpr uint32_t kMaxRun = 255;
+
+// The same folding in place, for workgroups whose round buffer cannot hold L words: the
+// zero-bit words become pointer words (bits 16-31: LDS index of the state reached, 8-15:
+// zero-bit steps z to it, nb 0) that jump along their runs; words whose step reads bits
+// are never written, and a zero-bit word is read whole in either form, so jumps taken in
+// any order stay consistent.  Then a word whose run ends on a state that reads bits takes
+// that state's word with z (1 + z steps per lookup, as above); a run stopped by kMaxRun
+// keeps its pointer with z - 1 (z - 1 zero-bit steps, then one more to the state it
+// points at).  Bit 7 marks the latter between two barriers.
+template <int W>
+__device__ void fold_zero_bit_runs_in_place(uint32_t* lds, uint32_t L) {
+    const uint32_t tb = lds_addr(lds) >> 2;
+    uint32_t any = 0u;
+    for (uint32_t i = threadIdx.x; i < L; i += W) {
+        const uint32_t w = lds[i];
+        if ((w & 31u) == 0u) {
+            lds[i] = (w & 0xFFFF0000u) | (1u << 8);
+            any = 1u;
+        }
+    }
+    if (!__syncthreads_or(any)) return;
+    for (;;) {
+        uint32_t changed = 0u;
+        for (uint32_t i = threadIdx.x; i < L; i += W) {
+            const uint32_t w = lds[i];
+            if (w & 31u) continue;                  // a state whose step reads bits
+            const uint32_t q = lds[(w >> 16) - tb];
+            if (q & 31u) continue;                  // resolved
+            const uint32_t z = ((w >> 8) & 0xFFu) + ((q >> 8) & 0xFFu);
+            if (z > kMaxRun) continue;              // at the run cap
+            lds[i] = (q & 0xFFFF0000u) | (z << 8);
+            changed = 1u;
+        }
+        if (!__syncthreads_or(changed)) break;
+    }
+    for (uint32_t i = threadIdx.x; i < L; i += W) {
+        const uint32_t w = lds[i];
+        if ((w & 31u) == 0u && (lds[(w >> 16) - tb] & 31u) == 0u) lds[i] = w | 0x80u;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < L; i += W) {
+        const uint32_t w = lds[i];
+        if (w & 31u) continue;
+        lds[i] = (w & 0x80u) ? (w & ~0x80u) - (1u << 8) : lds[(w >> 16) - tb] | (w & 0xFF00u);
+    }
+    __syncthreads();
+}
 
 template <int W>
 __device__ void fold_zero_bit_runs(uint32_t* lds, uint32_t L, uint32_t* scratch,
                                    uint32_t cap_words) {
-    if (L > cap_words || L > 4096u) return;         // uniform
+    if (L > 4096u) return;                          // uniform
+    if (L > cap_words) {
+        fold_zero_bit_runs_in_place<W>(lds, L);
+        return;
+    }
     const uint32_t tb = lds_addr(lds) >> 2;
     uint32_t any = 0u;
     for (uint32_t i = threadIdx.x; i < L; i += blockDim.x) {
